@@ -1,0 +1,866 @@
+// rf_engine.cpp -- host orchestration and the C ABI of librforest_hip.so.
+//
+// One process-wide context holds the prime set, the constant and twiddle
+// tables in device memory, a stream, a grow-only device arena and a pinned
+// staging buffer.  A call (one product or a batch) plans every product,
+// lays out its buffers in the arena, writes task tables, and enqueues
+//   k_fwd (per transform length) -> k_inv (per length) -> k_crt (per prime
+//   count) -> k_carry -> k_apply
+// on the stream.  Host mpz entry points add one H2D copy of the limbs and
+// one D2H copy of the results around that.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rforest_hip.h"
+#include "../../include/rforest_hip_stages.h"
+#include "rf_kernels.h"
+#include "rf_plan.h"
+
+extern "C" {
+int hw_disable_fft = 0;
+int mpzfft_threads = 1;
+}
+
+namespace {
+
+struct Ctx {
+  bool ready = false;
+  RfPrimes primes;
+  RfConsts* d_consts = nullptr;
+  double* d_twf = nullptr;
+  double* d_twi = nullptr;
+  RfDevTables tables{};
+  hipStream_t stream = nullptr;
+  char* d_arena = nullptr;
+  size_t arena_cap = 0;
+  char* h_stage = nullptr;
+  size_t stage_cap = 0;
+  // profiling
+  bool prof = false;
+  struct Ev {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> pending;
+  double prof_ms[RF_NSTAGES] = {0};
+  uint64_t prof_n[RF_NSTAGES] = {0};
+};
+
+Ctx g;
+std::mutex g_mu;
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      return fail(RF_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                          \
+  } while (0)
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int setup_locked() {
+  if (g.ready) return RF_OK;
+  rf_primes_init(&g.primes);
+  if (g.primes.k < RF_MAXP) return fail(RF_ERR_ARG, "prime generation found only %d primes", g.primes.k);
+  RfConsts* hc = (RfConsts*)calloc(1, sizeof(RfConsts));
+  std::vector<double> twf((size_t)RF_MAXP * RF_FWD_TW * 2), twi((size_t)RF_MAXP * RF_INV_TW * 2);
+  if (!hc) return fail(RF_ERR_NOMEM, "host allocation failed");
+  rf_consts_init(&g.primes, hc);
+  rf_twiddles_init(&g.primes, twf.data(), twi.data());
+  HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc((void**)&g.d_consts, sizeof(RfConsts)));
+  HIPCHK(hipMalloc((void**)&g.d_twf, twf.size() * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&g.d_twi, twi.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(g.d_consts, hc, sizeof(RfConsts), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g.d_twf, twf.data(), twf.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g.d_twi, twi.data(), twi.size() * sizeof(double), hipMemcpyHostToDevice));
+  free(hc);
+  g.tables.consts = g.d_consts;
+  g.tables.tw_fwd = g.d_twf;
+  g.tables.tw_inv = g.d_twi;
+  g.ready = true;
+  return RF_OK;
+}
+
+void clear_locked() {
+  if (!g.ready) return;
+  hipStreamSynchronize(g.stream);
+  for (auto& e : g.pending) {
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  g.pending.clear();
+  hipFree(g.d_consts);
+  hipFree(g.d_twf);
+  hipFree(g.d_twi);
+  if (g.d_arena) hipFree(g.d_arena);
+  if (g.h_stage) hipHostFree(g.h_stage);
+  hipStreamDestroy(g.stream);
+  g = Ctx();
+}
+
+int ensure_arena(size_t bytes) {
+  if (bytes <= g.arena_cap) return RF_OK;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (g.d_arena) HIPCHK(hipFree(g.d_arena));
+  g.d_arena = nullptr;
+  size_t cap = std::max(bytes, g.arena_cap * 3 / 2);
+  hipError_t e = hipMalloc((void**)&g.d_arena, cap);
+  if (e != hipSuccess) {
+    g.arena_cap = 0;
+    return fail(RF_ERR_NOMEM, "device arena of %zu bytes: %s", cap, hipGetErrorString(e));
+  }
+  g.arena_cap = cap;
+  return RF_OK;
+}
+
+int ensure_stage(size_t bytes) {
+  if (bytes <= g.stage_cap) return RF_OK;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (g.h_stage) HIPCHK(hipHostFree(g.h_stage));
+  g.h_stage = nullptr;
+  size_t cap = std::max(bytes, g.stage_cap * 3 / 2);
+  hipError_t e = hipHostMalloc((void**)&g.h_stage, cap, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    g.stage_cap = 0;
+    return fail(RF_ERR_NOMEM, "pinned staging of %zu bytes: %s", cap, hipGetErrorString(e));
+  }
+  g.stage_cap = cap;
+  return RF_OK;
+}
+
+// ---- profiling around launches ----
+struct ProfScope {
+  int stage;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(int s, hipStream_t stream) : stage(s), st(stream) {
+    if (!g.prof) return;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, st);
+  }
+  ~ProfScope() {
+    if (!g.prof) return;
+    hipEventRecord(b, st);
+    g.pending.push_back({stage, a, b});
+  }
+};
+
+void prof_collect() {
+  for (auto& e : g.pending) {
+    hipEventSynchronize(e.b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e.a, e.b);
+    g.prof_ms[e.stage] += ms;
+    g.prof_n[e.stage]++;
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  g.pending.clear();
+}
+
+// ---- batch description shared by all entry points ----
+struct Entry {
+  uint64_t limb_off;     // host entries: offset in the staged limb buffer
+  const uint64_t* dptr;  // device pointer to the magnitude limbs
+  uint64_t nlimbs;
+  uint64_t bits;  // upper bound on the magnitude's bit length
+  int sign;
+};
+struct Product {
+  int r, d;
+  std::vector<Entry> A, B;
+  std::vector<uint64_t*> out;  // device pointers, plan.rn limbs each
+  RfPlan plan;
+  uint64_t spec_off, coef_off;  // in doubles, within their regions
+};
+
+int plan_product(Product& P) {
+  uint64_t LA = 0, LB = 0;
+  for (auto& e : P.A) LA = std::max(LA, e.sign ? e.bits : 0);
+  for (auto& e : P.B) LB = std::max(LB, e.sign ? e.bits : 0);
+  if (rf_plan_choose(&g.primes, LA, LB, P.r, P.d, &P.plan) != 0)
+    return fail(RF_ERR_UNSUPPORTED,
+                "no LDS-resident transform plan for entries of %llu x %llu bits (r=%d, d=%d)",
+                (unsigned long long)LA, (unsigned long long)LB, P.r, P.d);
+  return RF_OK;
+}
+
+uint64_t out_limbs_for(uint64_t la_bits, uint64_t lb_bits, int d) {
+  if (la_bits == 0 || lb_bits == 0) return 0;
+  int cl = 0;
+  while ((1 << cl) < d) cl++;
+  return (la_bits + lb_bits + cl + 1 + 63) / 64;
+}
+
+// Scratch layout of one pipeline run (after the caller's `base` bytes).
+struct Layout {
+  uint64_t spec_n = 0, coef_n = 0;  // doubles
+  uint32_t nblk = 0;                // CRT blocks in total
+  size_t nfwd = 0, ninv = 0, ncrt = 0, ncarry = 0;
+  size_t spec_b = 0, coef_b = 0, tasks_b = 0, pre_b = 0, fun_b = 0;
+  size_t total() const { return spec_b + coef_b + tasks_b + pre_b + 2 * fun_b; }
+};
+
+uint32_t crt_blocks(const RfPlan& pl) {
+  const uint64_t ndig = (pl.rn + pl.t - 1) / pl.t;
+  return (uint32_t)((ndig + RF_DIGITS_PER_BLOCK - 1) / RF_DIGITS_PER_BLOCK);
+}
+
+Layout layout_of(std::vector<Product>& prods) {
+  Layout L;
+  for (auto& P : prods) {
+    const RfPlan& pl = P.plan;
+    P.spec_off = L.spec_n;
+    P.coef_off = L.coef_n;
+    if (pl.k == 0) continue;
+    const uint64_t N = (uint64_t)1 << pl.lgN;
+    const int rd = P.r * P.d, dd = P.d * P.d;
+    L.spec_n += (uint64_t)(rd + dd) * pl.k * N;
+    L.coef_n += (uint64_t)rd * pl.k * N;
+    L.nfwd += (size_t)(rd + dd) * pl.k;
+    L.ninv += (size_t)rd * pl.k;
+    L.ncrt += (size_t)rd * crt_blocks(pl);
+    L.ncarry += (size_t)rd;
+    L.nblk += (uint32_t)(rd * crt_blocks(pl));
+  }
+  L.spec_b = align256(L.spec_n * 8);
+  L.coef_b = align256(L.coef_n * 8);
+  L.tasks_b = align256(L.nfwd * sizeof(RfFwdTask)) + align256(L.ninv * sizeof(RfInvTask)) +
+              align256(L.ncrt * sizeof(RfCrtTask)) + align256(L.ncarry * sizeof(RfCarryTask));
+  L.pre_b = align256((size_t)L.nblk * RF_DIGITS_PER_BLOCK);
+  L.fun_b = align256(L.nblk);
+  return L;
+}
+
+// Enqueue the pipeline for planned products whose inputs are at
+// Entry::dptr and outputs at Product::out.  The arena must already hold
+// base + L.total() bytes; scratch starts at offset base.
+int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipStream_t st) {
+  if (g.arena_cap < base + L.total()) return fail(RF_ERR_NOMEM, "arena too small for pipeline");
+  std::map<int, std::vector<RfFwdTask>> fwd;
+  std::map<int, std::vector<RfInvTask>> inv;
+  std::map<int, std::vector<RfCrtTask>> crt;
+  std::vector<RfCarryTask> carry;
+  uint32_t nblk_total = 0, npre = 0;
+  for (auto& P : prods) {
+    const RfPlan& pl = P.plan;
+    if (pl.k == 0) continue;
+    const uint64_t N = (uint64_t)1 << pl.lgN;
+    const int r = P.r, d = P.d, k = pl.k;
+    auto spec_of = [&](int poly, int q) { return P.spec_off + ((uint64_t)poly * k + q) * N; };
+    for (int e = 0; e < r * d + d * d; e++) {
+      const Entry& E = e < r * d ? P.A[e] : P.B[e - r * d];
+      const bool isB = e >= r * d;
+      for (int q = 0; q < k; q++) {
+        RfFwdTask t{};
+        t.limbs = E.dptr;
+        t.spec_off = spec_of(e, q);
+        t.nlimbs = (uint32_t)E.nlimbs;
+        t.nchunk = E.sign ? (uint32_t)std::min<uint64_t>((E.nlimbs + pl.t - 1) / pl.t, N) : 0;
+        t.sign = E.sign;
+        t.q = (uint16_t)q;
+        t.t = (uint8_t)pl.t;
+        t.scale = isB ? (uint8_t)pl.lgN : 0;
+        fwd[pl.lgN].push_back(t);
+      }
+    }
+    for (int q = 0; q < k; q++)
+      for (int i = 0; i < r; i++)
+        for (int j = 0; j < d; j++) {
+          RfInvTask t{};
+          t.a_off = spec_of(i * d, q);
+          t.a_stride = (uint32_t)(k * N);
+          t.b_off = spec_of(r * d + j, q);
+          t.b_stride = (uint32_t)(d * k * N);
+          t.out_off = P.coef_off + ((uint64_t)(i * d + j) * k + q) * N;
+          t.q = (uint16_t)q;
+          t.d = (uint16_t)d;
+          inv[pl.lgN].push_back(t);
+        }
+    const uint32_t nblk = crt_blocks(pl);
+    for (int e = 0; e < r * d; e++) {
+      carry.push_back(RfCarryTask{nblk_total, nblk});
+      for (uint32_t b = 0; b < nblk; b++) {
+        RfCrtTask t{};
+        t.coef_off = P.coef_off + (uint64_t)e * k * N;
+        t.out = P.out[e];
+        t.pre_off = npre;
+        t.fun_idx = nblk_total + b;
+        t.N = (uint32_t)N;
+        t.nout = (uint32_t)pl.nout;
+        t.rn = (uint32_t)pl.rn;
+        t.digit0 = b * RF_DIGITS_PER_BLOCK;
+        t.t = (uint8_t)pl.t;
+        t.k = (uint8_t)k;
+        t.s = (uint8_t)pl.s;
+        crt[k].push_back(t);
+        npre += RF_DIGITS_PER_BLOCK;
+      }
+      nblk_total += nblk;
+    }
+  }
+  char* A = g.d_arena + base;
+  double* d_spec = (double*)A;
+  double* d_coef = (double*)(A + L.spec_b);
+  char* d_tasks = A + L.spec_b + L.coef_b;
+  uint8_t* d_pre = (uint8_t*)(d_tasks + L.tasks_b);
+  uint8_t* d_fun = d_pre + L.pre_b;
+  int8_t* d_cin = (int8_t*)(d_fun + L.fun_b);
+  int rc = ensure_stage(L.tasks_b);
+  if (rc) return rc;
+  // Earlier copies out of the staging buffer must finish before it is reused.
+  HIPCHK(hipStreamSynchronize(st));
+  char* h = g.h_stage;
+  size_t o = 0;
+  struct Range {
+    size_t off, n;
+  };
+  std::vector<std::pair<int, Range>> fwd_r, inv_r, crt_r;
+  for (auto& kv : fwd) {
+    fwd_r.push_back({kv.first, {o, kv.second.size()}});
+    memcpy(h + o, kv.second.data(), kv.second.size() * sizeof(RfFwdTask));
+    o += kv.second.size() * sizeof(RfFwdTask);
+  }
+  o = align256(o);
+  for (auto& kv : inv) {
+    inv_r.push_back({kv.first, {o, kv.second.size()}});
+    memcpy(h + o, kv.second.data(), kv.second.size() * sizeof(RfInvTask));
+    o += kv.second.size() * sizeof(RfInvTask);
+  }
+  o = align256(o);
+  for (auto& kv : crt) {
+    crt_r.push_back({kv.first, {o, kv.second.size()}});
+    memcpy(h + o, kv.second.data(), kv.second.size() * sizeof(RfCrtTask));
+    o += kv.second.size() * sizeof(RfCrtTask);
+  }
+  o = align256(o);
+  const size_t carry_o = o;
+  memcpy(h + o, carry.data(), carry.size() * sizeof(RfCarryTask));
+  o += carry.size() * sizeof(RfCarryTask);
+  if (o > L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
+  if (o) HIPCHK(hipMemcpyAsync(d_tasks, h, o, hipMemcpyHostToDevice, st));
+  for (auto& kv : fwd_r) {
+    ProfScope ps(RF_STAGE_FWD, st);
+    HIPCHK(rf_launch_fwd(g.tables, kv.first, (const RfFwdTask*)(d_tasks + kv.second.off),
+                         (int)kv.second.n, d_spec, st));
+  }
+  for (auto& kv : inv_r) {
+    ProfScope ps(RF_STAGE_INV, st);
+    HIPCHK(rf_launch_inv(g.tables, kv.first, (const RfInvTask*)(d_tasks + kv.second.off),
+                         (int)kv.second.n, d_spec, d_coef, st));
+  }
+  for (auto& kv : crt_r) {
+    ProfScope ps(RF_STAGE_CRT, st);
+    HIPCHK(rf_launch_crt(g.tables, kv.first, (const RfCrtTask*)(d_tasks + kv.second.off),
+                         (int)kv.second.n, d_coef, d_pre, d_fun, st));
+  }
+  if (!carry.empty()) {
+    ProfScope ps(RF_STAGE_CARRY, st);
+    HIPCHK(rf_launch_carry((const RfCarryTask*)(d_tasks + carry_o), (int)carry.size(), d_fun, d_cin, st));
+  }
+  for (auto& kv : crt_r) {
+    ProfScope ps(RF_STAGE_APPLY, st);
+    HIPCHK(rf_launch_apply((const RfCrtTask*)(d_tasks + kv.second.off), (int)kv.second.n, d_pre,
+                           d_cin, st));
+  }
+  return RF_OK;
+}
+
+uint64_t bitlen_limbs(const uint64_t* x, uint64_t n) {
+  while (n && x[n - 1] == 0) n--;
+  return n ? 64 * n - (uint64_t)__builtin_clzll(x[n - 1]) : 0;
+}
+
+// Host-array product: the engine behind every host entry point.  Entry e of
+// A is ap[e] (an[e] limbs, sign as[e]); likewise B.  Results come back as
+// two's complement, rn limbs per C entry, in out_tc.
+int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, const int* as,
+                 const uint64_t* const* bp, const uint64_t* bn, const int* bs,
+                 std::vector<std::vector<uint64_t>>& out_tc) {
+  if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
+  Product P;
+  P.r = r;
+  P.d = d;
+  uint64_t total_limbs = 0;
+  for (int e = 0; e < r * d + d * d; e++) {
+    const bool isA = e < r * d;
+    const int ei = isA ? e : e - r * d;
+    const uint64_t* p = isA ? ap[ei] : bp[ei];
+    uint64_t n = isA ? an[ei] : bn[ei];
+    int s = isA ? as[ei] : bs[ei];
+    Entry E{};
+    E.bits = s ? bitlen_limbs(p, n) : 0;
+    E.nlimbs = (E.bits + 63) / 64;
+    E.sign = E.bits ? (s < 0 ? -1 : 1) : 0;
+    E.limb_off = total_limbs;
+    total_limbs += E.nlimbs;
+    (isA ? P.A : P.B).push_back(E);
+  }
+  int rc = plan_product(P);
+  if (rc) return rc;
+  const RfPlan pl = P.plan;
+  out_tc.assign(r * d, std::vector<uint64_t>());
+  if (pl.k == 0) return RF_OK;  // a zero operand: all products are zero
+  std::vector<Product> prods{P};
+  const Layout L = layout_of(prods);
+  const size_t in_b = align256(total_limbs * 8);
+  const size_t out_n = (size_t)r * d * pl.rn;
+  const size_t out_b = align256(out_n * 8);
+  rc = ensure_arena(in_b + out_b + L.total());
+  if (rc) return rc;
+  rc = ensure_stage(std::max(std::max(in_b, out_b), L.tasks_b));
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  uint64_t* hs = (uint64_t*)g.h_stage;
+  Product& Q = prods[0];
+  for (int e = 0; e < r * d + d * d; e++) {
+    const bool isA = e < r * d;
+    const int ei = isA ? e : e - r * d;
+    const Entry& E = isA ? Q.A[ei] : Q.B[ei];
+    if (E.nlimbs) memcpy(hs + E.limb_off, isA ? ap[ei] : bp[ei], E.nlimbs * 8);
+  }
+  uint64_t* d_in = (uint64_t*)g.d_arena;
+  uint64_t* d_out = (uint64_t*)(g.d_arena + in_b);
+  if (total_limbs) HIPCHK(hipMemcpyAsync(d_in, hs, total_limbs * 8, hipMemcpyHostToDevice, g.stream));
+  for (auto* v : {&Q.A, &Q.B})
+    for (auto& E : *v) E.dptr = d_in + E.limb_off;
+  for (int e = 0; e < r * d; e++) Q.out.push_back(d_out + (size_t)e * pl.rn);
+  rc = run_pipeline(prods, L, in_b + out_b, g.stream);
+  if (rc) return rc;
+  // The task tables went through the staging buffer; the stream orders the
+  // result copy after them.
+  HIPCHK(hipStreamSynchronize(g.stream));
+  HIPCHK(hipMemcpyAsync(g.h_stage, d_out, out_n * 8, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (g.prof) prof_collect();
+  for (int e = 0; e < r * d; e++)
+    out_tc[e].assign((uint64_t*)g.h_stage + (size_t)e * pl.rn,
+                     (uint64_t*)g.h_stage + (size_t)(e + 1) * pl.rn);
+  return RF_OK;
+}
+
+// two's complement -> (magnitude limbs, negative?)
+void tc_to_signmag(std::vector<uint64_t>& x, bool& neg) {
+  neg = !x.empty() && (x.back() >> 63);
+  if (neg) {
+    uint64_t br = 1;
+    for (auto& v : x) {
+      uint64_t y = ~v + br;
+      br = (br && y == 0);
+      v = y;
+    }
+  }
+  while (!x.empty() && x.back() == 0) x.pop_back();
+}
+
+int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d) {
+  std::vector<const uint64_t*> ap(r * d), bp(d * d);
+  std::vector<uint64_t> an(r * d), bn(d * d);
+  std::vector<int> as(r * d), bs(d * d);
+  for (int e = 0; e < r * d; e++) {
+    ap[e] = (const uint64_t*)A[e]->_mp_d;
+    an[e] = mpz_size(A[e]);
+    as[e] = mpz_sgn(A[e]);
+  }
+  for (int e = 0; e < d * d; e++) {
+    bp[e] = (const uint64_t*)B[e]->_mp_d;
+    bn[e] = mpz_size(B[e]);
+    bs[e] = mpz_sgn(B[e]);
+  }
+  std::vector<std::vector<uint64_t>> out;
+  int rc = host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), out);
+  if (rc) return rc;
+  for (int e = 0; e < r * d; e++) {
+    bool neg = false;
+    tc_to_signmag(out[e], neg);
+    size_t n = out[e].size();
+    if (n == 0) {
+      mpz_set_ui(C[e], 0);
+      continue;
+    }
+    _mpz_realloc(C[e], (mp_size_t)n);  // src/lib/hwmpz.c:114
+    memcpy(C[e]->_mp_d, out[e].data(), n * 8);
+    C[e]->_mp_size = neg ? -(int)n : (int)n;  // src/lib/hwmpz.c:116-122
+  }
+  return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hw_mpz_setup(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (hw_disable_fft) return;
+  int rc = setup_locked();
+  if (rc) {
+    fprintf(stderr, "hw_mpz_setup: %s\n", g_err.c_str());
+    abort();
+  }
+}
+
+void hw_mpz_clear(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  clear_locked();
+}
+
+int rf_mpz_rmatrix_mult(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  return mpz_product(C, A, r, B, d);
+}
+
+void mpz_rmatrix_mult_fft(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, mpz_t w) {
+  (void)w;
+  int rc = rf_mpz_rmatrix_mult(C, A, r, B, d);
+  if (rc) {
+    fprintf(stderr, "mpz_rmatrix_mult_fft: %s\n", g_err.c_str());
+    abort();
+  }
+}
+
+void cu_mpz_rmatrix_mult_fft(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, mpz_t w) {
+  mpz_rmatrix_mult_fft(C, A, r, B, d, w);
+}
+
+int mpz_rmatrix_gpu_fft_suitable(mpz_t* A, int r, mpz_t* B, int d) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return 0;
+  uint64_t LA = 0, LB = 0;
+  for (int e = 0; e < r * d; e++) LA = std::max<uint64_t>(LA, mpz_sgn(A[e]) ? mpz_sizeinbase(A[e], 2) : 0);
+  for (int e = 0; e < d * d; e++) LB = std::max<uint64_t>(LB, mpz_sgn(B[e]) ? mpz_sizeinbase(B[e], 2) : 0);
+  RfPlan pl;
+  return rf_plan_choose(&g.primes, LA, LB, r, d, &pl) == 0;
+}
+
+const char* rf_last_error(void) { return g_err.c_str(); }
+
+int rf_rmatrix_mult_limbs(uint64_t* c, const size_t* coff, const size_t* ccap, long* csize,
+                          const uint64_t* a, const size_t* aoff, const size_t* an, const int* as,
+                          const uint64_t* b, const size_t* boff, const size_t* bn, const int* bs,
+                          int r, int d) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
+  std::vector<const uint64_t*> ap(r * d), bp(d * d);
+  std::vector<uint64_t> anv(r * d), bnv(d * d);
+  for (int e = 0; e < r * d; e++) ap[e] = a + aoff[e], anv[e] = an[e];
+  for (int e = 0; e < d * d; e++) bp[e] = b + boff[e], bnv[e] = bn[e];
+  std::vector<std::vector<uint64_t>> out;
+  int rc = host_product(r, d, ap.data(), anv.data(), as, bp.data(), bnv.data(), bs, out);
+  if (rc) return rc;
+  for (int e = 0; e < r * d; e++) {
+    bool neg = false;
+    if (out[e].empty()) {
+      csize[e] = 0;
+      continue;
+    }
+    tc_to_signmag(out[e], neg);
+    if (out[e].size() > ccap[e]) return fail(RF_ERR_CAPACITY, "output %d needs %zu limbs", e, out[e].size());
+    memcpy(c + coff[e], out[e].data(), out[e].size() * 8);
+    csize[e] = neg ? -(long)out[e].size() : (long)out[e].size();
+  }
+  return RF_OK;
+}
+
+uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d) {
+  return out_limbs_for(64 * la_limbs, 64 * lb_limbs, d);
+}
+
+int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, int async) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (nprod < 0 || (nprod > 0 && !prods)) return fail(RF_ERR_ARG, "bad product list");
+  hipStream_t st = stream ? (hipStream_t)stream : g.stream;
+  std::vector<Product> ps(nprod);
+  for (int i = 0; i < nprod; i++) {
+    const rf_dev_product& dp = prods[i];
+    Product& P = ps[i];
+    P.r = dp.r;
+    P.d = dp.d;
+    if (P.r <= 0 || P.d <= 0) return fail(RF_ERR_ARG, "product %d: bad dimensions", i);
+    uint64_t la = 0, lb = 0;
+    for (int e = 0; e < P.r * P.d; e++) {
+      const rf_dev_entry& x = dp.A[e];
+      Entry E{};
+      E.dptr = x.limbs;
+      E.nlimbs = x.nlimbs;
+      E.sign = x.nlimbs ? x.sign : 0;
+      E.bits = E.sign ? 64 * x.nlimbs : 0;
+      la = std::max(la, E.sign ? x.nlimbs : 0);
+      P.A.push_back(E);
+    }
+    for (int e = 0; e < P.d * P.d; e++) {
+      const rf_dev_entry& x = dp.B[e];
+      Entry E{};
+      E.dptr = x.limbs;
+      E.nlimbs = x.nlimbs;
+      E.sign = x.nlimbs ? x.sign : 0;
+      E.bits = E.sign ? 64 * x.nlimbs : 0;
+      lb = std::max(lb, E.sign ? x.nlimbs : 0);
+      P.B.push_back(E);
+    }
+    int rc = plan_product(P);
+    if (rc) return rc;
+    uint64_t rn = P.plan.k ? P.plan.rn : 0;
+    for (int e = 0; e < P.r * P.d; e++) {
+      if (dp.C[e].cap < rn) return fail(RF_ERR_CAPACITY, "product %d output %d needs %llu limbs", i, e, (unsigned long long)rn);
+      P.out.push_back(dp.C[e].limbs);
+      if (rn == 0 && dp.C[e].cap) HIPCHK(hipMemsetAsync(dp.C[e].limbs, 0, dp.C[e].cap * 8, st));
+    }
+    (void)la;
+    (void)lb;
+  }
+  const Layout L = layout_of(ps);
+  int rc = ensure_arena(L.total());
+  if (rc) return rc;
+  rc = run_pipeline(ps, L, 0, st);
+  if (rc) return rc;
+  if (!async) {
+    HIPCHK(hipStreamSynchronize(st));
+    if (g.prof) prof_collect();
+  }
+  return RF_OK;
+}
+
+int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  RfPlan pl;
+  if (rf_plan_choose(&g.primes, la_bits, lb_bits, r, d, &pl) != 0) return fail(RF_ERR_UNSUPPORTED, "no plan");
+  *t = pl.t;
+  *lgN = pl.lgN;
+  *k = pl.k;
+  return RF_OK;
+}
+
+int rf_primes(uint64_t* out, int max) {
+  RfPrimes pr;
+  rf_primes_init(&pr);
+  for (int i = 0; i < max && i < pr.k; i++) out[i] = pr.p[i];
+  return pr.k;
+}
+
+void rf_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g.prof = on != 0;
+}
+
+int rf_profile_read(double* ms, uint64_t* launches, int nstages) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  prof_collect();
+  for (int i = 0; i < nstages && i < RF_NSTAGES; i++) {
+    ms[i] = g.prof_ms[i];
+    launches[i] = g.prof_n[i];
+  }
+  return RF_OK;
+}
+
+void rf_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  prof_collect();
+  for (int i = 0; i < RF_NSTAGES; i++) g.prof_ms[i] = 0, g.prof_n[i] = 0;
+}
+
+// ---- stage entry points ----
+
+uint64_t rf_root(int q, int lgN) {
+  RfPrimes pr;
+  rf_primes_init(&pr);
+  if (q < 0 || q >= pr.k || lgN < 0 || lgN > RF_S) return 0;
+  uint64_t p = pr.p[q];
+  uint64_t w = pr.g[q];
+  for (int i = lgN; i < RF_S; i++) w = (uint64_t)(((unsigned __int128)w * w) % p);
+  return w;
+}
+
+static void canon_out(const double* x, size_t n, uint64_t p, uint64_t* out) {
+  for (size_t i = 0; i < n; i++) {
+    long long v = (long long)x[i];
+    long long m = v % (long long)p;
+    if (m < 0) m += (long long)p;
+    out[i] = (uint64_t)m;
+  }
+}
+
+int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs, const int* signs,
+                 int n, int t, uint32_t nchunk, int lgN, int q, int scale, uint64_t* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = setup_locked();
+  if (rc) return rc;
+  if (n <= 0 || t < 1 || t > RF_TMAX || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_MAXP ||
+      (scale != 0 && scale != lgN) || nchunk > (1u << lgN))
+    return fail(RF_ERR_ARG, "rf_stage_fwd: bad arguments");
+  const size_t N = (size_t)1 << lgN;
+  size_t total = 0;
+  for (int e = 0; e < n; e++) total = std::max(total, off[e] + nlimbs[e]);
+  const size_t in_b = align256(total * 8 + 8), spec_b = align256((size_t)n * N * 8),
+               task_b = align256(n * sizeof(RfFwdTask));
+  rc = ensure_arena(in_b + spec_b + task_b);
+  if (rc) return rc;
+  rc = ensure_stage(std::max(in_b + task_b, spec_b));
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  uint64_t* d_in = (uint64_t*)g.d_arena;
+  double* d_spec = (double*)(g.d_arena + in_b);
+  RfFwdTask* d_task = (RfFwdTask*)(g.d_arena + in_b + spec_b);
+  memcpy(g.h_stage, limbs, total * 8);
+  RfFwdTask* ht = (RfFwdTask*)(g.h_stage + in_b);
+  for (int e = 0; e < n; e++) {
+    RfFwdTask tk{};
+    tk.limbs = d_in + off[e];
+    tk.spec_off = (uint64_t)e * N;
+    tk.nlimbs = (uint32_t)nlimbs[e];
+    tk.nchunk = nchunk;
+    tk.sign = signs[e] < 0 ? -1 : (signs[e] > 0 ? 1 : 0);
+    tk.q = (uint16_t)q;
+    tk.t = (uint8_t)t;
+    tk.scale = (uint8_t)scale;
+    ht[e] = tk;
+  }
+  HIPCHK(hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(d_task, ht, n * sizeof(RfFwdTask), hipMemcpyHostToDevice, g.stream));
+  HIPCHK(rf_launch_fwd(g.tables, lgN, d_task, n, d_spec, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  HIPCHK(hipMemcpy(g.h_stage, d_spec, (size_t)n * N * 8, hipMemcpyDeviceToHost));
+  canon_out((const double*)g.h_stage, (size_t)n * N, g.primes.p[q], out);
+  return RF_OK;
+}
+
+int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int lgN, int q,
+                 uint64_t* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = setup_locked();
+  if (rc) return rc;
+  if (r <= 0 || d <= 0 || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_MAXP)
+    return fail(RF_ERR_ARG, "rf_stage_inv: bad arguments");
+  const size_t N = (size_t)1 << lgN;
+  const size_t np = (size_t)(r * d + d * d);
+  const size_t spec_b = align256(np * N * 8), coef_b = align256((size_t)r * d * N * 8),
+               task_b = align256((size_t)r * d * sizeof(RfInvTask));
+  rc = ensure_arena(spec_b + coef_b + task_b);
+  if (rc) return rc;
+  rc = ensure_stage(std::max(spec_b + task_b, coef_b));
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  double* hs = (double*)g.h_stage;
+  const uint64_t p = g.primes.p[q];
+  // centred representatives, as the forward stage stores them
+  for (size_t i = 0; i < (size_t)r * d * N; i++) {
+    uint64_t v = specA[i] % p;
+    hs[i] = v > p / 2 ? (double)v - (double)p : (double)v;
+  }
+  for (size_t i = 0; i < (size_t)d * d * N; i++) {
+    uint64_t v = specB[i] % p;
+    hs[(size_t)r * d * N + i] = v > p / 2 ? (double)v - (double)p : (double)v;
+  }
+  double* d_spec = (double*)g.d_arena;
+  double* d_coef = (double*)(g.d_arena + spec_b);
+  RfInvTask* d_task = (RfInvTask*)(g.d_arena + spec_b + coef_b);
+  RfInvTask* ht = (RfInvTask*)(g.h_stage + spec_b);
+  for (int i = 0; i < r; i++)
+    for (int j = 0; j < d; j++) {
+      RfInvTask tk{};
+      tk.a_off = (uint64_t)(i * d) * N;
+      tk.a_stride = (uint32_t)N;
+      tk.b_off = (uint64_t)(r * d + j) * N;
+      tk.b_stride = (uint32_t)(d * N);
+      tk.out_off = (uint64_t)(i * d + j) * N;
+      tk.q = (uint16_t)q;
+      tk.d = (uint16_t)d;
+      ht[i * d + j] = tk;
+    }
+  HIPCHK(hipMemcpyAsync(d_spec, hs, np * N * 8, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(d_task, ht, (size_t)r * d * sizeof(RfInvTask), hipMemcpyHostToDevice, g.stream));
+  HIPCHK(rf_launch_inv(g.tables, lgN, d_task, r * d, d_spec, d_coef, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  HIPCHK(hipMemcpy(g.h_stage, d_coef, (size_t)r * d * N * 8, hipMemcpyDeviceToHost));
+  canon_out((const double*)g.h_stage, (size_t)r * d * N, p, out);
+  return RF_OK;
+}
+
+int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, uint64_t* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = setup_locked();
+  if (rc) return rc;
+  if (k < 1 || k > 24 || t < 1 || t > RF_TMAX || nout < 1 || rn < 1)
+    return fail(RF_ERR_ARG, "rf_stage_crt: bad arguments");
+  const size_t coef_b = align256((size_t)k * nout * 8), out_b = align256(rn * 8);
+  const uint64_t ndig = (rn + t - 1) / t;
+  const uint32_t nblk = (uint32_t)((ndig + RF_DIGITS_PER_BLOCK - 1) / RF_DIGITS_PER_BLOCK);
+  const size_t task_b = align256(nblk * sizeof(RfCrtTask)) + align256(sizeof(RfCarryTask));
+  const size_t pre_b = align256((size_t)nblk * RF_DIGITS_PER_BLOCK), fun_b = align256(nblk);
+  rc = ensure_arena(coef_b + out_b + task_b + pre_b + 2 * fun_b);
+  if (rc) return rc;
+  rc = ensure_stage(std::max(coef_b + task_b, out_b));
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  double* hs = (double*)g.h_stage;
+  for (int j = 0; j < k; j++) {
+    const uint64_t p = g.primes.p[j];
+    for (uint32_t i = 0; i < nout; i++) {
+      uint64_t v = res[(size_t)j * nout + i] % p;
+      hs[(size_t)j * nout + i] = v > p / 2 ? (double)v - (double)p : (double)v;
+    }
+  }
+  char* A = g.d_arena;
+  double* d_coef = (double*)A;
+  uint64_t* d_out = (uint64_t*)(A + coef_b);
+  char* d_task = A + coef_b + out_b;
+  uint8_t* d_pre = (uint8_t*)(d_task + task_b);
+  uint8_t* d_fun = d_pre + pre_b;
+  int8_t* d_cin = (int8_t*)(d_fun + fun_b);
+  char* ht = g.h_stage + coef_b;
+  for (uint32_t b = 0; b < nblk; b++) {
+    RfCrtTask tk{};
+    tk.coef_off = 0;
+    tk.out = d_out;
+    tk.pre_off = (uint64_t)b * RF_DIGITS_PER_BLOCK;
+    tk.fun_idx = b;
+    tk.N = nout;
+    tk.nout = nout;
+    tk.rn = (uint32_t)rn;
+    tk.digit0 = b * RF_DIGITS_PER_BLOCK;
+    tk.t = (uint8_t)t;
+    tk.k = (uint8_t)k;
+    tk.s = (uint8_t)((48 * k + 1 + 63) / 64);
+    ((RfCrtTask*)ht)[b] = tk;
+  }
+  RfCarryTask ct{0, nblk};
+  memcpy(ht + align256(nblk * sizeof(RfCrtTask)), &ct, sizeof ct);
+  HIPCHK(hipMemcpyAsync(d_coef, hs, (size_t)k * nout * 8, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
+  const RfCrtTask* tasks = (const RfCrtTask*)d_task;
+  const RfCarryTask* ctask = (const RfCarryTask*)(d_task + align256(nblk * sizeof(RfCrtTask)));
+  HIPCHK(rf_launch_crt(g.tables, k, tasks, (int)nblk, d_coef, d_pre, d_fun, g.stream));
+  HIPCHK(rf_launch_carry(ctask, 1, d_fun, d_cin, g.stream));
+  HIPCHK(rf_launch_apply(tasks, (int)nblk, d_pre, d_cin, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  HIPCHK(hipMemcpy(out, d_out, rn * 8, hipMemcpyDeviceToHost));
+  return RF_OK;
+}
+
+}  // extern "C"

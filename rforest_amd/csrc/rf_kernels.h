@@ -1,0 +1,80 @@
+// rf_kernels.h -- task descriptors and launchers for the product pipeline.
+//
+// Device data layout (all in one arena per call or batch):
+//   limbs   u64   input magnitudes, entry after entry
+//   spec    f64   [poly][prime][N] spectra; polys are a product's A entries
+//                 (row-major r x d) then its B entries (d x d)
+//   coef    f64   [C entry][prime][N] coefficient residues after the inverse
+//   out     u64   [C entry][rn] two's complement result limbs
+// Residues are integers in doubles, |x| <= p/2 + 8 whenever stored.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rf_plan.h"
+
+// One forward task: pack chunks of one entry modulo prime q, transform.
+struct RfFwdTask {
+  const uint64_t* limbs;  // entry magnitude (device pointer)
+  uint64_t spec_off;      // spectrum at spec[spec_off ... + N)
+  uint32_t nlimbs;    // magnitude length in limbs
+  uint32_t nchunk;    // chunks to pack (<= N; the rest is zero)
+  int32_t sign;       // -1, 0, +1
+  uint16_t q;         // prime index
+  uint8_t t;          // limbs per chunk
+  uint8_t scale;      // 0: none; v: multiply by 2^-v (B operands, v = lgN)
+};
+
+// One inverse task: C[i][j] at prime q = sum_l A[i][l] B[l][j], inverse
+// transform, coefficient residues out.
+struct RfInvTask {
+  uint64_t a_off;     // spec offset of A[i][0] at prime q
+  uint64_t b_off;     // spec offset of B[0][j] at prime q
+  uint64_t out_off;   // coef offset of C[i][j] at prime q
+  uint32_t a_stride;  // spec distance between A[i][l] and A[i][l+1]
+  uint32_t b_stride;  // spec distance between B[l][j] and B[l+1][j]
+  uint16_t q;
+  uint16_t d;
+};
+
+// One CRT/recompose block: RF_DIGITS_PER_BLOCK digits of one C entry.
+#define RF_DIGITS_PER_BLOCK 256
+struct RfCrtTask {
+  uint64_t coef_off;  // coef offset of prime 0 of this entry; prime j at + j * N
+  uint64_t* out;      // output limbs of this entry (device pointer)
+  uint64_t pre_off;   // per-digit prefix carry function codes (bytes)
+  uint32_t fun_idx;   // index of this block's aggregate in the block arrays
+  uint32_t N;         // transform length (prime stride in coef)
+  uint32_t nout;      // number of coefficients
+  uint32_t rn;        // output limbs
+  uint32_t digit0;    // first digit of the block
+  uint8_t t;          // limbs per digit
+  uint8_t k;          // primes
+  uint8_t s;          // CRT limbs per coefficient
+  uint8_t pad;
+};
+
+// Per C entry, for the block-carry pass.
+struct RfCarryTask {
+  uint32_t fun_idx;   // first block's index in fun / cin
+  uint32_t nblk;
+};
+
+// Device-side pointers to the constant tables.
+struct RfDevTables {
+  const RfConsts* consts;
+  const double* tw_fwd;  // RF_MAXP * RF_FWD_TW * 2
+  const double* tw_inv;  // RF_MAXP * RF_INV_TW * 2
+};
+
+hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+                         double* spec, hipStream_t st);
+hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
+                         const double* spec, double* coef, hipStream_t st);
+// Blocks with the same k must be launched together (the kernel is specialised on k).
+hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, int ntasks,
+                         const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st);
+hipError_t rf_launch_carry(const RfCarryTask* tasks, int ntasks, const uint8_t* fun, int8_t* cin,
+                           hipStream_t st);
+hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre,
+                           const int8_t* cin, hipStream_t st);

@@ -1,0 +1,494 @@
+// rf_kernels.hip -- gfx950 kernels of the exact matrix-product pipeline.
+//
+//   k_fwd     pack (limb chunks -> residues, sign and 1/N folded in) + forward
+//             number-theoretic transform, natural order in, bit-reversed out
+//   k_inv     pointwise r x d . d x d product per frequency + inverse
+//             transform, bit-reversed in, natural order out
+//   k_crt     Garner CRT of each coefficient + digit sums of the overlapping
+//             coefficients + block scan of carry transfer functions
+//   k_carry   per C entry: carry into each block from the block aggregates
+//   k_apply   adds the resolved carries into the output limbs
+//
+// Transform definition (as src/include/fft62.h:35-54): with omega of order N,
+// forward output position m holds sum_i a_i omega^(i rev(m)).  Both
+// directions use Cooley-Tukey butterflies (mulmod then add/sub), so a value
+// grows by at most p per stage; every fourth stage reduces, which keeps all
+// mulmod inputs under 4.5 p + 8 < 2^51.
+#include "rf_field.h"
+#include "rf_kernels.h"
+
+#define RF_NTT_THREADS 512
+
+namespace {
+
+// Workgroup ids are dispatched round-robin over the 8 XCDs; remap so that
+// consecutive tasks (same entry's primes, or same prime's entries) land on
+// the same XCD and share its L2.  grid is a multiple of 8.
+__device__ __forceinline__ int xcd_task(int bid, int grid, int ntasks) {
+  int per = grid >> 3;
+  int t = (bid & 7) * per + (bid >> 3);
+  return t < ntasks ? t : -1;
+}
+
+// Residue of chunk i (t limbs) of an entry, scaled by the pack constants
+// pk[m] = 2^(24 m) S mod p split in 24-bit halves.  Pieces g_m < 2^24, so
+// each product g * half < 2^48 and the sums of at most 22 terms stay below
+// 2^53: exact.  Then s_hi 2^24 + s_lo is reduced (inputs < 2^53).
+__device__ __forceinline__ double pack_chunk(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+                                             uint32_t i, int t, const double (*pk)[2], double p,
+                                             double pinv) {
+  uint64_t c[RF_TMAX];
+  uint64_t base = (uint64_t)i * t;
+#pragma unroll
+  for (int l = 0; l < RF_TMAX; l++) c[l] = (l < t && base + l < nlimbs) ? limbs[base + l] : 0;
+  const int np = (64 * t + RF_PIECE - 1) / RF_PIECE;
+  double shi = 0.0, slo = 0.0;
+#pragma unroll
+  for (int m = 0; m < RF_MAXPIECES; m++) {
+    if (m < np) {
+      const int bit = RF_PIECE * m, li = bit >> 6, sh = bit & 63;
+      uint64_t g = c[li] >> sh;
+      if (sh > 64 - RF_PIECE && li + 1 < RF_TMAX) g |= c[li + 1] << (64 - sh);
+      g &= (1u << RF_PIECE) - 1;
+      double gd = (double)(uint32_t)g;
+      shi = __builtin_fma(gd, pk[m][0], shi);
+      slo = __builtin_fma(gd, pk[m][1], slo);
+    }
+  }
+  double r = rf_mulmod(shi, 16777216.0, 16777216.0 * pinv, p) + slo;
+  return rf_reduce(r, p, pinv);
+}
+
+template <int LGN>
+__global__ __launch_bounds__(RF_NTT_THREADS) void k_fwd(RfDevTables tb,
+                                                        const RfFwdTask* __restrict__ tasks,
+                                                        int ntasks,
+                                                        double* __restrict__ spec) {
+  constexpr int N = 1 << LGN;
+  extern __shared__ double sh[];
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int ti = xcd_task(blockIdx.x, gridDim.x, ntasks);
+  if (ti < 0) return;
+  const RfFwdTask tk = tasks[ti];
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  const double(*pk)[2] = C->pk[tk.q][tk.scale];
+  for (int i = tid; i < N; i += T) {
+    double v = 0.0;
+    if ((uint32_t)i < tk.nchunk && tk.sign != 0) {
+      v = pack_chunk(tk.limbs, tk.nlimbs, (uint32_t)i, tk.t, pk, p, pinv);
+      if (tk.sign < 0) v = -v;
+    }
+    sh[i] = v;
+  }
+  __syncthreads();
+  const double* tw = tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2;
+#pragma unroll 1
+  for (int L = 0; L < LGN; L++) {
+    const int lh = LGN - 1 - L;  // log2 of the half block length
+    for (int bi = tid; bi < N / 2; bi += T) {
+      const int m = bi >> lh, j = bi & ((1 << lh) - 1);
+      const int i0 = (m << (lh + 1)) + j, i1 = i0 + (1 << lh);
+      const double w = tw[2 * m], wq = tw[2 * m + 1];
+      const double x = sh[i0], y = sh[i1];
+      const double z = rf_mulmod(y, w, wq, p);
+      double u = x + z, v = x - z;
+      if ((L & 3) == 3) {
+        u = rf_reduce(u, p, pinv);
+        v = rf_reduce(v, p, pinv);
+      }
+      sh[i0] = u;
+      sh[i1] = v;
+    }
+    __syncthreads();
+  }
+  double* out = spec + tk.spec_off;
+  for (int i = tid; i < N; i += T) out[i] = rf_reduce(sh[i], p, pinv);
+}
+
+template <int LGN>
+__global__ __launch_bounds__(RF_NTT_THREADS) void k_inv(RfDevTables tb,
+                                                        const RfInvTask* __restrict__ tasks,
+                                                        int ntasks, const double* __restrict__ spec,
+                                                        double* __restrict__ coef) {
+  constexpr int N = 1 << LGN;
+  extern __shared__ double sh[];
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int ti = xcd_task(blockIdx.x, gridDim.x, ntasks);
+  if (ti < 0) return;
+  const RfInvTask tk = tasks[ti];
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  for (int i = tid; i < N; i += T) {
+    double acc = 0.0;
+    for (int l = 0; l < tk.d; l++) {
+      const double a = spec[tk.a_off + (size_t)l * tk.a_stride + i];
+      const double b = spec[tk.b_off + (size_t)l * tk.b_stride + i];
+      acc += rf_mulmod_var(a, b, p, pinv);  // |term| <= p/2 + 1
+      if ((l & 7) == 7) acc = rf_reduce(acc, p, pinv);
+    }
+    sh[i] = rf_reduce(acc, p, pinv);
+  }
+  __syncthreads();
+  const double* tw = tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2;
+#pragma unroll 1
+  for (int L = 0; L < LGN; L++) {
+    const int len = 1 << L;
+    for (int bi = tid; bi < N / 2; bi += T) {
+      const int blk = bi >> L, j = bi & (len - 1);
+      const int i0 = (blk << (L + 1)) + j, i1 = i0 + len;
+      const double w = tw[2 * (len + j)], wq = tw[2 * (len + j) + 1];
+      const double x = sh[i0], y = sh[i1];
+      const double z = rf_mulmod(y, w, wq, p);
+      double u = x + z, v = x - z;
+      if ((L & 3) == 3) {
+        u = rf_reduce(u, p, pinv);
+        v = rf_reduce(v, p, pinv);
+      }
+      sh[i0] = u;
+      sh[i1] = v;
+    }
+    __syncthreads();
+  }
+  double* out = coef + tk.out_off;
+  for (int i = tid; i < N; i += T) out[i] = rf_reduce(sh[i], p, pinv);
+}
+
+// ---- CRT, digit sums and carry transfer functions ----
+//
+// A carry transfer function maps an incoming carry c in {-1, 0, 1} to the
+// outgoing one, also in {-1, 0, 1}; code = sum_c (f(c) + 1) << (2 (c + 1)).
+__device__ __forceinline__ int fn_apply(uint32_t code, int c) {
+  return (int)((code >> (2 * (c + 1))) & 3u) - 1;
+}
+__device__ __forceinline__ uint32_t fn_compose(uint32_t g, uint32_t f) {  // g o f
+  uint32_t h = 0;
+#pragma unroll
+  for (int c = -1; c <= 1; c++) h |= (uint32_t)(fn_apply(g, fn_apply(f, c)) + 1) << (2 * (c + 1));
+  return h;
+}
+constexpr uint32_t FN_ID = (0u << 0) | (1u << 2) | (2u << 4);
+
+// Garner's algorithm on residues res[j * N] (|x| <= p_j/2 + 8) into the
+// S-limb two's complement of the representative in (-P/2, P/2)
+// (src/lib/crt.c:11-94 states the same CRT on 62-bit primes).
+// Mixed-radix digits v_j in [0, p_j): v_j = (x_j - [v_0 + p_0 (v_1 + ...)]) /
+// (p_0 ... p_{j-1}) mod p_j; all fp64 values stay below 2^50.
+template <int K, int S>
+__device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t N,
+                                       const RfConsts* __restrict__ C, uint64_t (&X)[S]) {
+  double v[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const double p = C->p[j];
+    const double x = res[(size_t)j * N];
+    double r;
+    if (j == 0) {
+      r = x;
+    } else {
+      double acc = v[j - 1];
+#pragma unroll
+      for (int l = j - 2; l >= 0; l--) acc = rf_mulmod(acc, C->pm[j][l][0], C->pm[j][l][1], p) + v[l];
+      r = rf_mulmod(x - acc, C->ci[j][0], C->ci[j][1], p);
+    }
+    v[j] = rf_canon(r, p);
+  }
+#pragma unroll
+  for (int l = 0; l < S; l++) X[l] = 0;
+  X[0] = (uint64_t)v[K - 1];
+#pragma unroll
+  for (int j = K - 2; j >= 0; j--) {
+    const uint64_t pj = C->pu[j];
+    uint64_t carry = (uint64_t)v[j];
+    // after this step X < p_0 ... p_{K-1} / (p_0 ... p_{j-1}) < 2^(48 (K - j))
+    constexpr int dummy = 0;
+    (void)dummy;
+#pragma unroll
+    for (int l = 0; l < S; l++) {
+      if (l * 64 < 48 * (K - j) + 64) {
+        const uint64_t lo = X[l] * pj;
+        const uint64_t hi = __umul64hi(X[l], pj);
+        const uint64_t s = lo + carry;
+        carry = hi + (s < lo);
+        X[l] = s;
+      }
+    }
+  }
+  // symmetric lift: X > floor(P / 2)  =>  X - P (two's complement in S limbs)
+  bool gt = false, decided = false;
+#pragma unroll
+  for (int l = S - 1; l >= 0; l--) {
+    const uint64_t h = C->Ph[K][l];
+    if (!decided && X[l] != h) {
+      gt = X[l] > h;
+      decided = true;
+    }
+  }
+  if (gt) {
+    uint64_t br = 0;
+#pragma unroll
+    for (int l = 0; l < S; l++) {
+      const uint64_t y = C->P[K][l];
+      const uint64_t d = X[l] - y - br;
+      br = (X[l] < y) || (X[l] == y && br);
+      X[l] = d;
+    }
+  }
+}
+
+// Sum of the slices of coefficients X_j (j0-based in LDS) falling on digit i
+// (limbs [t i, t i + t)), plus the incoming carry cin; writes limb l through
+// emit(l, value) and returns the signed carry out.  Coefficient j occupies
+// limbs [t j, t j + S) unsigned, minus 2^64 at limb t j + S if negative.
+template <int S, typename Emit>
+__device__ __forceinline__ int64_t digit_sum(const uint64_t* __restrict__ X, int j0, int i, int t,
+                                             int M, int64_t cin, Emit emit) {
+  const int jneg = i - S / t, lneg = S % t;
+  bool neg = false;
+  if (jneg >= j0 && jneg >= 0) neg = (X[(size_t)(jneg - j0) * S + (S - 1)] >> 63) != 0;
+  uint64_t lo = (uint64_t)cin;
+  int64_t hi = cin >> 63;
+  for (int l = 0; l < t; l++) {
+    for (int m = 0; m < M; m++) {
+      const int idx = m * t + l;
+      const int j = i - m;
+      if (idx < S && j >= 0) {
+        const uint64_t x = X[(size_t)(j - j0) * S + idx];
+        lo += x;
+        hi += (lo < x);
+      }
+    }
+    if (neg && l == lneg) {
+      hi -= (lo == 0);
+      lo -= 1;
+    }
+    emit(l, lo);
+    lo = (uint64_t)hi;
+    hi = hi >> 63;
+  }
+  return (int64_t)lo;
+}
+
+template <int K>
+__global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_crt(RfDevTables tb,
+                                                            const RfCrtTask* __restrict__ tasks,
+                                                            const double* __restrict__ coef,
+                                                            uint8_t* __restrict__ pre,
+                                                            uint8_t* __restrict__ fun) {
+  constexpr int S = (48 * K + 1 + 63) / 64;
+  constexpr int NC = RF_DIGITS_PER_BLOCK + S + 2;  // coefficients held per block
+  __shared__ uint64_t X[NC * S];
+  __shared__ int64_t E[RF_DIGITS_PER_BLOCK + 1];
+  __shared__ uint32_t code[RF_DIGITS_PER_BLOCK];
+  const int tid = threadIdx.x;
+  const RfCrtTask tk = tasks[blockIdx.x];
+  const int t = tk.t, M = (S + t - 1) / t;
+  const int digit0 = (int)tk.digit0;
+  const int j0 = digit0 - M - 2;  // digits digit0-1 .. need X from digit0 - 1 - M and jneg
+  const int nc = RF_DIGITS_PER_BLOCK + M + 2;
+  for (int c = tid; c < nc; c += RF_DIGITS_PER_BLOCK) {
+    const int j = j0 + c;
+    uint64_t x[S];
+    if (j >= 0 && (uint32_t)j < tk.nout) {
+      garner<K, S>(coef + tk.coef_off + (uint32_t)j, tk.N, tb.consts, x);
+    } else {
+#pragma unroll
+      for (int l = 0; l < S; l++) x[l] = 0;
+    }
+#pragma unroll
+    for (int l = 0; l < S; l++) X[c * S + l] = x[l];
+  }
+  __syncthreads();
+  // pass 1: carry out E_i of each digit on its own
+  {
+    auto none = [](int, uint64_t) {};
+    const int i = digit0 + tid;
+    E[tid + 1] = digit_sum<S>(X, j0, i, t, M, 0, none);
+    if (tid == 0) E[0] = digit0 > 0 ? digit_sum<S>(X, j0, digit0 - 1, t, M, 0, none) : 0;
+  }
+  __syncthreads();
+  // pass 2: U_i = (V_i + E_{i-1}) mod 2^(64 t), F_i = carry - E_i in {-1, 0, 1}
+  const int i = digit0 + tid;
+  const uint32_t ndig = (tk.rn + t - 1) / t;
+  uint32_t f = FN_ID;
+  if ((uint32_t)i < ndig) {
+    bool all0 = true, all1 = true;
+    uint64_t* dst = tk.out + (size_t)i * t;
+    const uint32_t lim = tk.rn - (uint32_t)i * t;
+    auto emit = [&](int l, uint64_t v) {
+      all0 &= (v == 0);
+      all1 &= (v == ~(uint64_t)0);
+      if ((uint32_t)l < lim) dst[l] = v;
+    };
+    const int64_t co = digit_sum<S>(X, j0, i, t, M, E[tid], emit);
+    const int F = (int)(co - E[tid + 1]);
+    f = 0;
+#pragma unroll
+    for (int c = -1; c <= 1; c++) {
+      int o = F + ((c == -1 && all0) ? -1 : 0) + ((c == 1 && all1) ? 1 : 0);
+      f |= (uint32_t)(o + 1) << (2 * (c + 1));
+    }
+  }
+  code[tid] = f;
+  __syncthreads();
+  // inclusive scan by composition: code[i] = f_i o ... o f_digit0
+#pragma unroll 1
+  for (int off = 1; off < RF_DIGITS_PER_BLOCK; off <<= 1) {
+    uint32_t g = code[tid];
+    if (tid >= off) g = fn_compose(g, code[tid - off]);
+    __syncthreads();
+    code[tid] = g;
+    __syncthreads();
+  }
+  pre[tk.pre_off + tid] = (uint8_t)(tid == 0 ? FN_ID : code[tid - 1]);
+  if (tid == RF_DIGITS_PER_BLOCK - 1) fun[tk.fun_idx] = (uint8_t)code[tid];
+}
+
+__global__ void k_carry(const RfCarryTask* __restrict__ tasks, int ntasks,
+                        const uint8_t* __restrict__ fun, int8_t* __restrict__ cin) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ntasks) return;
+  const RfCarryTask tk = tasks[e];
+  int c = 0;
+  for (uint32_t b = 0; b < tk.nblk; b++) {
+    cin[tk.fun_idx + b] = (int8_t)c;
+    c = fn_apply(fun[tk.fun_idx + b], c);
+  }
+}
+
+__global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_apply(const RfCrtTask* __restrict__ tasks,
+                                                              const uint8_t* __restrict__ pre,
+                                                              const int8_t* __restrict__ cin) {
+  const int tid = threadIdx.x;
+  const RfCrtTask tk = tasks[blockIdx.x];
+  const int t = tk.t;
+  const uint32_t i = tk.digit0 + tid;
+  const uint32_t ndig = (tk.rn + t - 1) / t;
+  if (i >= ndig) return;
+  const int c = fn_apply(pre[tk.pre_off + tid], cin[tk.fun_idx]);
+  if (c == 0) return;
+  uint64_t* dst = tk.out + (size_t)i * t;
+  const uint32_t lim = min((uint32_t)t, tk.rn - i * t);
+  for (uint32_t l = 0; l < lim; l++) {
+    const uint64_t x = dst[l];
+    if (c > 0) {
+      dst[l] = x + 1;
+      if (x != ~(uint64_t)0) break;
+    } else {
+      dst[l] = x - 1;
+      if (x != 0) break;
+    }
+  }
+}
+
+template <int LGN>
+hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntasks, double* spec,
+                        hipStream_t st) {
+  constexpr int N = 1 << LGN;
+  const int T = N / 2 < 64 ? 64 : (N / 2 > RF_NTT_THREADS ? RF_NTT_THREADS : N / 2);
+  const int grid = (ntasks + 7) & ~7;
+  hipLaunchKernelGGL(k_fwd<LGN>, dim3(grid), dim3(T), N * sizeof(double), st, tb, tasks, ntasks,
+                     spec);
+  return hipGetLastError();
+}
+
+template <int LGN>
+hipError_t launch_inv_t(const RfDevTables& tb, const RfInvTask* tasks, int ntasks,
+                        const double* spec, double* coef, hipStream_t st) {
+  constexpr int N = 1 << LGN;
+  const int T = N / 2 < 64 ? 64 : (N / 2 > RF_NTT_THREADS ? RF_NTT_THREADS : N / 2);
+  const int grid = (ntasks + 7) & ~7;
+  hipLaunchKernelGGL(k_inv<LGN>, dim3(grid), dim3(T), N * sizeof(double), st, tb, tasks, ntasks,
+                     spec, coef);
+  return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_crt_t(const RfDevTables& tb, const RfCrtTask* tasks, int ntasks,
+                        const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st) {
+  hipLaunchKernelGGL(k_crt<K>, dim3(ntasks), dim3(RF_DIGITS_PER_BLOCK), 0, st, tb, tasks, coef,
+                     pre, fun);
+  return hipGetLastError();
+}
+
+bool g_lds_attr_set = false;
+
+template <int LGN>
+void set_lds_attr() {
+  constexpr int bytes = (1 << LGN) * sizeof(double);
+  if (bytes > 65536) {
+    hipFuncSetAttribute((const void*)k_fwd<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_inv<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  }
+}
+
+void ensure_lds_attrs() {
+  if (g_lds_attr_set) return;
+  set_lds_attr<14>();
+  g_lds_attr_set = true;
+}
+
+}  // namespace
+
+hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+                         double* spec, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  ensure_lds_attrs();
+  switch (lgN) {
+#define RF_CASE(L) \
+  case L:          \
+    return launch_fwd_t<L>(tb, tasks, ntasks, spec, st);
+    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
+    RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
+#undef RF_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
+                         const double* spec, double* coef, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  ensure_lds_attrs();
+  switch (lgN) {
+#define RF_CASE(L) \
+  case L:          \
+    return launch_inv_t<L>(tb, tasks, ntasks, spec, coef, st);
+    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
+    RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
+#undef RF_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, int ntasks,
+                         const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  switch (k) {
+#define RF_CASE(K) \
+  case K:          \
+    return launch_crt_t<K>(tb, tasks, ntasks, coef, pre, fun, st);
+    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7) RF_CASE(8)
+    RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14) RF_CASE(15)
+    RF_CASE(16) RF_CASE(17) RF_CASE(18) RF_CASE(19) RF_CASE(20) RF_CASE(21) RF_CASE(22)
+    RF_CASE(23) RF_CASE(24)
+#undef RF_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rf_launch_carry(const RfCarryTask* tasks, int ntasks, const uint8_t* fun, int8_t* cin,
+                           hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_carry, dim3((ntasks + 63) / 64), dim3(64), 0, st, tasks, ntasks, fun, cin);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre,
+                           const int8_t* cin, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply, dim3(ntasks), dim3(RF_DIGITS_PER_BLOCK), 0, st, tasks, pre, cin);
+  return hipGetLastError();
+}

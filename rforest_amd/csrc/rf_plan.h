@@ -1,0 +1,84 @@
+// rf_plan.h -- prime set, transform plans and device constant tables.
+//
+// A plan fixes, for one r x d times d x d product, the chunk width b = 64 t
+// bits, the transform length N = 2^lgN and the number k of primes.  The
+// planner proves for each candidate that the exact convolution coefficients
+// fit: with nA, nB chunks per entry, nA + nB - 1 <= N and
+// 2 * d * min(nA, nB) * (2^b - 1)^2 < P = p_0 ... p_{k-1}, checked in exact
+// multiword integer arithmetic (the reference checks a floating-point
+// estimate instead, src/lib/mpnfft.c:44-62).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#define RF_MAXP 32        // primes generated at setup
+#define RF_S 24           // every prime is 1 mod 2^RF_S
+#define RF_LGN_MIN 1
+#define RF_LGN_MAX 14     // largest LDS-resident transform (2^14 doubles = 128 KiB)
+#define RF_TMAX 8         // chunk width up to 8 limbs = 512 bits
+#define RF_PIECE 24       // pack splits a chunk into 24-bit pieces
+#define RF_MAXPIECES 22   // ceil(512 / 24)
+#define RF_MAXS 24        // CRT limbs per coefficient, >= ceil((48 RF_MAXP + 2) / 64)
+
+struct RfPrimes {
+  int k;
+  uint64_t p[RF_MAXP];
+  uint64_t g[RF_MAXP];  // root of order 2^RF_S: smallest u >= 2 with u^((p-1)/2^S) of full order
+};
+
+struct RfPlan {
+  int t;        // limbs per chunk; b = 64 t
+  int lgN;      // transform length 2^lgN
+  int k;        // primes used (the first k of RfPrimes)
+  int s;        // CRT output limbs per coefficient
+  int nA, nB;   // chunks per A / B entry (max over the product's entries)
+  int nout;     // nA + nB - 1 (0 if either side is zero)
+  int M;        // coefficients overlapping one output digit: ceil(s / t)
+  size_t rn;    // output limbs per C entry (two's complement, sign included)
+  double cost;  // planner's estimate, arbitrary units
+};
+
+// Generates the RF_MAXP largest primes 2^47 < p < 2^48 with p = 1 mod 2^RF_S.
+void rf_primes_init(RfPrimes* pr);
+
+// Exact test of f * (2^b - 1)^2 < p_0 ... p_{k-1}.
+bool rf_bound_holds(const RfPrimes* pr, int k, int b, uint64_t f);
+
+// floor(log2(p_0 ... p_{k-1})).
+int rf_floor_log2_P(const RfPrimes* pr, int k);
+
+// Chooses the cheapest proven plan for an r x d times d x d product whose A
+// entries have at most LA bits and B entries at most LB bits.  Returns 0, or
+// -1 if no plan within the LDS-resident transform sizes exists.
+int rf_plan_choose(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, RfPlan* out);
+
+// Fills a plan for a fixed (t, lgN); returns 0 or -1 if it cannot hold.
+int rf_plan_fixed(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t, int lgN,
+                  RfPlan* out);
+
+// ---- constant tables shared by all kernels (one copy in device memory) ----
+struct RfConsts {
+  double p[RF_MAXP];
+  double pinv[RF_MAXP];
+  // Garner: pm[j][l] = (p_l mod p_j, fl(that / p_j)) for l < j;
+  // ci[j] = ((p_0...p_{j-1})^-1 mod p_j, fl(. / p_j)).
+  double pm[RF_MAXP][RF_MAXP][2];
+  double ci[RF_MAXP][2];
+  uint64_t pu[RF_MAXP];                  // the primes as integers
+  uint64_t P[RF_MAXP + 1][RF_MAXS];      // P_k = p_0 ... p_{k-1}
+  uint64_t Ph[RF_MAXP + 1][RF_MAXS];     // floor(P_k / 2)
+  // pack: pk[q][v][m] = (hi, lo) 24-bit halves of 2^(24 m) * S_v mod p_q, where
+  // S_0 = 1 and S_v = 2^-v mod p_q for v = lgN (B operands absorb 1/N).
+  double pk[RF_MAXP][RF_LGN_MAX + 1][RF_MAXPIECES][2];
+};
+
+// Twiddle tables, per prime, as (w, fl(w/p)) pairs:
+//   fwd[q][m] = omega^(rev_{RF_LGN_MAX-1}(m)), m < 2^(RF_LGN_MAX-1),
+//     omega of order 2^RF_LGN_MAX; the prefix of length N/2 serves length N.
+//   inv[q][len + j] = omega_{2 len}^(-j), len = 2^e < 2^RF_LGN_MAX, j < len.
+#define RF_FWD_TW (1 << (RF_LGN_MAX - 1))
+#define RF_INV_TW (1 << RF_LGN_MAX)
+
+void rf_consts_init(const RfPrimes* pr, RfConsts* c);
+void rf_twiddles_init(const RfPrimes* pr, double* fwd /* RF_MAXP*RF_FWD_TW*2 */,
+                      double* inv /* RF_MAXP*RF_INV_TW*2 */);
