@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Headline benchmark: exact r=2 1-Mbit matrix products per second on one MI355X.
+
+One step = one pass of the whole product pipeline (pack + forward NTT,
+pointwise product + inverse NTT, Garner CRT + carry scan) over a batch of
+independent products 2x2 . 2x2 whose entries are 1,048,576-bit integers with
+the top bit set (GMP generator seeded 42), inputs and outputs resident in
+HBM.  Prints one JSON line with the throughput, a roofline entry for the
+dominant kernel measured with HIP events on the library's stream, and the
+reference CPU path timed on a bounded sample of the same products.
+"""
+import argparse
+import json
+import os
+import random
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "2x2 . 2x2 exact integer matrix product, 1,048,576-bit unsigned entries (top bit set), GMP seed 42"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def modcheck(products, results, q):
+    """C == A B modulo a random 61-bit prime, for every product (a
+    size-independent property of the exact result)."""
+    bad = 0
+    for (r, d, A, B), C in zip(products, results):
+        Am = [x % q for x in A]
+        Bm = [x % q for x in B]
+        for i in range(r):
+            for j in range(d):
+                want = sum(Am[i * d + k] * Bm[k * d + j] for k in range(d)) % q
+                if C[i * d + j] % q != want:
+                    bad += 1
+    return bad
+
+
+def traffic_from_profiles(kernel, batch, plan):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/pmc_traffic.json), if it was measured for this batch
+    and plan; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("batch") != batch or list(rec.get("plan", [])) != list(plan):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(products, gpu_results, budget_s):
+    import oracle
+    if oracle.ref_available():
+        fn, kind = oracle.ref_rmatrix_mult, "reference"
+        desc = "reference CPU mpz_rmatrix_mult_fft (src/lib/hwmpz.c:65-130) built from its sources"
+    else:
+        def fn(A, r, B, d):
+            pl, _ = oracle.ref_plan(max((abs(x).bit_length() + 63) // 64 for x in A + B), d)
+            return oracle.rmatrix_mult(A, r, B, d, pl)
+        kind = "port"
+        desc = "C restatement of the reference path (oracle/rf_oracle.c)"
+    n, el, mismatches = 0, 0.0, 0
+    while n < len(products) and (n < 2 or el < budget_s):
+        r, d, A, B = products[n]
+        t0 = time.perf_counter()
+        C = fn(A, r, B, d)
+        el += time.perf_counter() - t0
+        mismatches += int(C != gpu_results[n])
+        n += 1
+    return {"value": n / el, "unit": "products/s", "cores": 1, "kind": kind,
+            "sample": f"{n} products of the same workload, one thread, {el:.1f} s; {desc}",
+            "gpu_results_identical": mismatches == 0}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=64, help="products per step")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the product pipeline runs on a single MI355X; "
+              "multi-GPU sharding is described in DESIGN.md and not measured (--gpus 1 only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import rforest_amd as rf
+    from rforest_amd import workloads
+
+    log("generating inputs")
+    products = workloads.headline(args.batch, seed=42)
+    plan = rf.plan(1 << 20, 1 << 20, 2, 2)
+    t, lgN, k = plan
+    N = 1 << lgN
+    rf.hw_mpz_setup()
+    batch = rf.DeviceBatch(products)
+    log(f"plan: chunk {64 * t} bits, N = 2^{lgN}, k = {k} primes; batch {args.batch}")
+
+    for _ in range(max(args.warmup, 1)):
+        batch.run(sync=False)
+    rf.device_sync()
+    results = batch.results()
+    exact0 = results[0] == rf.mpz_rmatrix_mult_naive(products[0][2], 2, products[0][3], 2)
+    q = (1 << 61) - 1
+    bad = modcheck(products, results, q)
+    if not exact0 or bad:
+        log(f"RESULT MISMATCH: exact product 0 {exact0}, modular mismatches {bad}")
+        sys.exit(1)
+
+    rf.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(sync=False)
+    rf.device_sync()
+    el = time.perf_counter() - t0
+    ms_per_step = el / args.steps * 1e3
+    value = args.batch * args.steps / el
+
+    # Per-kernel timing with HIP events on the library's stream.
+    rf.profile(enable=True)
+    rf.profile(reset=True)
+    psteps = max(3, min(args.steps, 10))
+    for _ in range(psteps):
+        batch.run(sync=False)
+    rf.device_sync()
+    prof = rf.profile()
+    rf.profile(enable=False)
+    # algorithmic bytes: one read and one write of every residue word per
+    # transform (per prime), fp64 words
+    per_transform = 2 * N * 8
+    fwd_bytes = args.batch * (4 + 4) * k * per_transform
+    inv_bytes = args.batch * 4 * k * per_transform
+    alg = {"fwd": fwd_bytes, "inv": inv_bytes}
+    kern_names = {"fwd": "k_fwd (pack + forward NTT)", "inv": "k_inv (pointwise + inverse NTT)"}
+    dom = max(("fwd", "inv"), key=lambda s: prof[s][0])
+    ms, nl = prof[dom]
+    avg_s = ms / nl / 1e3
+    achieved = alg[dom] / avg_s / 1e9
+    traffic = traffic_from_profiles(dom, args.batch, plan)
+    ntt_ms = prof["fwd"][0] + prof["inv"][0]
+    ntt_launch_s = ntt_ms / psteps / 1e3
+    ntt_ach = (fwd_bytes + inv_bytes) / ntt_launch_s / 1e9
+    stage_ms = {s: round(prof[s][0] / psteps, 4) for s in prof}
+
+    # PCIe-inclusive drop-in (host mpz in, host mpz out), one product per call
+    rf.mpz_rmatrix_mult_fft(products[0][2], 2, products[0][3], 2)
+    nd = min(8, args.batch)
+    t1 = time.perf_counter()
+    for i in range(nd):
+        rf.mpz_rmatrix_mult_fft(products[i][2], 2, products[i][3], 2)
+    dropin = nd / (time.perf_counter() - t1)
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        cpu = cpu_baseline(products, results, args.cpu_seconds)
+
+    out = {
+        "metric": "r=2 1-Mbit matmuls/sec",
+        "value": round(value, 2),
+        "unit": "products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "products_per_step": args.batch, "r": 2, "d": 2,
+                   "entry_bits": 1 << 20, "chunk_bits": 64 * t, "ntt_length": N, "primes": k,
+                   "prime_bits": 48, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": kern_names[dom], "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "alg_bytes_per_launch": alg[dom],
+                     "avg_launch_ms": round(avg_s * 1e3, 5)},
+        "roofline_ntt_passes": {"achieved": round(ntt_ach, 2), "frac": round(ntt_ach / HBM_PEAK_GBS, 4),
+                                "alg_bytes_per_product": (fwd_bytes + inv_bytes) // args.batch},
+        "stage_ms_per_step": stage_ms,
+        "dropin_pcie_products_per_s": round(dropin, 2),
+        "cpu_baseline": cpu,
+        "checks": {"exact_product0": exact0, "mod_2^61-1_mismatches": bad},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -115,6 +115,21 @@ uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d);
 int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, int async);
 
 /* ---------------------------------------------------------------------------
+ * Device and pinned memory (src/include/hwmpz.h:38-46 declares the same
+ * helpers for the reference's CUDA path; callers of rf_batch_mult_device use
+ * them to place entries in HBM without another runtime in the process)
+ * ------------------------------------------------------------------------- */
+void* cu_malloc_pinned(size_t n);
+void cu_free_pinned(void* p);
+void* cu_device_malloc(size_t n);
+void cu_device_free(void* p);
+void cu_memcpy_d2h(void* dst, const void* src, size_t n);
+/* Host to device copy; returns RF_OK or RF_ERR_HIP. */
+int rf_memcpy_h2d(void* dst, const void* src, size_t n);
+/* Waits for all work the library enqueued (hipDeviceSynchronize). */
+int rf_device_sync(void);
+
+/* ---------------------------------------------------------------------------
  * Introspection (no reference counterpart)
  * ------------------------------------------------------------------------- */
 

@@ -90,6 +90,19 @@ def lib():
     L.rf_profile_enable.argtypes = [ctypes.c_int]
     L.rf_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int]
     L.mpz_rmatrix_gpu_fft_suitable.restype = ctypes.c_int
+    L.cu_device_malloc.restype = ctypes.c_void_p
+    L.cu_device_malloc.argtypes = [ctypes.c_size_t]
+    L.cu_device_free.restype = None
+    L.cu_device_free.argtypes = [ctypes.c_void_p]
+    L.cu_malloc_pinned.restype = ctypes.c_void_p
+    L.cu_malloc_pinned.argtypes = [ctypes.c_size_t]
+    L.cu_free_pinned.restype = None
+    L.cu_free_pinned.argtypes = [ctypes.c_void_p]
+    L.cu_memcpy_d2h.restype = None
+    L.cu_memcpy_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    L.rf_memcpy_h2d.restype = ctypes.c_int
+    L.rf_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    L.rf_device_sync.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -227,4 +240,4 @@ def profile(enable=None, reset=False):
     return {STAGE_NAMES[i]: (ms[i], int(n[i])) for i in range(5)}
 
 
-from .device import DeviceBatch  # noqa: E402,F401
+from .device import DeviceBatch, DeviceBuffer, device_sync  # noqa: E402,F401

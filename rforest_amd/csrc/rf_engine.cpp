@@ -33,7 +33,8 @@ int mpzfft_threads = 1;
 namespace {
 
 struct Ctx {
-  bool ready = false;
+  bool ready = false;       // GPU state initialised
+  bool host_ready = false;  // prime set initialised (no GPU needed)
   RfPrimes primes;
   RfConsts* d_consts = nullptr;
   double* d_twf = nullptr;
@@ -79,10 +80,18 @@ int fail(int code, const char* fmt, ...) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int setup_locked() {
-  if (g.ready) return RF_OK;
+int host_init_locked() {
+  if (g.host_ready) return RF_OK;
   rf_primes_init(&g.primes);
   if (g.primes.k < RF_MAXP) return fail(RF_ERR_ARG, "prime generation found only %d primes", g.primes.k);
+  g.host_ready = true;
+  return RF_OK;
+}
+
+int setup_locked() {
+  if (g.ready) return RF_OK;
+  int rc0 = host_init_locked();
+  if (rc0) return rc0;
   RfConsts* hc = (RfConsts*)calloc(1, sizeof(RfConsts));
   std::vector<double> twf((size_t)RF_MAXP * RF_FWD_TW * 2), twi((size_t)RF_MAXP * RF_INV_TW * 2);
   if (!hc) return fail(RF_ERR_NOMEM, "host allocation failed");
@@ -548,7 +557,7 @@ void cu_mpz_rmatrix_mult_fft(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, mpz_t w
 
 int mpz_rmatrix_gpu_fft_suitable(mpz_t* A, int r, mpz_t* B, int d) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g.ready) return 0;
+  if (host_init_locked()) return 0;
   uint64_t LA = 0, LB = 0;
   for (int e = 0; e < r * d; e++) LA = std::max<uint64_t>(LA, mpz_sgn(A[e]) ? mpz_sizeinbase(A[e], 2) : 0);
   for (int e = 0; e < d * d; e++) LB = std::max<uint64_t>(LB, mpz_sgn(B[e]) ? mpz_sizeinbase(B[e], 2) : 0);
@@ -648,7 +657,9 @@ int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, i
 
 int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  int rc = host_init_locked();
+  if (rc) return rc;
+  if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
   RfPlan pl;
   if (rf_plan_choose(&g.primes, la_bits, lb_bits, r, d, &pl) != 0) return fail(RF_ERR_UNSUPPORTED, "no plan");
   *t = pl.t;
@@ -683,6 +694,54 @@ void rf_profile_reset(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   prof_collect();
   for (int i = 0; i < RF_NSTAGES; i++) g.prof_ms[i] = 0, g.prof_n[i] = 0;
+}
+
+// ---- device and pinned memory helpers ----
+
+void* cu_malloc_pinned(size_t n) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, n ? n : 1, hipHostMallocDefault) != hipSuccess) {
+    fail(RF_ERR_NOMEM, "hipHostMalloc(%zu) failed", n);
+    return nullptr;
+  }
+  return p;
+}
+
+void cu_free_pinned(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
+void* cu_device_malloc(size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, n ? n : 1) != hipSuccess) {
+    fail(RF_ERR_NOMEM, "hipMalloc(%zu) failed", n);
+    return nullptr;
+  }
+  return p;
+}
+
+void cu_device_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void cu_memcpy_d2h(void* dst, const void* src, size_t n) {
+  hipError_t e = hipMemcpy(dst, src, n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    fprintf(stderr, "cu_memcpy_d2h: %s\n", hipGetErrorString(e));
+    abort();
+  }
+}
+
+int rf_memcpy_h2d(void* dst, const void* src, size_t n) {
+  HIPCHK(hipMemcpy(dst, src, n, hipMemcpyHostToDevice));
+  return RF_OK;
+}
+
+int rf_device_sync(void) {
+  HIPCHK(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g.prof) prof_collect();
+  return RF_OK;
 }
 
 // ---- stage entry points ----

@@ -1,21 +1,60 @@
 """Device-resident batches: many products of one product-tree level in one
-call of rf_batch_mult_device, with inputs and outputs in HBM (torch
-tensors as plain device memory)."""
+call of rf_batch_mult_device, inputs and outputs in HBM.  Device memory
+comes from the library itself (cu_device_malloc), so no second GPU runtime
+shares the process."""
 import ctypes
 
-from . import DevEntry, DevOut, DevProduct, check, lib
+from . import DevEntry, DevOut, DevProduct, RfError, RF_ERR_NOMEM, check, lib
+
+
+class DeviceBuffer:
+    """nbytes of device memory owned by librforest_hip.so."""
+
+    def __init__(self, nbytes):
+        L = lib()
+        self.nbytes = max(int(nbytes), 8)
+        self.ptr = L.cu_device_malloc(self.nbytes)
+        if not self.ptr:
+            raise RfError(RF_ERR_NOMEM, L.rf_last_error().decode())
+
+    def upload(self, host_array):
+        import numpy as np
+        a = np.ascontiguousarray(host_array)
+        assert a.nbytes <= self.nbytes
+        check(lib().rf_memcpy_h2d(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+    def download(self, dtype, count):
+        import numpy as np
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        lib().cu_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr), out.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cu_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync():
+    check(lib().rf_device_sync())
 
 
 class DeviceBatch:
     """Uploads a list of products [(r, d, A, B), ...] (Python ints) once; run()
     enqueues the whole pipeline over all of them; results() reads back."""
 
-    def __init__(self, products, device="cuda"):
+    def __init__(self, products):
         import numpy as np
-        import torch
         self.products = products
         L = lib()
-        chunks, descs = [], []
+        chunks = []
         pos = 0
         plan_in = []
         for (r, d, A, B) in products:
@@ -24,19 +63,19 @@ class DeviceBatch:
                 x = int(x)
                 n = (abs(x).bit_length() + 63) // 64
                 if n:
-                    chunks.append(np.frombuffer(abs(x).to_bytes(8 * n, "little"), dtype=np.int64))
+                    chunks.append(np.frombuffer(abs(x).to_bytes(8 * n, "little"), dtype=np.uint64))
                 ents.append((pos, n, (x > 0) - (x < 0)))
                 pos += n
             la = max([n for (_, n, s) in ents[:r * d] if s] or [0])
             lb = max([n for (_, n, s) in ents[r * d:] if s] or [0])
             plan_in.append((r, d, ents, int(L.rf_out_limbs(la, lb, d))))
-        host = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.int64)
-        self.limbs = torch.from_numpy(np.ascontiguousarray(host)).to(device)
-        self.out_sizes = []
+        host = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.uint64)
+        self.limbs = DeviceBuffer(host.nbytes)
+        self.limbs.upload(host)
         total_out = sum(max(rn, 1) * r * d for (r, d, _, rn) in plan_in)
-        self.out = torch.zeros(max(total_out, 1), dtype=torch.int64, device=device)
-        base = self.limbs.data_ptr()
-        obase = self.out.data_ptr()
+        self.out = DeviceBuffer(8 * max(total_out, 1))
+        self.total_out = total_out
+        base, obase = self.limbs.ptr, self.out.ptr
         self._keep = []
         prods = (DevProduct * len(products))()
         opos = 0
@@ -70,9 +109,8 @@ class DeviceBatch:
     def results(self):
         """Per product, the r*d entries of C as Python ints."""
         import numpy as np
-        import torch
-        torch.cuda.synchronize()
-        host = self.out.cpu().numpy().view(np.uint64)
+        device_sync()
+        host = self.out.download(np.uint64, max(self.total_out, 1))
         res = []
         for cpos in self.layout:
             vals = []

@@ -1,0 +1,115 @@
+"""CPU-side checks of librforest_hip.so: it loads, exports every function the
+public headers declare, and its host-only parts (prime set, roots, planner,
+error returns before setup) behave as documented.  No GPU work here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import rforest_amd as rf
+from helpers import is_prime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("rforest_hip.h", "rforest_hip_stages.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", src, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "while", "for", "return", "sizeof", "defined"):
+                names.add(name)
+    return sorted(names)
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = rf.lib()
+    decl = declared_functions()
+    assert "mpz_rmatrix_mult_fft" in decl and "rf_batch_mult_device" in decl and "rf_stage_fwd" in decl
+    for name in decl:
+        assert hasattr(L, name), f"{name} declared in include/ but not exported"
+    for var in ("hw_disable_fft", "mpzfft_threads"):
+        ctypes.c_int.in_dll(L, var)
+
+
+def test_prime_set():
+    ps = rf.primes(32)
+    assert len(set(ps)) == 32
+    assert ps == sorted(ps, reverse=True)
+    for p in ps:
+        assert (1 << 47) < p < (1 << 48)
+        assert p % (1 << 24) == 1
+        assert is_prime(p)
+
+
+def test_roots_match_oracle_rule():
+    import oracle
+    ps = rf.primes(32)
+    for q in (0, 1, 7, 31):
+        for lgN in (1, 2, 9, 13, 14, 24):
+            w = rf.root(q, lgN)
+            assert w == oracle.root(ps[q], 24, lgN)
+            assert pow(w, 1 << (lgN - 1), ps[q]) == ps[q] - 1
+
+
+def _bound_ok(t, lgN, k, la, lb, d):
+    b = 64 * t
+    na, nb = -(-la // b), -(-lb // b)
+    P = 1
+    for p in rf.primes(k):
+        P *= p
+    return na + nb - 1 <= (1 << lgN) and 2 * d * min(na, nb) * ((1 << b) - 1) ** 2 < P
+
+
+@pytest.mark.parametrize("la,lb,r,d", [(1 << 20, 1 << 20, 2, 2), (1 << 20, 1 << 20, 4, 4), (4096, 4096, 2, 2),
+                                       (1 << 10, 1 << 10, 2, 2), (1 << 18, 1 << 17, 2, 2), (64, 64, 1, 1),
+                                       (4 << 20, 1 << 20, 3, 3), (1, 1, 1, 1), (3000, 50000, 2, 5)])
+def test_planner_bound_is_proven_and_minimal(la, lb, r, d):
+    t, lgN, k = rf.plan(la, lb, r, d)
+    assert 1 <= t <= 8 and 1 <= lgN <= 14
+    assert _bound_ok(t, lgN, k, la, lb, d)
+    if k > 1:  # one prime fewer must not satisfy the exact bound
+        b = 64 * t
+        na, nb = -(-la // b), -(-lb // b)
+        P = 1
+        for p in rf.primes(k - 1):
+            P *= p
+        assert not 2 * d * min(na, nb) * ((1 << b) - 1) ** 2 < P
+
+
+def test_planner_headline_plan():
+    # 2x2 of 1,048,576-bit entries: 256-bit chunks, 4096 chunks, 2^13 points
+    assert rf.plan(1 << 20, 1 << 20, 2, 2) == (4, 13, 11)
+
+
+def test_unsupported_sizes_are_reported():
+    with pytest.raises(rf.RfError) as e:
+        rf.plan(1 << 24, 1 << 24, 2, 2)
+    assert e.value.code == rf.RF_ERR_UNSUPPORTED
+
+
+def test_out_limbs_formula():
+    assert rf.out_limbs(16384, 16384, 2) == (64 * 32768 + 1 + 1 + 63) // 64
+    assert rf.out_limbs(0, 5, 2) == 0
+    assert rf.out_limbs(1, 1, 1) == 3
+
+
+def test_calls_before_setup_return_not_init():
+    import numpy as np
+    L = rf.lib()
+    z = np.zeros(4, dtype=np.uint64)
+    zs = np.zeros(4, dtype=np.uintp)
+    zi = np.zeros(4, dtype=np.intc)
+    zl = np.zeros(4, dtype=np.int_)
+    u64, sz, ip, lp = rf._u64p, rf._szp, rf._intp, rf._longp
+    rc = L.rf_rmatrix_mult_limbs(z.ctypes.data_as(u64), zs.ctypes.data_as(sz), zs.ctypes.data_as(sz),
+                                 zl.ctypes.data_as(lp), z.ctypes.data_as(u64), zs.ctypes.data_as(sz),
+                                 zs.ctypes.data_as(sz), zi.ctypes.data_as(ip), z.ctypes.data_as(u64),
+                                 zs.ctypes.data_as(sz), zs.ctypes.data_as(sz), zi.ctypes.data_as(ip), 1, 1)
+    if rc == rf.RF_OK:
+        pytest.skip("library already initialised in this process")
+    assert rc == rf.RF_ERR_NOT_INIT
+    assert b"hw_mpz_setup" in L.rf_last_error()

@@ -1,0 +1,141 @@
+"""End-to-end parity of the GPU product with schoolbook GMP-equivalent
+arithmetic (Python integers) and with the reference CPU path, on the
+benchmark configurations and the edge cases the reference's path admits
+(signs, zeros, ragged sizes, r != d, all-ones entries)."""
+import random
+
+import pytest
+
+import oracle
+import rforest_amd as rf
+from helpers import naive
+from rforest_amd import workloads
+from rforest_amd.gmp import MpzArray
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def setup():
+    rf.hw_mpz_setup()
+    yield
+
+
+def _rand_product(rng, bits, r, d, signed=True):
+    s = (lambda: rng.choice([-1, 1])) if signed else (lambda: 1)
+    return ([rng.getrandbits(bits) * s() for _ in range(r * d)],
+            [rng.getrandbits(bits) * s() for _ in range(d * d)])
+
+
+@pytest.mark.parametrize("bits,r,d", [(1, 1, 1), (64, 1, 1), (65, 2, 2), (4096, 2, 2), (50000, 2, 2),
+                                      (3000, 3, 3), (100, 1, 4), (200000, 2, 2), (12345, 2, 5),
+                                      (777, 4, 4), (30000, 5, 1), (100000, 1, 2)])
+def test_product_vs_schoolbook(bits, r, d):
+    rng = random.Random(bits + 100 * r + d)
+    A, B = _rand_product(rng, bits, r, d)
+    assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == naive(A, r, B, d)
+
+
+def test_headline_product_vs_reference_cpu_path():
+    (r, d, A, B), = workloads.headline(1, seed=42)
+    C = rf.mpz_rmatrix_mult_fft(A, r, B, d)
+    assert C == naive(A, r, B, d)
+    if oracle.ref_available():
+        assert C == oracle.ref_rmatrix_mult(A, r, B, d)
+
+
+def test_cpu_reference_case_seed_1001():
+    r, d, A, B = workloads.cpu_reference_case(1001)
+    C = rf.mpz_rmatrix_mult_fft(A, r, B, d)
+    assert C == naive(A, r, B, d)
+    if oracle.ref_available():
+        assert C == oracle.ref_rmatrix_mult(A, r, B, d)
+
+
+def test_plain_gpu_case_4x4_seed_1002():
+    r, d, A, B = workloads.plain_gpu_case(1002)
+    assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == naive(A, r, B, d)
+
+
+def test_zero_and_ragged_entries():
+    rng = random.Random(11)
+    A = [0, rng.getrandbits(9000), -rng.getrandbits(10), 0]
+    B = [rng.getrandbits(100), 0, 0, -rng.getrandbits(8000)]
+    assert rf.mpz_rmatrix_mult_fft(A, 2, B, 2) == naive(A, 2, B, 2)
+    assert rf.mpz_rmatrix_mult_fft([0] * 4, 2, B, 2) == [0] * 4
+    assert rf.mpz_rmatrix_mult_fft(A, 2, [0] * 4, 2) == [0] * 4
+
+
+def test_all_ones_entries_and_cancellation():
+    L = 70000
+    m = (1 << L) - 1
+    A = [m, m, -m, m]
+    B = [m, -m, m, m]
+    assert rf.mpz_rmatrix_mult_fft(A, 2, B, 2) == naive(A, 2, B, 2)  # includes exact zeros
+    A = [m, m, m, m]
+    assert rf.mpz_rmatrix_mult_fft(A, 2, A, 2) == naive(A, 2, A, 2)
+
+
+def test_powers_of_two():
+    A = [1 << 50000, 1, -(1 << 63), 1 << 64]
+    B = [1 << 1, -(1 << 70000), 1, 3]
+    assert rf.mpz_rmatrix_mult_fft(A, 2, B, 2) == naive(A, 2, B, 2)
+
+
+def test_uneven_operand_sizes():
+    rng = random.Random(3)
+    A = [rng.getrandbits(300000) for _ in range(4)]
+    B = [rng.getrandbits(5000) * rng.choice([-1, 1]) for _ in range(4)]
+    assert rf.mpz_rmatrix_mult_fft(A, 2, B, 2) == naive(A, 2, B, 2)
+
+
+def test_dropin_mpz_abi():
+    """mpz_rmatrix_mult_fft through its C prototype on GMP mpz_t arrays."""
+    rng = random.Random(5)
+    for (bits, r, d) in [(5000, 2, 2), (40000, 3, 2), (1, 1, 1)]:
+        A, B = _rand_product(rng, bits, r, d)
+        mA, mB, mC, w = MpzArray(A), MpzArray(B), MpzArray.zeros(r * d), MpzArray.zeros(1)
+        # C entries pre-set to garbage of another size must be overwritten
+        mC.set([12345678901234567890 ** 3] * (r * d))
+        rf.lib().mpz_rmatrix_mult_fft(mC.ptr(), mA.ptr(), r, mB.ptr(), d, w.ptr())
+        assert mC.values() == naive(A, r, B, d)
+        assert rf.lib().mpz_rmatrix_gpu_fft_suitable(mA.ptr(), r, mB.ptr(), d) == 1
+        mC2 = MpzArray.zeros(r * d)
+        rf.lib().cu_mpz_rmatrix_mult_fft(mC2.ptr(), mA.ptr(), r, mB.ptr(), d, w.ptr())
+        assert mC2.values() == naive(A, r, B, d)
+        assert rf.lib().rf_mpz_rmatrix_mult(mC2.ptr(), mA.ptr(), r, mB.ptr(), d) == rf.RF_OK
+
+
+def test_unsupported_size_returns_error():
+    rng = random.Random(1)
+    big = [rng.getrandbits(1 << 24) for _ in range(4)]
+    mA = MpzArray(big)
+    mC = MpzArray.zeros(4)
+    assert rf.lib().mpz_rmatrix_gpu_fft_suitable(mA.ptr(), 2, mA.ptr(), 2) == 0
+    rc = rf.lib().rf_mpz_rmatrix_mult(mC.ptr(), mA.ptr(), 2, mA.ptr(), 2)
+    assert rc == rf.RF_ERR_UNSUPPORTED
+
+
+def test_device_batch_mixed_level_matches_alone():
+    """A product-tree level of uneven products in one call; each result equals
+    the product computed alone and the schoolbook value."""
+    prods = workloads.tree_level(48, seed=1003)
+    prods += [workloads.plain_gpu_case(1002)]
+    batch = rf.DeviceBatch(prods)
+    batch.run()
+    res = batch.results()
+    for (r, d, A, B), C in zip(prods, res):
+        assert C == naive(A, r, B, d)
+    for (r, d, A, B), C in list(zip(prods, res))[:5]:
+        assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == C
+
+
+def test_device_batch_repeat_is_deterministic():
+    prods = workloads.headline(3, seed=42)
+    batch = rf.DeviceBatch(prods)
+    batch.run()
+    first = batch.results()
+    for _ in range(3):
+        batch.run(sync=False)
+    assert batch.results() == first
+    assert first[0] == naive(prods[0][2], 2, prods[0][3], 2)
