@@ -114,6 +114,18 @@ uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d);
    otherwise after completion. */
 int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, int async);
 
+/* A prepared batch: the products, their plans, task tables and scratch
+   memory fixed once, for callers that run the same level shape repeatedly
+   (serving loops, benchmarks, graph replay).  Inputs are read and outputs
+   written at the device addresses given at creation, on every launch. */
+typedef struct rf_batch rf_batch;
+int rf_batch_create(const rf_dev_product* prods, int nprod, rf_batch** out);
+/* Enqueues the whole pipeline on stream (NULL: the library's stream) and
+   returns; with use_graph != 0 the launch sequence is captured once into a
+   hipGraph and replayed.  Completion: rf_device_sync or the stream. */
+int rf_batch_launch(rf_batch* b, void* stream, int use_graph);
+void rf_batch_destroy(rf_batch* b);
+
 /* ---------------------------------------------------------------------------
  * Device and pinned memory (src/include/hwmpz.h:38-46 declares the same
  * helpers for the reference's CUDA path; callers of rf_batch_mult_device use
@@ -138,7 +150,7 @@ int rf_device_sync(void);
 int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k);
 
 /* Copies up to max primes of the library's prime set (all 1 mod 2^24,
-   between 2^47 and 2^48); returns how many exist. */
+   between 2^47 and 2^51 / 9); returns how many exist. */
 int rf_primes(uint64_t* out, int max);
 
 /* Per-kernel timing: when enabled, every launch of the named stage is

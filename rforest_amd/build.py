@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librforest_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rf_kernels.hip", "rf_engine.cpp", "rf_plan.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rf_ntt.hip", "rf_crt.hip", "rf_engine.cpp", "rf_plan.cpp")]
+OBJDIR = os.path.join(ROOT, "_build")
 GMP_PREFIX = os.environ.get("RF_GMP_PREFIX", "/opt/conda")
 
 HIPCC_FLAGS = [
@@ -20,7 +21,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # exactness of the fp64 modular arithmetic relies on explicit FMAs only
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -30,23 +30,37 @@ HIPCC_FLAGS = [
 
 
 def build(verbose=False):
+    """Compiles every source for gfx950 in parallel, then links the library."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + [
-        "-I" + os.path.join(GMP_PREFIX, "include"),
-        "-o", LIB + ".tmp",
-    ] + SOURCES + [
+    os.makedirs(OBJDIR, exist_ok=True)
+    inc = ["-I" + os.path.join(GMP_PREFIX, "include")]
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmd = [hipcc] + HIPCC_FLAGS + inc + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = False
+    for src, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed compiling librforest_hip.so sources")
+    link = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + [
         # gmp.h comes from GMP_PREFIX; the library is linked by soname only
         # (no run path into GMP_PREFIX, whose older libstdc++ would shadow
         # the one libamdhip64 needs), so the system libgmp.so.10 serves it.
         "-L" + os.path.join(GMP_PREFIX, "lib"),
         "-l:libgmp.so.10",
     ]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    res = subprocess.run(link, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building librforest_hip.so")
+        raise RuntimeError("linking librforest_hip.so failed")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -1,158 +1,18 @@
-// rf_kernels.hip -- gfx950 kernels of the exact matrix-product pipeline.
+// rf_crt.hip -- Garner CRT and recomposition of the product pipeline (gfx950).
 //
-//   k_fwd     pack (limb chunks -> residues, sign and 1/N folded in) + forward
-//             number-theoretic transform, natural order in, bit-reversed out
-//   k_inv     pointwise r x d . d x d product per frequency + inverse
-//             transform, bit-reversed in, natural order out
-//   k_crt     Garner CRT of each coefficient + digit sums of the overlapping
-//             coefficients + block scan of carry transfer functions
-//   k_carry   per C entry: carry into each block from the block aggregates
-//   k_apply   adds the resolved carries into the output limbs
+//   k_crt    Garner CRT of each coefficient + digit sums of the overlapping
+//            coefficients + block scan of carry transfer functions
+//   k_carry  per C entry: the carry into each block from the block aggregates
+//   k_apply  adds the resolved carries into the output limbs
 //
-// Transform definition (as src/include/fft62.h:35-54): with omega of order N,
-// forward output position m holds sum_i a_i omega^(i rev(m)).  Both
-// directions use Cooley-Tukey butterflies (mulmod then add/sub), so a value
-// grows by at most p per stage; every fourth stage reduces, which keeps all
-// mulmod inputs under 4.5 p + 8 < 2^51.
+// The carry-propagating sum of src/lib/recompose.c:11-107 and the serial
+// fix-up of src/lib/crt_recompose.c:103-153 become a segmented scan: every
+// output digit's carry behaviour is a function {-1, 0, 1} -> {-1, 0, 1} of its
+// incoming carry, and such functions compose associatively.
 #include "rf_field.h"
 #include "rf_kernels.h"
 
-#define RF_NTT_THREADS 512
-
 namespace {
-
-// Workgroup ids are dispatched round-robin over the 8 XCDs; remap so that
-// consecutive tasks (same entry's primes, or same prime's entries) land on
-// the same XCD and share its L2.  grid is a multiple of 8.
-__device__ __forceinline__ int xcd_task(int bid, int grid, int ntasks) {
-  int per = grid >> 3;
-  int t = (bid & 7) * per + (bid >> 3);
-  return t < ntasks ? t : -1;
-}
-
-// Residue of chunk i (t limbs) of an entry, scaled by the pack constants
-// pk[m] = 2^(24 m) S mod p split in 24-bit halves.  Pieces g_m < 2^24, so
-// each product g * half < 2^48 and the sums of at most 22 terms stay below
-// 2^53: exact.  Then s_hi 2^24 + s_lo is reduced (inputs < 2^53).
-__device__ __forceinline__ double pack_chunk(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
-                                             uint32_t i, int t, const double (*pk)[2], double p,
-                                             double pinv) {
-  uint64_t c[RF_TMAX];
-  uint64_t base = (uint64_t)i * t;
-#pragma unroll
-  for (int l = 0; l < RF_TMAX; l++) c[l] = (l < t && base + l < nlimbs) ? limbs[base + l] : 0;
-  const int np = (64 * t + RF_PIECE - 1) / RF_PIECE;
-  double shi = 0.0, slo = 0.0;
-#pragma unroll
-  for (int m = 0; m < RF_MAXPIECES; m++) {
-    if (m < np) {
-      const int bit = RF_PIECE * m, li = bit >> 6, sh = bit & 63;
-      uint64_t g = c[li] >> sh;
-      if (sh > 64 - RF_PIECE && li + 1 < RF_TMAX) g |= c[li + 1] << (64 - sh);
-      g &= (1u << RF_PIECE) - 1;
-      double gd = (double)(uint32_t)g;
-      shi = __builtin_fma(gd, pk[m][0], shi);
-      slo = __builtin_fma(gd, pk[m][1], slo);
-    }
-  }
-  double r = rf_mulmod(shi, 16777216.0, 16777216.0 * pinv, p) + slo;
-  return rf_reduce(r, p, pinv);
-}
-
-template <int LGN>
-__global__ __launch_bounds__(RF_NTT_THREADS) void k_fwd(RfDevTables tb,
-                                                        const RfFwdTask* __restrict__ tasks,
-                                                        int ntasks,
-                                                        double* __restrict__ spec) {
-  constexpr int N = 1 << LGN;
-  extern __shared__ double sh[];
-  const int tid = threadIdx.x, T = blockDim.x;
-  const int ti = xcd_task(blockIdx.x, gridDim.x, ntasks);
-  if (ti < 0) return;
-  const RfFwdTask tk = tasks[ti];
-  const RfConsts* C = tb.consts;
-  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double(*pk)[2] = C->pk[tk.q][tk.scale];
-  for (int i = tid; i < N; i += T) {
-    double v = 0.0;
-    if ((uint32_t)i < tk.nchunk && tk.sign != 0) {
-      v = pack_chunk(tk.limbs, tk.nlimbs, (uint32_t)i, tk.t, pk, p, pinv);
-      if (tk.sign < 0) v = -v;
-    }
-    sh[i] = v;
-  }
-  __syncthreads();
-  const double* tw = tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2;
-#pragma unroll 1
-  for (int L = 0; L < LGN; L++) {
-    const int lh = LGN - 1 - L;  // log2 of the half block length
-    for (int bi = tid; bi < N / 2; bi += T) {
-      const int m = bi >> lh, j = bi & ((1 << lh) - 1);
-      const int i0 = (m << (lh + 1)) + j, i1 = i0 + (1 << lh);
-      const double w = tw[2 * m], wq = tw[2 * m + 1];
-      const double x = sh[i0], y = sh[i1];
-      const double z = rf_mulmod(y, w, wq, p);
-      double u = x + z, v = x - z;
-      if ((L & 3) == 3) {
-        u = rf_reduce(u, p, pinv);
-        v = rf_reduce(v, p, pinv);
-      }
-      sh[i0] = u;
-      sh[i1] = v;
-    }
-    __syncthreads();
-  }
-  double* out = spec + tk.spec_off;
-  for (int i = tid; i < N; i += T) out[i] = rf_reduce(sh[i], p, pinv);
-}
-
-template <int LGN>
-__global__ __launch_bounds__(RF_NTT_THREADS) void k_inv(RfDevTables tb,
-                                                        const RfInvTask* __restrict__ tasks,
-                                                        int ntasks, const double* __restrict__ spec,
-                                                        double* __restrict__ coef) {
-  constexpr int N = 1 << LGN;
-  extern __shared__ double sh[];
-  const int tid = threadIdx.x, T = blockDim.x;
-  const int ti = xcd_task(blockIdx.x, gridDim.x, ntasks);
-  if (ti < 0) return;
-  const RfInvTask tk = tasks[ti];
-  const RfConsts* C = tb.consts;
-  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  for (int i = tid; i < N; i += T) {
-    double acc = 0.0;
-    for (int l = 0; l < tk.d; l++) {
-      const double a = spec[tk.a_off + (size_t)l * tk.a_stride + i];
-      const double b = spec[tk.b_off + (size_t)l * tk.b_stride + i];
-      acc += rf_mulmod_var(a, b, p, pinv);  // |term| <= p/2 + 1
-      if ((l & 7) == 7) acc = rf_reduce(acc, p, pinv);
-    }
-    sh[i] = rf_reduce(acc, p, pinv);
-  }
-  __syncthreads();
-  const double* tw = tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2;
-#pragma unroll 1
-  for (int L = 0; L < LGN; L++) {
-    const int len = 1 << L;
-    for (int bi = tid; bi < N / 2; bi += T) {
-      const int blk = bi >> L, j = bi & (len - 1);
-      const int i0 = (blk << (L + 1)) + j, i1 = i0 + len;
-      const double w = tw[2 * (len + j)], wq = tw[2 * (len + j) + 1];
-      const double x = sh[i0], y = sh[i1];
-      const double z = rf_mulmod(y, w, wq, p);
-      double u = x + z, v = x - z;
-      if ((L & 3) == 3) {
-        u = rf_reduce(u, p, pinv);
-        v = rf_reduce(v, p, pinv);
-      }
-      sh[i0] = u;
-      sh[i1] = v;
-    }
-    __syncthreads();
-  }
-  double* out = coef + tk.out_off;
-  for (int i = tid; i < N; i += T) out[i] = rf_reduce(sh[i], p, pinv);
-}
 
 // ---- CRT, digit sums and carry transfer functions ----
 //
@@ -381,28 +241,6 @@ __global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_apply(const RfCrtTask* 
   }
 }
 
-template <int LGN>
-hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntasks, double* spec,
-                        hipStream_t st) {
-  constexpr int N = 1 << LGN;
-  const int T = N / 2 < 64 ? 64 : (N / 2 > RF_NTT_THREADS ? RF_NTT_THREADS : N / 2);
-  const int grid = (ntasks + 7) & ~7;
-  hipLaunchKernelGGL(k_fwd<LGN>, dim3(grid), dim3(T), N * sizeof(double), st, tb, tasks, ntasks,
-                     spec);
-  return hipGetLastError();
-}
-
-template <int LGN>
-hipError_t launch_inv_t(const RfDevTables& tb, const RfInvTask* tasks, int ntasks,
-                        const double* spec, double* coef, hipStream_t st) {
-  constexpr int N = 1 << LGN;
-  const int T = N / 2 < 64 ? 64 : (N / 2 > RF_NTT_THREADS ? RF_NTT_THREADS : N / 2);
-  const int grid = (ntasks + 7) & ~7;
-  hipLaunchKernelGGL(k_inv<LGN>, dim3(grid), dim3(T), N * sizeof(double), st, tb, tasks, ntasks,
-                     spec, coef);
-  return hipGetLastError();
-}
-
 template <int K>
 hipError_t launch_crt_t(const RfDevTables& tb, const RfCrtTask* tasks, int ntasks,
                         const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st) {
@@ -411,56 +249,7 @@ hipError_t launch_crt_t(const RfDevTables& tb, const RfCrtTask* tasks, int ntask
   return hipGetLastError();
 }
 
-bool g_lds_attr_set = false;
-
-template <int LGN>
-void set_lds_attr() {
-  constexpr int bytes = (1 << LGN) * sizeof(double);
-  if (bytes > 65536) {
-    hipFuncSetAttribute((const void*)k_fwd<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_inv<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  }
-}
-
-void ensure_lds_attrs() {
-  if (g_lds_attr_set) return;
-  set_lds_attr<14>();
-  g_lds_attr_set = true;
-}
-
 }  // namespace
-
-hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
-                         double* spec, hipStream_t st) {
-  if (ntasks <= 0) return hipSuccess;
-  ensure_lds_attrs();
-  switch (lgN) {
-#define RF_CASE(L) \
-  case L:          \
-    return launch_fwd_t<L>(tb, tasks, ntasks, spec, st);
-    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
-    RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
-#undef RF_CASE
-    default:
-      return hipErrorInvalidValue;
-  }
-}
-
-hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
-                         const double* spec, double* coef, hipStream_t st) {
-  if (ntasks <= 0) return hipSuccess;
-  ensure_lds_attrs();
-  switch (lgN) {
-#define RF_CASE(L) \
-  case L:          \
-    return launch_inv_t<L>(tb, tasks, ntasks, spec, coef, st);
-    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
-    RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
-#undef RF_CASE
-    default:
-      return hipErrorInvalidValue;
-  }
-}
 
 hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, int ntasks,
                          const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st) {

@@ -165,13 +165,13 @@ struct ProfScope {
   hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(int s, hipStream_t stream) : stage(s), st(stream) {
-    if (!g.prof) return;
+    if (!g.prof || stage < 0) return;
     hipEventCreate(&a);
     hipEventCreate(&b);
     hipEventRecord(a, st);
   }
   ~ProfScope() {
-    if (!g.prof) return;
+    if (!g.prof || stage < 0) return;
     hipEventRecord(b, st);
     g.pending.push_back({stage, a, b});
   }
@@ -240,6 +240,8 @@ uint32_t crt_blocks(const RfPlan& pl) {
 
 Layout layout_of(std::vector<Product>& prods) {
   Layout L;
+  // task counts per launch group, as build_pipeline groups them
+  std::map<int, size_t> fwd_n, inv_n, crt_n;
   for (auto& P : prods) {
     const RfPlan& pl = P.plan;
     P.spec_off = L.spec_n;
@@ -249,26 +251,54 @@ Layout layout_of(std::vector<Product>& prods) {
     const int rd = P.r * P.d, dd = P.d * P.d;
     L.spec_n += (uint64_t)(rd + dd) * pl.k * N;
     L.coef_n += (uint64_t)rd * pl.k * N;
-    L.nfwd += (size_t)(rd + dd) * pl.k;
-    L.ninv += (size_t)rd * pl.k;
-    L.ncrt += (size_t)rd * crt_blocks(pl);
+    fwd_n[pl.lgN] += (size_t)(rd + dd) * pl.k;
+    inv_n[pl.lgN] += (size_t)rd * pl.k;
+    crt_n[pl.k] += (size_t)rd * crt_blocks(pl);
     L.ncarry += (size_t)rd;
     L.nblk += (uint32_t)(rd * crt_blocks(pl));
   }
   L.spec_b = align256(L.spec_n * 8);
   L.coef_b = align256(L.coef_n * 8);
-  L.tasks_b = align256(L.nfwd * sizeof(RfFwdTask)) + align256(L.ninv * sizeof(RfInvTask)) +
-              align256(L.ncrt * sizeof(RfCrtTask)) + align256(L.ncarry * sizeof(RfCarryTask));
+  size_t tb = 0;
+  for (auto& kv : fwd_n) tb += align256(kv.second * sizeof(RfFwdTask)), L.nfwd += kv.second;
+  for (auto& kv : inv_n) tb += align256(kv.second * sizeof(RfInvTask)), L.ninv += kv.second;
+  for (auto& kv : crt_n) tb += align256(kv.second * sizeof(RfCrtTask)), L.ncrt += kv.second;
+  tb += align256(L.ncarry * sizeof(RfCarryTask));
+  L.tasks_b = tb;
   L.pre_b = align256((size_t)L.nblk * RF_DIGITS_PER_BLOCK);
   L.fun_b = align256(L.nblk);
   return L;
 }
 
-// Enqueue the pipeline for planned products whose inputs are at
-// Entry::dptr and outputs at Product::out.  The arena must already hold
-// base + L.total() bytes; scratch starts at offset base.
-int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipStream_t st) {
-  if (g.arena_cap < base + L.total()) return fail(RF_ERR_NOMEM, "arena too small for pipeline");
+// A built pipeline: task tables in one host byte image (uploaded to the
+// scratch region's task area) and the launch sequence over them.  Task
+// tables hold absolute device pointers for limbs and outputs and offsets
+// into the scratch region for spectra and coefficients, so one image serves
+// every launch of the same batch.
+struct LaunchRec {
+  int stage;   // RF_STAGE_*
+  int param;   // lgN for transforms, k for CRT
+  size_t off;  // byte offset of its task table in the image
+  int n;       // tasks
+};
+struct Pipeline {
+  Layout L;
+  std::vector<char> image;
+  std::vector<LaunchRec> launches;
+};
+
+template <typename T>
+size_t append_tasks(std::vector<char>& img, const std::vector<T>& v) {
+  size_t off = align256(img.size());
+  img.resize(off + v.size() * sizeof(T));
+  if (!v.empty()) memcpy(img.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
+}
+
+int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
+  pp.L = layout_of(prods);
+  pp.image.clear();
+  pp.launches.clear();
   std::map<int, std::vector<RfFwdTask>> fwd;
   std::map<int, std::vector<RfInvTask>> inv;
   std::map<int, std::vector<RfCrtTask>> crt;
@@ -331,71 +361,89 @@ int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipS
       nblk_total += nblk;
     }
   }
-  char* A = g.d_arena + base;
-  double* d_spec = (double*)A;
-  double* d_coef = (double*)(A + L.spec_b);
-  char* d_tasks = A + L.spec_b + L.coef_b;
-  uint8_t* d_pre = (uint8_t*)(d_tasks + L.tasks_b);
-  uint8_t* d_fun = d_pre + L.pre_b;
-  int8_t* d_cin = (int8_t*)(d_fun + L.fun_b);
-  int rc = ensure_stage(L.tasks_b);
+  for (auto& kv : fwd)
+    pp.launches.push_back({RF_STAGE_FWD, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
+  for (auto& kv : inv)
+    pp.launches.push_back({RF_STAGE_INV, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
+  std::vector<LaunchRec> crt_l;
+  for (auto& kv : crt) {
+    size_t off = append_tasks(pp.image, kv.second);
+    pp.launches.push_back({RF_STAGE_CRT, kv.first, off, (int)kv.second.size()});
+    crt_l.push_back({RF_STAGE_APPLY, kv.first, off, (int)kv.second.size()});
+  }
+  if (!carry.empty())
+    pp.launches.push_back({RF_STAGE_CARRY, 0, append_tasks(pp.image, carry), (int)carry.size()});
+  for (auto& l : crt_l) pp.launches.push_back(l);
+  if (pp.image.size() > pp.L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
+  return RF_OK;
+}
+
+struct ScratchPtrs {
+  double* spec;
+  double* coef;
+  char* tasks;
+  uint8_t* pre;
+  uint8_t* fun;
+  int8_t* cin;
+};
+
+ScratchPtrs scratch_ptrs(char* base, const Layout& L) {
+  ScratchPtrs s;
+  s.spec = (double*)base;
+  s.coef = (double*)(base + L.spec_b);
+  s.tasks = base + L.spec_b + L.coef_b;
+  s.pre = (uint8_t*)(s.tasks + L.tasks_b);
+  s.fun = s.pre + L.pre_b;
+  s.cin = (int8_t*)(s.fun + L.fun_b);
+  return s;
+}
+
+// Enqueues every kernel of a built pipeline whose task image already sits in
+// the scratch region at `base`.  No host synchronisation, no allocation.
+int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
+  const ScratchPtrs S = scratch_ptrs(base, pp.L);
+  for (const LaunchRec& l : pp.launches) {
+    ProfScope ps(prof ? l.stage : -1, st);
+    const char* t = S.tasks + l.off;
+    switch (l.stage) {
+      case RF_STAGE_FWD:
+        HIPCHK(rf_launch_fwd(g.tables, l.param, (const RfFwdTask*)t, l.n, S.spec, st));
+        break;
+      case RF_STAGE_INV:
+        HIPCHK(rf_launch_inv(g.tables, l.param, (const RfInvTask*)t, l.n, S.spec, S.coef, st));
+        break;
+      case RF_STAGE_CRT:
+        HIPCHK(rf_launch_crt(g.tables, l.param, (const RfCrtTask*)t, l.n, S.coef, S.pre, S.fun, st));
+        break;
+      case RF_STAGE_CARRY:
+        HIPCHK(rf_launch_carry((const RfCarryTask*)t, l.n, S.fun, S.cin, st));
+        break;
+      case RF_STAGE_APPLY:
+        HIPCHK(rf_launch_apply((const RfCrtTask*)t, l.n, S.pre, S.cin, st));
+        break;
+    }
+  }
+  return RF_OK;
+}
+
+// Build, upload the task image through the pinned staging buffer, launch.
+// The arena must hold base + L.total() bytes (scratch starts at base).
+int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipStream_t st) {
+  if (g.arena_cap < base + L.total()) return fail(RF_ERR_NOMEM, "arena too small for pipeline");
+  Pipeline pp;
+  int rc = build_pipeline(prods, pp);
+  if (rc) return rc;
+  char* sbase = g.d_arena + base;
+  const ScratchPtrs S = scratch_ptrs(sbase, pp.L);
+  rc = ensure_stage(pp.image.size());
   if (rc) return rc;
   // Earlier copies out of the staging buffer must finish before it is reused.
   HIPCHK(hipStreamSynchronize(st));
-  char* h = g.h_stage;
-  size_t o = 0;
-  struct Range {
-    size_t off, n;
-  };
-  std::vector<std::pair<int, Range>> fwd_r, inv_r, crt_r;
-  for (auto& kv : fwd) {
-    fwd_r.push_back({kv.first, {o, kv.second.size()}});
-    memcpy(h + o, kv.second.data(), kv.second.size() * sizeof(RfFwdTask));
-    o += kv.second.size() * sizeof(RfFwdTask);
+  if (!pp.image.empty()) {
+    memcpy(g.h_stage, pp.image.data(), pp.image.size());
+    HIPCHK(hipMemcpyAsync(S.tasks, g.h_stage, pp.image.size(), hipMemcpyHostToDevice, st));
   }
-  o = align256(o);
-  for (auto& kv : inv) {
-    inv_r.push_back({kv.first, {o, kv.second.size()}});
-    memcpy(h + o, kv.second.data(), kv.second.size() * sizeof(RfInvTask));
-    o += kv.second.size() * sizeof(RfInvTask);
-  }
-  o = align256(o);
-  for (auto& kv : crt) {
-    crt_r.push_back({kv.first, {o, kv.second.size()}});
-    memcpy(h + o, kv.second.data(), kv.second.size() * sizeof(RfCrtTask));
-    o += kv.second.size() * sizeof(RfCrtTask);
-  }
-  o = align256(o);
-  const size_t carry_o = o;
-  memcpy(h + o, carry.data(), carry.size() * sizeof(RfCarryTask));
-  o += carry.size() * sizeof(RfCarryTask);
-  if (o > L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
-  if (o) HIPCHK(hipMemcpyAsync(d_tasks, h, o, hipMemcpyHostToDevice, st));
-  for (auto& kv : fwd_r) {
-    ProfScope ps(RF_STAGE_FWD, st);
-    HIPCHK(rf_launch_fwd(g.tables, kv.first, (const RfFwdTask*)(d_tasks + kv.second.off),
-                         (int)kv.second.n, d_spec, st));
-  }
-  for (auto& kv : inv_r) {
-    ProfScope ps(RF_STAGE_INV, st);
-    HIPCHK(rf_launch_inv(g.tables, kv.first, (const RfInvTask*)(d_tasks + kv.second.off),
-                         (int)kv.second.n, d_spec, d_coef, st));
-  }
-  for (auto& kv : crt_r) {
-    ProfScope ps(RF_STAGE_CRT, st);
-    HIPCHK(rf_launch_crt(g.tables, kv.first, (const RfCrtTask*)(d_tasks + kv.second.off),
-                         (int)kv.second.n, d_coef, d_pre, d_fun, st));
-  }
-  if (!carry.empty()) {
-    ProfScope ps(RF_STAGE_CARRY, st);
-    HIPCHK(rf_launch_carry((const RfCarryTask*)(d_tasks + carry_o), (int)carry.size(), d_fun, d_cin, st));
-  }
-  for (auto& kv : crt_r) {
-    ProfScope ps(RF_STAGE_APPLY, st);
-    HIPCHK(rf_launch_apply((const RfCrtTask*)(d_tasks + kv.second.off), (int)kv.second.n, d_pre,
-                           d_cin, st));
-  }
-  return RF_OK;
+  return launch_pipeline(pp, sbase, st, g.prof);
 }
 
 uint64_t bitlen_limbs(const uint64_t* x, uint64_t n) {
@@ -599,52 +647,60 @@ uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d) {
   return out_limbs_for(64 * la_limbs, 64 * lb_limbs, d);
 }
 
-int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, int async) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+}  // extern "C"
+
+namespace {
+int products_from_dev(const rf_dev_product* prods, int nprod, std::vector<Product>& ps) {
   if (nprod < 0 || (nprod > 0 && !prods)) return fail(RF_ERR_ARG, "bad product list");
-  hipStream_t st = stream ? (hipStream_t)stream : g.stream;
-  std::vector<Product> ps(nprod);
+  ps.assign(nprod, Product());
   for (int i = 0; i < nprod; i++) {
     const rf_dev_product& dp = prods[i];
     Product& P = ps[i];
     P.r = dp.r;
     P.d = dp.d;
-    if (P.r <= 0 || P.d <= 0) return fail(RF_ERR_ARG, "product %d: bad dimensions", i);
-    uint64_t la = 0, lb = 0;
-    for (int e = 0; e < P.r * P.d; e++) {
-      const rf_dev_entry& x = dp.A[e];
+    if (P.r <= 0 || P.d <= 0 || !dp.A || !dp.B || !dp.C) return fail(RF_ERR_ARG, "product %d: bad descriptor", i);
+    for (int e = 0; e < P.r * P.d + P.d * P.d; e++) {
+      const rf_dev_entry& x = e < P.r * P.d ? dp.A[e] : dp.B[e - P.r * P.d];
       Entry E{};
       E.dptr = x.limbs;
       E.nlimbs = x.nlimbs;
-      E.sign = x.nlimbs ? x.sign : 0;
+      E.sign = x.nlimbs ? (x.sign > 0 ? 1 : (x.sign < 0 ? -1 : 0)) : 0;
       E.bits = E.sign ? 64 * x.nlimbs : 0;
-      la = std::max(la, E.sign ? x.nlimbs : 0);
-      P.A.push_back(E);
-    }
-    for (int e = 0; e < P.d * P.d; e++) {
-      const rf_dev_entry& x = dp.B[e];
-      Entry E{};
-      E.dptr = x.limbs;
-      E.nlimbs = x.nlimbs;
-      E.sign = x.nlimbs ? x.sign : 0;
-      E.bits = E.sign ? 64 * x.nlimbs : 0;
-      lb = std::max(lb, E.sign ? x.nlimbs : 0);
-      P.B.push_back(E);
+      (e < P.r * P.d ? P.A : P.B).push_back(E);
     }
     int rc = plan_product(P);
     if (rc) return rc;
-    uint64_t rn = P.plan.k ? P.plan.rn : 0;
+    const uint64_t rn = P.plan.k ? P.plan.rn : 0;  // a zero product writes no limbs
     for (int e = 0; e < P.r * P.d; e++) {
-      if (dp.C[e].cap < rn) return fail(RF_ERR_CAPACITY, "product %d output %d needs %llu limbs", i, e, (unsigned long long)rn);
+      if (dp.C[e].cap < rn)
+        return fail(RF_ERR_CAPACITY, "product %d output %d needs %llu limbs", i, e, (unsigned long long)rn);
       P.out.push_back(dp.C[e].limbs);
-      if (rn == 0 && dp.C[e].cap) HIPCHK(hipMemsetAsync(dp.C[e].limbs, 0, dp.C[e].cap * 8, st));
     }
-    (void)la;
-    (void)lb;
   }
+  return RF_OK;
+}
+}  // namespace
+
+struct rf_batch {
+  std::vector<Product> prods;
+  Pipeline pp;
+  char* scratch = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+extern "C" {
+
+int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, int async) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  hipStream_t st = stream ? (hipStream_t)stream : g.stream;
+  std::vector<Product> ps;
+  int rc = products_from_dev(prods, nprod, ps);
+  if (rc) return rc;
   const Layout L = layout_of(ps);
-  int rc = ensure_arena(L.total());
+  rc = ensure_arena(L.total());
   if (rc) return rc;
   rc = run_pipeline(ps, L, 0, st);
   if (rc) return rc;
@@ -653,6 +709,79 @@ int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, i
     if (g.prof) prof_collect();
   }
   return RF_OK;
+}
+
+int rf_batch_create(const rf_dev_product* prods, int nprod, rf_batch** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!out) return fail(RF_ERR_ARG, "rf_batch_create: null output");
+  *out = nullptr;
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  rf_batch* b = new rf_batch();
+  int rc = products_from_dev(prods, nprod, b->prods);
+  if (!rc) {
+    b->pp.L = layout_of(b->prods);
+    rc = build_pipeline(b->prods, b->pp);
+  }
+  if (!rc) {
+    hipError_t e = hipMalloc((void**)&b->scratch, std::max<size_t>(b->pp.L.total(), 256));
+    if (e != hipSuccess) rc = fail(RF_ERR_NOMEM, "batch scratch of %zu bytes: %s", b->pp.L.total(), hipGetErrorString(e));
+  }
+  if (!rc && !b->pp.image.empty()) {
+    const ScratchPtrs S = scratch_ptrs(b->scratch, b->pp.L);
+    hipError_t e = hipMemcpy(S.tasks, b->pp.image.data(), b->pp.image.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(RF_ERR_HIP, "task upload: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    if (b->scratch) (void)hipFree(b->scratch);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return RF_OK;
+}
+
+int rf_batch_launch(rf_batch* b, void* stream, int use_graph) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!b) return fail(RF_ERR_ARG, "rf_batch_launch: null batch");
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  hipStream_t st = stream ? (hipStream_t)stream : g.stream;
+  if (!use_graph || g.prof) return launch_pipeline(b->pp, b->scratch, st, g.prof);
+  if (b->exec && b->graph_stream != st) {
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  if (!b->exec) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = launch_pipeline(b->pp, b->scratch, st, false);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(RF_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(gr);
+      return fail(RF_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    b->graph = gr;
+    b->graph_stream = st;
+  }
+  HIPCHK(hipGraphLaunch(b->exec, st));
+  return RF_OK;
+}
+
+void rf_batch_destroy(rf_batch* b) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!b) return;
+  if (g.ready) (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->scratch) (void)hipFree(b->scratch);
+  delete b;
 }
 
 int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k) {

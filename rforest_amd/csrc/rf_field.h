@@ -1,4 +1,4 @@
-// rf_field.h -- exact modular arithmetic on fp64 lanes for primes 2^47 < p < 2^48.
+// rf_field.h -- exact modular arithmetic on fp64 lanes for primes 2^47 < p < 2^51 / 9.
 //
 // gfx950 issues v_fma_f64, v_mul_f64 and v_rndne_f64 at full VALU rate, so a
 // residue of ~48 bits costs about six full-rate instructions per modular

@@ -1,0 +1,383 @@
+// rf_ntt.hip -- number-theoretic transforms of the product pipeline (gfx950).
+//
+//   k_fwd  pack (limb chunks -> residues, sign and 1/N folded into the pack
+//          constants) + forward transform, natural order in, bit-reversed out
+//   k_inv  pointwise r x d . d x d product per frequency + inverse transform,
+//          bit-reversed in, natural order out (unscaled: the 1/N sits in the
+//          B operands' pack constants)
+//
+// Transform definition (as src/include/fft62.h:35-54): with omega of order N,
+// forward output position m holds sum_i a_i omega^(i rev(m)).
+//
+// Organisation.  Each thread keeps E = 16 residues in registers.  A pass runs
+// R <= 4 butterfly stages on groups of 2^R residues whose indices differ only
+// in R adjacent bits; between passes the workgroup exchanges residues through
+// LDS (one write and one read of every residue, padded by one double per 32
+// so that strided accesses spread over the banks).  An N = 2^13 transform is
+// four passes (4 + 4 + 4 + 1 stages) and three LDS exchanges.  Transforms
+// smaller than a wave share a workgroup.
+//
+// Both directions use Cooley-Tukey butterflies (x, y) -> (x + w y, x - w y):
+// a residue grows by at most p per stage, and every eighth stage reduces, so
+// all rf_mulmod inputs stay below 8.5 p + 8 < 2^51 (primes < 2^51 / 9).
+#include "rf_field.h"
+#include "rf_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ int xcd_task(int bid, int grid, int ntasks) {
+  // Workgroup ids are dispatched round-robin over the 8 XCDs; remap so that
+  // consecutive tasks (one entry's primes, or one prime's entries) share an
+  // XCD and its L2.  grid is a multiple of 8.
+  const int per = grid >> 3;
+  const int t = (bid & 7) * per + (bid >> 3);
+  return t < ntasks ? t : -1;
+}
+
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+
+template <int LGN>
+struct Shape {
+  static constexpr int N = 1 << LGN;
+  static constexpr int E = LGN >= 4 ? 16 : N;          // residues per thread
+  static constexpr int T = N / E;                      // threads per transform
+  static constexpr int G = T >= 64 ? 1 : 64 / T;       // transforms per workgroup
+  static constexpr int NP = (LGN + 3) / 4;             // passes
+  static constexpr int LDSN = N + (N >> 5);            // padded doubles per transform
+  static constexpr int THREADS = T * G;
+};
+
+template <int LGN, int P>
+struct Pass {
+  static constexpr int L0 = 4 * P;                           // first stage of the pass
+  static constexpr int R = (LGN - L0) < 4 ? (LGN - L0) : 4;  // stages in the pass
+  static constexpr int NG = Shape<LGN>::E >> R;              // groups per thread
+  static constexpr int GS = 1 << R;                          // group size
+  // Reduce before this pass if the stages since the last reduction would
+  // exceed eight.  Passes start after a reduction at P = 0.
+  static constexpr int SINCE = (P == 0) ? 0 : (Pass<LGN, P - 1>::SINCE_AFTER);
+  static constexpr bool REDUCE_BEFORE = SINCE + R > 8;
+  static constexpr int SINCE_AFTER = (REDUCE_BEFORE ? 0 : SINCE) + R;
+};
+template <int LGN>
+struct Pass<LGN, -1> {
+  static constexpr int SINCE_AFTER = 0;
+};
+
+// Forward group indexing: stages L0 .. L0+R-1 act on index bits
+// LGN-1-L0 .. LGN-L0-R.  Group g = (H, lo): i = H 2^(LGN-L0) + j 2^NLO + lo.
+template <int LGN, int P>
+__device__ __forceinline__ int fwd_index(int g, int j) {
+  using Q = Pass<LGN, P>;
+  constexpr int NLO = LGN - Q::L0 - Q::R;
+  const int H = g >> NLO, lo = g & ((1 << NLO) - 1);
+  return (H << (LGN - Q::L0)) | (j << NLO) | lo;
+}
+
+// Inverse group indexing: stages L0 .. L0+R-1 act on index bits L0 .. L0+R-1.
+template <int LGN, int P>
+__device__ __forceinline__ int inv_index(int g, int j) {
+  using Q = Pass<LGN, P>;
+  const int hi = g >> Q::L0, lo = g & ((1 << Q::L0) - 1);
+  return (hi << (Q::L0 + Q::R)) | (j << Q::L0) | lo;
+}
+
+// R forward stages on NG groups held in v[u * 2^R + j].  At stage L0 + s the
+// pair (j, j + 2^(R-1-s)) of group (H, lo) uses twiddle T[H 2^s + (j >> (R-s))]
+// (T[m] = omega^(rev(m)), shared prefix for all lengths).
+template <int LGN, int P>
+__device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __restrict__ tw,
+                                         double p) {
+  using Q = Pass<LGN, P>;
+  constexpr int NLO = LGN - Q::L0 - Q::R;
+#pragma unroll
+  for (int s = 0; s < Q::R; s++) {
+    const int half = 1 << (Q::R - 1 - s);
+#pragma unroll
+    for (int u = 0; u < Q::NG; u++) {
+      const int g = tid + Shape<LGN>::T * u;
+      const int H = g >> NLO;
+#pragma unroll
+      for (int b = 0; b < (1 << s); b++) {
+        const double2 w = tw[(H << s) + b];
+#pragma unroll
+        for (int jj = 0; jj < half; jj++) {
+          const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
+          const double z = rf_mulmod(v[j1], w.x, w.y, p);
+          const double x = v[j0];
+          v[j0] = x + z;
+          v[j1] = x - z;
+        }
+      }
+    }
+  }
+}
+
+// R inverse (decimation-in-time) stages: at stage L = L0 + s the pair
+// (j, j + 2^s) uses V[2^L + (j mod 2^s) 2^L0 + lo] = omega_{2^(L+1)}^-(i mod 2^L).
+template <int LGN, int P>
+__device__ __forceinline__ void inv_pass(double* v, int tid, const double2* __restrict__ tw,
+                                         double p) {
+  using Q = Pass<LGN, P>;
+#pragma unroll
+  for (int s = 0; s < Q::R; s++) {
+    const int half = 1 << s;
+    const int len = 1 << (Q::L0 + s);
+#pragma unroll
+    for (int u = 0; u < Q::NG; u++) {
+      const int g = tid + Shape<LGN>::T * u;
+      const int lo = g & ((1 << Q::L0) - 1);
+#pragma unroll
+      for (int jj = 0; jj < half; jj++) {
+        const double2 w = tw[len + (jj << Q::L0) + lo];
+#pragma unroll
+        for (int b = 0; b < (Q::GS >> (s + 1)); b++) {
+          const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
+          const double z = rf_mulmod(v[j1], w.x, w.y, p);
+          const double x = v[j0];
+          v[j0] = x + z;
+          v[j1] = x - z;
+        }
+      }
+    }
+  }
+}
+
+template <int E>
+__device__ __forceinline__ void reduce_all(double* v, double p, double pinv) {
+#pragma unroll
+  for (int e = 0; e < E; e++) v[e] = rf_reduce(v[e], p, pinv);
+}
+
+// Passes 1 .. NP-1 of the forward transform: LDS exchange, optional
+// reduction, stages.
+template <int LGN, int P>
+__device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const double2* tw,
+                                         double p, double pinv) {
+  if constexpr (P < Shape<LGN>::NP) {
+    using Q = Pass<LGN, P>;
+    using Qp = Pass<LGN, P - 1>;
+    if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
+#pragma unroll
+    for (int u = 0; u < Qp::NG; u++)
+#pragma unroll
+      for (int j = 0; j < Qp::GS; j++)
+        sh[lds_pad(fwd_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < Q::NG; u++)
+#pragma unroll
+      for (int j = 0; j < Q::GS; j++)
+        v[u * Q::GS + j] = sh[lds_pad(fwd_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
+    __syncthreads();
+    fwd_pass<LGN, P>(v, tid, tw, p);
+    fwd_rest<LGN, P + 1>(v, sh, tid, tw, p, pinv);
+  }
+}
+
+template <int LGN, int P>
+__device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const double2* tw,
+                                         double p, double pinv) {
+  if constexpr (P < Shape<LGN>::NP) {
+    using Q = Pass<LGN, P>;
+    using Qp = Pass<LGN, P - 1>;
+    if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
+#pragma unroll
+    for (int u = 0; u < Qp::NG; u++)
+#pragma unroll
+      for (int j = 0; j < Qp::GS; j++)
+        sh[lds_pad(inv_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < Q::NG; u++)
+#pragma unroll
+      for (int j = 0; j < Q::GS; j++)
+        v[u * Q::GS + j] = sh[lds_pad(inv_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
+    __syncthreads();
+    inv_pass<LGN, P>(v, tid, tw, p);
+    inv_rest<LGN, P + 1>(v, sh, tid, tw, p, pinv);
+  }
+}
+
+// Residue of chunk i (t limbs) of an entry, times the pack constants
+// pk[m] = 2^(24 m) S mod p split in 24-bit halves.  Pieces g_m < 2^24, so
+// each product g * half < 2^48 and the sums of at most 22 terms stay below
+// 2^53: exact.  Then s_hi 2^24 + s_lo (< 2^53) is reduced.
+__device__ __forceinline__ double pack_chunk(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+                                             uint32_t i, int t, const double (*pk)[2], double p,
+                                             double pinv) {
+  uint64_t c[RF_TMAX];
+  const uint64_t base = (uint64_t)i * t;
+#pragma unroll
+  for (int l = 0; l < RF_TMAX; l++) c[l] = (l < t && base + l < nlimbs) ? limbs[base + l] : 0;
+  const int np = (64 * t + RF_PIECE - 1) / RF_PIECE;
+  double shi = 0.0, slo = 0.0;
+#pragma unroll
+  for (int m = 0; m < RF_MAXPIECES; m++) {
+    if (m < np) {
+      const int bit = RF_PIECE * m, li = bit >> 6, sh = bit & 63;
+      uint64_t g = c[li] >> sh;
+      if (sh > 64 - RF_PIECE && li + 1 < RF_TMAX) g |= c[li + 1] << (64 - sh);
+      const double gd = (double)(uint32_t)(g & ((1u << RF_PIECE) - 1));
+      shi = __builtin_fma(gd, pk[m][0], shi);
+      slo = __builtin_fma(gd, pk[m][1], slo);
+    }
+  }
+  const double r = rf_mulmod(shi, 16777216.0, 16777216.0 * pinv, p) + slo;
+  return rf_reduce(r, p, pinv);
+}
+
+template <int LGN>
+__global__ __launch_bounds__(Shape<LGN>::THREADS) void k_fwd(RfDevTables tb,
+                                                             const RfFwdTask* __restrict__ tasks,
+                                                             int ntasks, double* __restrict__ spec) {
+  using S = Shape<LGN>;
+  using Q0 = Pass<LGN, 0>;
+  extern __shared__ double sh_all[];
+  const int sub = threadIdx.x / S::T, tid = threadIdx.x % S::T;
+  const int wg = xcd_task(blockIdx.x, gridDim.x, (ntasks + S::G - 1) / S::G);
+  if (wg < 0) return;
+  const int ti = wg * S::G + sub;
+  if (ti >= ntasks) return;  // only whole transforms drop out; no barrier spans them
+  const RfFwdTask tk = tasks[ti];
+  double* sh = sh_all + sub * S::LDSN;
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  const double(*pk)[2] = C->pk[tk.q][tk.scale];
+  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  double v[S::E];
+#pragma unroll
+  for (int u = 0; u < Q0::NG; u++)
+#pragma unroll
+    for (int j = 0; j < Q0::GS; j++) {
+      const int i = fwd_index<LGN, 0>(tid + S::T * u, j);
+      double x = 0.0;
+      if ((uint32_t)i < tk.nchunk && tk.sign != 0) {
+        x = pack_chunk(tk.limbs, tk.nlimbs, (uint32_t)i, tk.t, pk, p, pinv);
+        x = tk.sign < 0 ? -x : x;
+      }
+      v[u * Q0::GS + j] = x;
+    }
+  fwd_pass<LGN, 0>(v, tid, tw, p);
+  fwd_rest<LGN, 1>(v, sh, tid, tw, p, pinv);
+  using QL = Pass<LGN, S::NP - 1>;
+  double* out = spec + tk.spec_off;
+#pragma unroll
+  for (int u = 0; u < QL::NG; u++)
+#pragma unroll
+    for (int j = 0; j < QL::GS; j++)
+      out[fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+}
+
+template <int LGN>
+__global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
+                                                             const RfInvTask* __restrict__ tasks,
+                                                             int ntasks, const double* __restrict__ spec,
+                                                             double* __restrict__ coef) {
+  using S = Shape<LGN>;
+  using Q0 = Pass<LGN, 0>;
+  extern __shared__ double sh_all[];
+  const int sub = threadIdx.x / S::T, tid = threadIdx.x % S::T;
+  const int wg = xcd_task(blockIdx.x, gridDim.x, (ntasks + S::G - 1) / S::G);
+  if (wg < 0) return;
+  const int ti = wg * S::G + sub;
+  if (ti >= ntasks) return;
+  const RfInvTask tk = tasks[ti];
+  double* sh = sh_all + sub * S::LDSN;
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
+  double v[S::E];
+  // Pointwise product in coalesced order (consecutive lanes, consecutive
+  // frequencies) into LDS; pass 0 then reads its groups of 2^R consecutive
+  // frequencies from there.
+#pragma unroll
+  for (int e = 0; e < S::E; e++) {
+    const int i = tid + S::T * e;
+    double acc = 0.0;
+    for (int l = 0; l < tk.d; l++) {
+      const double a = spec[tk.a_off + (size_t)l * tk.a_stride + i];
+      const double b = spec[tk.b_off + (size_t)l * tk.b_stride + i];
+      acc += rf_mulmod_var(a, b, p, pinv);  // |term| <= 0.6 p
+      if ((l & 7) == 7) acc = rf_reduce(acc, p, pinv);
+    }
+    v[e] = rf_reduce(acc, p, pinv);
+  }
+  if constexpr (S::T > 1) {
+#pragma unroll
+    for (int e = 0; e < S::E; e++) sh[lds_pad(tid + S::T * e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < Q0::NG; u++)
+#pragma unroll
+      for (int j = 0; j < Q0::GS; j++) v[u * Q0::GS + j] = sh[lds_pad(inv_index<LGN, 0>(tid + S::T * u, j))];
+    __syncthreads();
+  }
+  inv_pass<LGN, 0>(v, tid, tw, p);
+  inv_rest<LGN, 1>(v, sh, tid, tw, p, pinv);
+  using QL = Pass<LGN, S::NP - 1>;
+  double* out = coef + tk.out_off;
+#pragma unroll
+  for (int u = 0; u < QL::NG; u++)
+#pragma unroll
+    for (int j = 0; j < QL::GS; j++)
+      out[inv_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+}
+
+template <int LGN>
+hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntasks, double* spec,
+                        hipStream_t st) {
+  using S = Shape<LGN>;
+  const int wgs = (ntasks + S::G - 1) / S::G;
+  const int grid = (wgs + 7) & ~7;
+  const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)k_fwd<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_fwd<LGN>, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec);
+  return hipGetLastError();
+}
+
+template <int LGN>
+hipError_t launch_inv_t(const RfDevTables& tb, const RfInvTask* tasks, int ntasks,
+                        const double* spec, double* coef, hipStream_t st) {
+  using S = Shape<LGN>;
+  const int wgs = (ntasks + S::G - 1) / S::G;
+  const int grid = (wgs + 7) & ~7;
+  const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)k_inv<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_inv<LGN>, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec, coef);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+                         double* spec, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  switch (lgN) {
+#define RF_CASE(L) \
+  case L:          \
+    return launch_fwd_t<L>(tb, tasks, ntasks, spec, st);
+    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
+    RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
+#undef RF_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
+                         const double* spec, double* coef, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  switch (lgN) {
+#define RF_CASE(L) \
+  case L:          \
+    return launch_inv_t<L>(tb, tasks, ntasks, spec, coef, st);
+    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
+    RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
+#undef RF_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}

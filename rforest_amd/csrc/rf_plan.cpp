@@ -61,9 +61,10 @@ int ceil_log2(uint64_t x) {
 
 void rf_primes_init(RfPrimes* pr) {
   memset(pr, 0, sizeof *pr);
-  // p = c 2^S + 1 < 2^48, scanning c downwards from the top.
-  for (uint64_t c = ((uint64_t)1 << (48 - RF_S)) - 1; pr->k < RF_MAXP; c--) {
+  // p = c 2^S + 1 < RF_PMAX, scanning c downwards from the top.
+  for (uint64_t c = (RF_PMAX - 1) >> RF_S; pr->k < RF_MAXP; c--) {
     uint64_t p = (c << RF_S) + 1;
+    if (p >= RF_PMAX) continue;
     if (p <= ((uint64_t)1 << 47)) break;
     if (is_prime_u64(p)) {
       pr->p[pr->k] = p;

@@ -12,6 +12,10 @@
 #include <stdint.h>
 
 #define RF_MAXP 32        // primes generated at setup
+// Upper limit of the primes: 2^51 / 9.  The transforms reduce every eighth
+// stage; a value starting at |x| <= p/2 + 8 grows by at most p per stage, so
+// it stays below 8.5 p + 8 < 2^51, the input bound of rf_mulmod.
+#define RF_PMAX ((((uint64_t)1) << 51) / 9)
 #define RF_S 24           // every prime is 1 mod 2^RF_S
 #define RF_LGN_MIN 1
 #define RF_LGN_MAX 14     // largest LDS-resident transform (2^14 doubles = 128 KiB)
@@ -38,7 +42,7 @@ struct RfPlan {
   double cost;  // planner's estimate, arbitrary units
 };
 
-// Generates the RF_MAXP largest primes 2^47 < p < 2^48 with p = 1 mod 2^RF_S.
+// Generates the RF_MAXP largest primes 2^47 < p < RF_PMAX with p = 1 mod 2^RF_S.
 void rf_primes_init(RfPrimes* pr);
 
 // Exact test of f * (2^b - 1)^2 < p_0 ... p_{k-1}.

@@ -103,8 +103,35 @@ class DeviceBatch:
             self._keep += [A, B, C]
         self.prods = prods
 
+        self.handle = None
+
     def run(self, sync=True):
+        """One-shot: plans, builds task tables and runs (rf_batch_mult_device)."""
         check(lib().rf_batch_mult_device(self.prods, len(self.products), None, 0 if sync else 1))
+
+    def prepare(self):
+        """Fixes plans, task tables and scratch once (rf_batch_create)."""
+        if self.handle is None:
+            h = ctypes.c_void_p()
+            check(lib().rf_batch_create(self.prods, len(self.products), ctypes.byref(h)))
+            self.handle = h
+        return self
+
+    def launch(self, graph=True):
+        """Enqueues the prepared pipeline; returns without waiting."""
+        self.prepare()
+        check(lib().rf_batch_launch(self.handle, None, 1 if graph else 0))
+
+    def release(self):
+        if self.handle is not None:
+            lib().rf_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
     def results(self):
         """Per product, the r*d entries of C as Python ints."""

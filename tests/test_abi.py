@@ -40,7 +40,7 @@ def test_prime_set():
     assert len(set(ps)) == 32
     assert ps == sorted(ps, reverse=True)
     for p in ps:
-        assert (1 << 47) < p < (1 << 48)
+        assert (1 << 47) < p < (1 << 51) // 9
         assert p % (1 << 24) == 1
         assert is_prime(p)
 

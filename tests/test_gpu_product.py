@@ -139,3 +139,28 @@ def test_device_batch_repeat_is_deterministic():
         batch.run(sync=False)
     assert batch.results() == first
     assert first[0] == naive(prods[0][2], 2, prods[0][3], 2)
+
+
+def test_prepared_batch_graph_replays_match_eager():
+    """The prepared pipeline, replayed from a captured hipGraph several times
+    with eager launches and synchronisations in between, rewrites every output
+    with the eager result each time (outputs are clobbered between runs)."""
+    import numpy as np
+    prods = workloads.tree_level(16, seed=7) + workloads.headline(2, seed=42)
+    batch = rf.DeviceBatch(prods)
+    batch.run()
+    eager = batch.results()
+    for (r, d, A, B), C in zip(prods, eager):
+        assert C == naive(A, r, B, d)
+    batch.prepare()
+    junk = np.full(max(batch.total_out, 1), 0x5A5A5A5A5A5A5A5A, dtype=np.uint64)
+    for _ in range(4):
+        batch.out.upload(junk)
+        batch.launch(graph=True)
+        rf.device_sync()
+        assert batch.results() == eager
+        batch.out.upload(junk)
+        batch.launch(graph=False)
+        rf.device_sync()
+        assert batch.results() == eager
+    batch.release()
