@@ -112,6 +112,12 @@ def main():
     t, lgN, k = plan
     N = 1 << lgN
     rf.hw_mpz_setup()
+    # One device-to-device copy of 256 MiB: under rocprofv3 --pmc its
+    # FETCH_SIZE / WRITE_SIZE calibrate the traffic counters (known bytes).
+    cal = [rf.DeviceBuffer(256 << 20) for _ in range(2)]
+    rf.check(rf.lib().rf_memcpy_d2d(cal[1].ptr, cal[0].ptr, 256 << 20))
+    for c in cal:
+        c.free()
     batch = rf.DeviceBatch(products)
     log(f"plan: chunk {64 * t} bits, N = 2^{lgN}, k = {k} primes; batch {args.batch}")
 

@@ -138,6 +138,9 @@ void cu_device_free(void* p);
 void cu_memcpy_d2h(void* dst, const void* src, size_t n);
 /* Host to device copy; returns RF_OK or RF_ERR_HIP. */
 int rf_memcpy_h2d(void* dst, const void* src, size_t n);
+/* Device to device copy (a plain copy kernel of known size: n bytes read,
+   n bytes written); returns RF_OK or RF_ERR_HIP. */
+int rf_memcpy_d2d(void* dst, const void* src, size_t n);
 /* Waits for all work the library enqueued (hipDeviceSynchronize). */
 int rf_device_sync(void);
 

@@ -103,6 +103,8 @@ def lib():
     L.rf_memcpy_h2d.restype = ctypes.c_int
     L.rf_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     L.rf_device_sync.restype = ctypes.c_int
+    L.rf_memcpy_d2d.restype = ctypes.c_int
+    L.rf_memcpy_d2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     L.rf_batch_create.restype = ctypes.c_int
     L.rf_batch_create.argtypes = [ctypes.POINTER(DevProduct), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.rf_batch_launch.restype = ctypes.c_int

@@ -53,27 +53,64 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
     }
     v[j] = rf_canon(r, p);
   }
+  // Mixed radix to binary: X = v_{K-1}; X = X p_j + v_j for j = K-2 .. 0, in
+  // 32-bit limbs so that every limb step is one v_mad_u64_u32 (32 x 32 + 64).
+  // p_j = phi 2^32 + plo with phi < 2^16.  Before step j, X < 2^(48 (K-1-j)).
+  constexpr int W = 2 * S;
+  uint32_t x[W];
 #pragma unroll
-  for (int l = 0; l < S; l++) X[l] = 0;
-  X[0] = (uint64_t)v[K - 1];
+  for (int l = 0; l < W; l++) x[l] = 0;
+  {
+    const uint64_t t = (uint64_t)v[K - 1];
+    x[0] = (uint32_t)t;
+    x[1] = (uint32_t)(t >> 32);
+  }
 #pragma unroll
   for (int j = K - 2; j >= 0; j--) {
+    const int nl = (48 * (K - 1 - j) + 31) / 32;  // limbs of X before the step
     const uint64_t pj = C->pu[j];
-    uint64_t carry = (uint64_t)v[j];
-    // after this step X < p_0 ... p_{K-1} / (p_0 ... p_{j-1}) < 2^(48 (K - j))
-    constexpr int dummy = 0;
-    (void)dummy;
+    const uint32_t plo = (uint32_t)pj, phi = (uint32_t)(pj >> 32);
+    const uint64_t vj = (uint64_t)v[j];
+    // y = X plo + v_j: the low 32 bits of v_j enter at limb 0, the high 16 at
+    // limb 1, so every partial sum stays below 2^64
+    uint32_t y[W];
+    uint64_t c = (uint32_t)vj;
 #pragma unroll
-    for (int l = 0; l < S; l++) {
-      if (l * 64 < 48 * (K - j) + 64) {
-        const uint64_t lo = X[l] * pj;
-        const uint64_t hi = __umul64hi(X[l], pj);
-        const uint64_t s = lo + carry;
-        carry = hi + (s < lo);
-        X[l] = s;
+    for (int l = 0; l < W; l++) {
+      if (l < nl) {
+        const uint64_t t = (uint64_t)x[l] * plo + c + (l == 1 ? (vj >> 32) : 0);
+        y[l] = (uint32_t)t;
+        c = t >> 32;
+      } else if (l == nl) {
+        y[l] = (uint32_t)c;  // nl >= 2, so v_j's high part went in at limb 1
+      } else {
+        y[l] = 0;
       }
     }
+    // X' = y + X phi 2^32, into z (x is still read while z is written)
+    uint32_t z[W];
+    uint64_t d = 0;
+    z[0] = y[0];
+#pragma unroll
+    for (int l = 0; l + 1 < W; l++) {
+      if (l < nl) {
+        const uint64_t t = (uint64_t)x[l] * phi + y[l + 1] + d;
+        z[l + 1] = (uint32_t)t;
+        d = t >> 32;
+      } else if (l == nl) {
+        z[l + 1] = (uint32_t)d;
+      } else {
+        z[l + 1] = 0;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < W; l++) x[l] = z[l];
   }
+  uint64_t X64[S];
+#pragma unroll
+  for (int l = 0; l < S; l++) X64[l] = (uint64_t)x[2 * l] | ((uint64_t)x[2 * l + 1] << 32);
+#pragma unroll
+  for (int l = 0; l < S; l++) X[l] = X64[l];
   // symmetric lift: X > floor(P / 2)  =>  X - P (two's complement in S limbs)
   bool gt = false, decided = false;
 #pragma unroll
@@ -130,24 +167,22 @@ __device__ __forceinline__ int64_t digit_sum(const uint64_t* __restrict__ X, int
 }
 
 template <int K>
-__global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_crt(RfDevTables tb,
-                                                            const RfCrtTask* __restrict__ tasks,
-                                                            const double* __restrict__ coef,
-                                                            uint8_t* __restrict__ pre,
-                                                            uint8_t* __restrict__ fun) {
+__global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
+                                                       const RfCrtTask* __restrict__ tasks,
+                                                       const double* __restrict__ coef,
+                                                       uint8_t* __restrict__ pre,
+                                                       uint8_t* __restrict__ fun) {
   constexpr int S = (48 * K + 1 + 63) / 64;
-  constexpr int NC = RF_DIGITS_PER_BLOCK + S + 2;  // coefficients held per block
-  __shared__ uint64_t X[NC * S];
-  __shared__ int64_t E[RF_DIGITS_PER_BLOCK + 1];
-  __shared__ uint32_t code[RF_DIGITS_PER_BLOCK];
+  __shared__ uint64_t X[RF_CRT_THREADS * S];
+  __shared__ int64_t E[RF_CRT_THREADS];
+  __shared__ uint32_t code[RF_CRT_THREADS];
   const int tid = threadIdx.x;
-  const RfCrtTask tk = tasks[blockIdx.x];
-  const int t = tk.t, M = (S + t - 1) / t;
+  const RfCrtTask tk = rf_wave_uniform_load(tasks + blockIdx.x);
+  const int t = tk.t, M = (S + t - 1) / t, H = tk.halo;
   const int digit0 = (int)tk.digit0;
-  const int j0 = digit0 - M - 2;  // digits digit0-1 .. need X from digit0 - 1 - M and jneg
-  const int nc = RF_DIGITS_PER_BLOCK + M + 2;
-  for (int c = tid; c < nc; c += RF_DIGITS_PER_BLOCK) {
-    const int j = j0 + c;
+  const int j0 = digit0 - H;  // coefficient of thread 0
+  {
+    const int j = j0 + tid;
     uint64_t x[S];
     if (j >= 0 && (uint32_t)j < tk.nout) {
       garner<K, S>(coef + tk.coef_off + (uint32_t)j, tk.N, tb.consts, x);
@@ -156,22 +191,23 @@ __global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_crt(RfDevTables tb,
       for (int l = 0; l < S; l++) x[l] = 0;
     }
 #pragma unroll
-    for (int l = 0; l < S; l++) X[c * S + l] = x[l];
+    for (int l = 0; l < S; l++) X[tid * S + l] = x[l];
   }
   __syncthreads();
+  // Thread tid >= H - 1 owns digit i = digit0 + tid - H (digit0 - 1 for
+  // tid = H - 1); its inputs X_{i-M+1..i} and the sign term X_{i - S/t}
+  // all lie at or above j0 + 1 because H = M + 2.
+  const int i = digit0 + tid - H;
   // pass 1: carry out E_i of each digit on its own
   {
     auto none = [](int, uint64_t) {};
-    const int i = digit0 + tid;
-    E[tid + 1] = digit_sum<S>(X, j0, i, t, M, 0, none);
-    if (tid == 0) E[0] = digit0 > 0 ? digit_sum<S>(X, j0, digit0 - 1, t, M, 0, none) : 0;
+    E[tid] = (tid >= H - 1 && i >= 0) ? digit_sum<S>(X, j0, i, t, M, 0, none) : 0;
   }
   __syncthreads();
   // pass 2: U_i = (V_i + E_{i-1}) mod 2^(64 t), F_i = carry - E_i in {-1, 0, 1}
-  const int i = digit0 + tid;
   const uint32_t ndig = (tk.rn + t - 1) / t;
   uint32_t f = FN_ID;
-  if ((uint32_t)i < ndig) {
+  if (tid >= H && (uint32_t)i < ndig) {
     bool all0 = true, all1 = true;
     uint64_t* dst = tk.out + (size_t)i * t;
     const uint32_t lim = tk.rn - (uint32_t)i * t;
@@ -180,8 +216,8 @@ __global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_crt(RfDevTables tb,
       all1 &= (v == ~(uint64_t)0);
       if ((uint32_t)l < lim) dst[l] = v;
     };
-    const int64_t co = digit_sum<S>(X, j0, i, t, M, E[tid], emit);
-    const int F = (int)(co - E[tid + 1]);
+    const int64_t co = digit_sum<S>(X, j0, i, t, M, E[tid - 1], emit);
+    const int F = (int)(co - E[tid]);
     f = 0;
 #pragma unroll
     for (int c = -1; c <= 1; c++) {
@@ -191,9 +227,9 @@ __global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_crt(RfDevTables tb,
   }
   code[tid] = f;
   __syncthreads();
-  // inclusive scan by composition: code[i] = f_i o ... o f_digit0
+  // inclusive scan by composition: code[tid] = f_tid o ... o f_H
 #pragma unroll 1
-  for (int off = 1; off < RF_DIGITS_PER_BLOCK; off <<= 1) {
+  for (int off = 1; off < RF_CRT_THREADS; off <<= 1) {
     uint32_t g = code[tid];
     if (tid >= off) g = fn_compose(g, code[tid - off]);
     __syncthreads();
@@ -201,7 +237,7 @@ __global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_crt(RfDevTables tb,
     __syncthreads();
   }
   pre[tk.pre_off + tid] = (uint8_t)(tid == 0 ? FN_ID : code[tid - 1]);
-  if (tid == RF_DIGITS_PER_BLOCK - 1) fun[tk.fun_idx] = (uint8_t)code[tid];
+  if (tid == RF_CRT_THREADS - 1) fun[tk.fun_idx] = (uint8_t)code[tid];
 }
 
 __global__ void k_carry(const RfCarryTask* __restrict__ tasks, int ntasks,
@@ -216,13 +252,14 @@ __global__ void k_carry(const RfCarryTask* __restrict__ tasks, int ntasks,
   }
 }
 
-__global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_apply(const RfCrtTask* __restrict__ tasks,
-                                                              const uint8_t* __restrict__ pre,
-                                                              const int8_t* __restrict__ cin) {
+__global__ __launch_bounds__(RF_CRT_THREADS) void k_apply(const RfCrtTask* __restrict__ tasks,
+                                                         const uint8_t* __restrict__ pre,
+                                                         const int8_t* __restrict__ cin) {
   const int tid = threadIdx.x;
-  const RfCrtTask tk = tasks[blockIdx.x];
-  const int t = tk.t;
-  const uint32_t i = tk.digit0 + tid;
+  const RfCrtTask tk = rf_wave_uniform_load(tasks + blockIdx.x);
+  const int t = tk.t, H = tk.halo;
+  if (tid < H) return;
+  const uint32_t i = tk.digit0 + (uint32_t)(tid - H);
   const uint32_t ndig = (tk.rn + t - 1) / t;
   if (i >= ndig) return;
   const int c = fn_apply(pre[tk.pre_off + tid], cin[tk.fun_idx]);
@@ -244,7 +281,7 @@ __global__ __launch_bounds__(RF_DIGITS_PER_BLOCK) void k_apply(const RfCrtTask* 
 template <int K>
 hipError_t launch_crt_t(const RfDevTables& tb, const RfCrtTask* tasks, int ntasks,
                         const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st) {
-  hipLaunchKernelGGL(k_crt<K>, dim3(ntasks), dim3(RF_DIGITS_PER_BLOCK), 0, st, tb, tasks, coef,
+  hipLaunchKernelGGL(k_crt<K>, dim3(ntasks), dim3(RF_CRT_THREADS), 0, st, tb, tasks, coef,
                      pre, fun);
   return hipGetLastError();
 }
@@ -278,6 +315,6 @@ hipError_t rf_launch_carry(const RfCarryTask* tasks, int ntasks, const uint8_t* 
 hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre,
                            const int8_t* cin, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_apply, dim3(ntasks), dim3(RF_DIGITS_PER_BLOCK), 0, st, tasks, pre, cin);
+  hipLaunchKernelGGL(k_apply, dim3(ntasks), dim3(RF_CRT_THREADS), 0, st, tasks, pre, cin);
   return hipGetLastError();
 }

@@ -235,7 +235,8 @@ struct Layout {
 
 uint32_t crt_blocks(const RfPlan& pl) {
   const uint64_t ndig = (pl.rn + pl.t - 1) / pl.t;
-  return (uint32_t)((ndig + RF_DIGITS_PER_BLOCK - 1) / RF_DIGITS_PER_BLOCK);
+  const uint64_t D = RF_CRT_THREADS - rf_crt_halo(pl.k, pl.t);
+  return (uint32_t)((ndig + D - 1) / D);
 }
 
 Layout layout_of(std::vector<Product>& prods) {
@@ -265,7 +266,7 @@ Layout layout_of(std::vector<Product>& prods) {
   for (auto& kv : crt_n) tb += align256(kv.second * sizeof(RfCrtTask)), L.ncrt += kv.second;
   tb += align256(L.ncarry * sizeof(RfCarryTask));
   L.tasks_b = tb;
-  L.pre_b = align256((size_t)L.nblk * RF_DIGITS_PER_BLOCK);
+  L.pre_b = align256((size_t)L.nblk * RF_CRT_THREADS);
   L.fun_b = align256(L.nblk);
   return L;
 }
@@ -351,12 +352,14 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
         t.N = (uint32_t)N;
         t.nout = (uint32_t)pl.nout;
         t.rn = (uint32_t)pl.rn;
-        t.digit0 = b * RF_DIGITS_PER_BLOCK;
+        const int halo = rf_crt_halo(k, pl.t);
+        t.digit0 = b * (uint32_t)(RF_CRT_THREADS - halo);
         t.t = (uint8_t)pl.t;
         t.k = (uint8_t)k;
         t.s = (uint8_t)pl.s;
+        t.halo = (uint8_t)halo;
         crt[k].push_back(t);
-        npre += RF_DIGITS_PER_BLOCK;
+        npre += RF_CRT_THREADS;
       }
       nblk_total += nblk;
     }
@@ -866,6 +869,11 @@ int rf_memcpy_h2d(void* dst, const void* src, size_t n) {
   return RF_OK;
 }
 
+int rf_memcpy_d2d(void* dst, const void* src, size_t n) {
+  HIPCHK(hipMemcpy(dst, src, n, hipMemcpyDeviceToDevice));
+  return RF_OK;
+}
+
 int rf_device_sync(void) {
   HIPCHK(hipDeviceSynchronize());
   std::lock_guard<std::mutex> lk(g_mu);
@@ -998,9 +1006,11 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
     return fail(RF_ERR_ARG, "rf_stage_crt: bad arguments");
   const size_t coef_b = align256((size_t)k * nout * 8), out_b = align256(rn * 8);
   const uint64_t ndig = (rn + t - 1) / t;
-  const uint32_t nblk = (uint32_t)((ndig + RF_DIGITS_PER_BLOCK - 1) / RF_DIGITS_PER_BLOCK);
+  const int halo = rf_crt_halo(k, t);
+  const uint64_t D = RF_CRT_THREADS - halo;
+  const uint32_t nblk = (uint32_t)((ndig + D - 1) / D);
   const size_t task_b = align256(nblk * sizeof(RfCrtTask)) + align256(sizeof(RfCarryTask));
-  const size_t pre_b = align256((size_t)nblk * RF_DIGITS_PER_BLOCK), fun_b = align256(nblk);
+  const size_t pre_b = align256((size_t)nblk * RF_CRT_THREADS), fun_b = align256(nblk);
   rc = ensure_arena(coef_b + out_b + task_b + pre_b + 2 * fun_b);
   if (rc) return rc;
   rc = ensure_stage(std::max(coef_b + task_b, out_b));
@@ -1026,15 +1036,16 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
     RfCrtTask tk{};
     tk.coef_off = 0;
     tk.out = d_out;
-    tk.pre_off = (uint64_t)b * RF_DIGITS_PER_BLOCK;
+    tk.pre_off = (uint64_t)b * RF_CRT_THREADS;
     tk.fun_idx = b;
     tk.N = nout;
     tk.nout = nout;
     tk.rn = (uint32_t)rn;
-    tk.digit0 = b * RF_DIGITS_PER_BLOCK;
+    tk.digit0 = b * (uint32_t)D;
     tk.t = (uint8_t)t;
     tk.k = (uint8_t)k;
     tk.s = (uint8_t)((48 * k + 1 + 63) / 64);
+    tk.halo = (uint8_t)halo;
     ((RfCrtTask*)ht)[b] = tk;
   }
   RfCarryTask ct{0, nblk};

@@ -37,8 +37,12 @@ struct RfInvTask {
   uint16_t d;
 };
 
-// One CRT/recompose block: RF_DIGITS_PER_BLOCK digits of one C entry.
-#define RF_DIGITS_PER_BLOCK 256
+// One CRT/recompose block of one C entry: RF_CRT_THREADS coefficients, one
+// per thread, covering the digits [digit0, digit0 + RF_CRT_THREADS - halo);
+// the first `halo` coefficients only feed those digits' overlaps, so
+// consecutive blocks recompute halo coefficients instead of any thread
+// computing two.
+#define RF_CRT_THREADS 256
 struct RfCrtTask {
   uint64_t coef_off;  // coef offset of prime 0 of this entry; prime j at + j * N
   uint64_t* out;      // output limbs of this entry (device pointer)
@@ -51,14 +55,37 @@ struct RfCrtTask {
   uint8_t t;          // limbs per digit
   uint8_t k;          // primes
   uint8_t s;          // CRT limbs per coefficient
-  uint8_t pad;
+  uint8_t halo;       // coefficients before digit0 the block recomputes
 };
+
+// Halo of a CRT block for k primes and t limbs per digit: ceil(S/t) + 2
+// coefficients with S = ceil((48 k + 1) / 64) limbs per coefficient.
+static inline int rf_crt_halo(int k, int t) {
+  const int S = (48 * k + 1 + 63) / 64;
+  return (S + t - 1) / t + 2;
+}
 
 // Per C entry, for the block-carry pass.
 struct RfCarryTask {
   uint32_t fun_idx;   // first block's index in fun / cin
   uint32_t nblk;
 };
+
+// A descriptor read by a whole wave: loaded as 32-bit words and broadcast
+// from the first lane, so its fields live in scalar registers and branches
+// on them stay uniform.  Only for descriptors all lanes of the wave share.
+template <typename T>
+__device__ __forceinline__ T rf_wave_uniform_load(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor size must be a multiple of 4");
+  union {
+    T t;
+    uint32_t w[sizeof(T) / 4];
+  } u;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; i++) u.w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return u.t;
+}
 
 // Device-side pointers to the constant tables.
 struct RfDevTables {

@@ -12,8 +12,8 @@
 // Organisation.  Each thread keeps E = 16 residues in registers.  A pass runs
 // R <= 4 butterfly stages on groups of 2^R residues whose indices differ only
 // in R adjacent bits; between passes the workgroup exchanges residues through
-// LDS (one write and one read of every residue, padded by one double per 32
-// so that strided accesses spread over the banks).  An N = 2^13 transform is
+// LDS (one write and one read of every residue, XOR-swizzled so that the
+// strided accesses spread over the banks).  An N = 2^13 transform is
 // four passes (4 + 4 + 4 + 1 stages) and three LDS exchanges.  Transforms
 // smaller than a wave share a workgroup.
 //
@@ -34,7 +34,11 @@ __device__ __forceinline__ int xcd_task(int bid, int grid, int ntasks) {
   return t < ntasks ? t : -1;
 }
 
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+// LDS slot of residue i: the low four index bits are XORed with the next four.
+// For every pass's access pattern each 16-lane group of a ds_read_b64 /
+// ds_write_b64 then covers 16 distinct 8-byte slots modulo 16 (consecutive
+// runs, stride-2 pairs, and groups spread by 32 or 16 residues alike).
+__device__ __forceinline__ int lds_pad(int i) { return i ^ ((i >> 4) & 15); }
 
 template <int LGN>
 struct Shape {
@@ -43,7 +47,7 @@ struct Shape {
   static constexpr int T = N / E;                      // threads per transform
   static constexpr int G = T >= 64 ? 1 : 64 / T;       // transforms per workgroup
   static constexpr int NP = (LGN + 3) / 4;             // passes
-  static constexpr int LDSN = N + (N >> 5);            // padded doubles per transform
+  static constexpr int LDSN = N;                       // doubles of LDS per transform
   static constexpr int THREADS = T * G;
 };
 
@@ -85,13 +89,13 @@ __device__ __forceinline__ int inv_index(int g, int j) {
 // R forward stages on NG groups held in v[u * 2^R + j].  At stage L0 + s the
 // pair (j, j + 2^(R-1-s)) of group (H, lo) uses twiddle T[H 2^s + (j >> (R-s))]
 // (T[m] = omega^(rev(m)), shared prefix for all lengths).
-template <int LGN, int P>
+template <int LGN, int P, int S0 = 0>
 __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __restrict__ tw,
                                          double p) {
   using Q = Pass<LGN, P>;
   constexpr int NLO = LGN - Q::L0 - Q::R;
 #pragma unroll
-  for (int s = 0; s < Q::R; s++) {
+  for (int s = S0; s < Q::R; s++) {
     const int half = 1 << (Q::R - 1 - s);
 #pragma unroll
     for (int u = 0; u < Q::NG; u++) {
@@ -99,11 +103,13 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __re
       const int H = g >> NLO;
 #pragma unroll
       for (int b = 0; b < (1 << s); b++) {
-        const double2 w = tw[(H << s) + b];
+        // in pass 0 (H = 0) block 0 uses T[0] = 1: no multiplication
+        const bool one = (P == 0) && (b == 0);
+        const double2 w = one ? make_double2(1.0, 0.0) : tw[(H << s) + b];
 #pragma unroll
         for (int jj = 0; jj < half; jj++) {
           const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
-          const double z = rf_mulmod(v[j1], w.x, w.y, p);
+          const double z = one ? v[j1] : rf_mulmod(v[j1], w.x, w.y, p);
           const double x = v[j0];
           v[j0] = x + z;
           v[j1] = x - z;
@@ -129,11 +135,13 @@ __device__ __forceinline__ void inv_pass(double* v, int tid, const double2* __re
       const int lo = g & ((1 << Q::L0) - 1);
 #pragma unroll
       for (int jj = 0; jj < half; jj++) {
-        const double2 w = tw[len + (jj << Q::L0) + lo];
+        // in pass 0 (lo = 0) the pair with jj = 0 uses V[len] = 1
+        const bool one = (P == 0) && (jj == 0);
+        const double2 w = one ? make_double2(1.0, 0.0) : tw[len + (jj << Q::L0) + lo];
 #pragma unroll
         for (int b = 0; b < (Q::GS >> (s + 1)); b++) {
           const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
-          const double z = rf_mulmod(v[j1], w.x, w.y, p);
+          const double z = one ? v[j1] : rf_mulmod(v[j1], w.x, w.y, p);
           const double x = v[j0];
           v[j0] = x + z;
           v[j1] = x - z;
@@ -199,32 +207,51 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const d
   }
 }
 
-// Residue of chunk i (t limbs) of an entry, times the pack constants
-// pk[m] = 2^(24 m) S mod p split in 24-bit halves.  Pieces g_m < 2^24, so
-// each product g * half < 2^48 and the sums of at most 22 terms stay below
-// 2^53: exact.  Then s_hi 2^24 + s_lo (< 2^53) is reduced.
-__device__ __forceinline__ double pack_chunk(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
-                                             uint32_t i, int t, const double (*pk)[2], double p,
-                                             double pinv) {
-  uint64_t c[RF_TMAX];
-  const uint64_t base = (uint64_t)i * t;
+// Residues of NCH chunks ci[c] (t limbs = 2t 32-bit words each) of an entry
+// modulo p, times the scale folded into the pack constants:
+// sum_w word_w K_w with K_w = 2^(32 w) S mod p split in 16-bit pieces
+// K_w = k0 + k1 2^16 + k2 2^32.  Each word * piece < 2^48 and at most 16
+// terms are summed, so the three accumulators stay below 2^52: exact.  Then
+// s0 + s1 2^16 + s2 2^32 mod p by two rf_mulmod (inputs < 2^52, outputs
+// <= 1.5 p) and one reduction of a value below 2^53.  Limb l of all NCH
+// chunks is loaded before any is used, so NCH loads are in flight; loads past
+// the entry's end read limb 0 and are discarded, so control flow depends only
+// on the uniform t.
+template <int NCH>
+__device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+                                            const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
+                                            const double* pk2, double p, double pinv,
+                                            double (&out)[NCH]) {
+  double s0[NCH], s1[NCH], s2[NCH];
 #pragma unroll
-  for (int l = 0; l < RF_TMAX; l++) c[l] = (l < t && base + l < nlimbs) ? limbs[base + l] : 0;
-  const int np = (64 * t + RF_PIECE - 1) / RF_PIECE;
-  double shi = 0.0, slo = 0.0;
+  for (int c = 0; c < NCH; c++) s0[c] = s1[c] = s2[c] = 0.0;
 #pragma unroll
-  for (int m = 0; m < RF_MAXPIECES; m++) {
-    if (m < np) {
-      const int bit = RF_PIECE * m, li = bit >> 6, sh = bit & 63;
-      uint64_t g = c[li] >> sh;
-      if (sh > 64 - RF_PIECE && li + 1 < RF_TMAX) g |= c[li + 1] << (64 - sh);
-      const double gd = (double)(uint32_t)(g & ((1u << RF_PIECE) - 1));
-      shi = __builtin_fma(gd, pk[m][0], shi);
-      slo = __builtin_fma(gd, pk[m][1], slo);
+  for (int l = 0; l < RF_TMAX; l++) {
+    if (l < t) {
+      uint64_t x[NCH];
+#pragma unroll
+      for (int c = 0; c < NCH; c++) {
+        const uint32_t idx = ci[c] * (uint32_t)t + (uint32_t)l;
+        const uint64_t y = limbs[idx < nlimbs ? idx : 0];
+        x[c] = idx < nlimbs ? y : 0;
+      }
+      const double a0 = pk[2 * l][0], a1 = pk[2 * l][1], a2 = pk[2 * l][2];
+      const double b0 = pk[2 * l + 1][0], b1 = pk[2 * l + 1][1], b2 = pk[2 * l + 1][2];
+#pragma unroll
+      for (int c = 0; c < NCH; c++) {
+        const double lo = (double)(uint32_t)x[c];
+        const double hi = (double)(uint32_t)(x[c] >> 32);
+        s0[c] = __builtin_fma(hi, b0, __builtin_fma(lo, a0, s0[c]));
+        s1[c] = __builtin_fma(hi, b1, __builtin_fma(lo, a1, s1[c]));
+        s2[c] = __builtin_fma(hi, b2, __builtin_fma(lo, a2, s2[c]));
+      }
     }
   }
-  const double r = rf_mulmod(shi, 16777216.0, 16777216.0 * pinv, p) + slo;
-  return rf_reduce(r, p, pinv);
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const double r = rf_mulmod(s1[c], pk2[0], pk2[1], p) + rf_mulmod(s2[c], pk2[2], pk2[3], p) + s0[c];
+    out[c] = rf_reduce(r, p, pinv);
+  }
 }
 
 template <int LGN>
@@ -239,26 +266,63 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_fwd(RfDevTables tb,
   if (wg < 0) return;
   const int ti = wg * S::G + sub;
   if (ti >= ntasks) return;  // only whole transforms drop out; no barrier spans them
-  const RfFwdTask tk = tasks[ti];
+  const RfFwdTask tk = S::G == 1 ? rf_wave_uniform_load(tasks + ti) : tasks[ti];
   double* sh = sh_all + sub * S::LDSN;
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double(*pk)[2] = C->pk[tk.q][tk.scale];
+  const double(*pk)[3] = C->pk[tk.q][tk.scale];
+  const double* pk2 = C->pk2[tk.q];
   const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  const double sg = tk.sign < 0 ? -1.0 : 1.0;
+  const uint32_t nchunk = tk.sign != 0 ? tk.nchunk : 0u;
   double v[S::E];
+  // Pass 0 holds, per group, residues i = j 2^(LGN-R) + g: the upper half of
+  // j covers positions >= N/2.  When the entry has at most N/2 chunks those
+  // are zero and the first stage (twiddle 1) just copies (x, 0) -> (x, x).
+  constexpr int CB = S::E / 2 < 8 ? (S::E / 2 > 0 ? S::E / 2 : 1) : 8;  // chunks per load batch
+  if (nchunk <= (uint32_t)(S::N / 2) && Q0::GS > 1) {
 #pragma unroll
-  for (int u = 0; u < Q0::NG; u++)
+    for (int u = 0; u < Q0::NG; u++)
 #pragma unroll
-    for (int j = 0; j < Q0::GS; j++) {
-      const int i = fwd_index<LGN, 0>(tid + S::T * u, j);
-      double x = 0.0;
-      if ((uint32_t)i < tk.nchunk && tk.sign != 0) {
-        x = pack_chunk(tk.limbs, tk.nlimbs, (uint32_t)i, tk.t, pk, p, pinv);
-        x = tk.sign < 0 ? -x : x;
+      for (int j0 = 0; j0 < Q0::GS / 2; j0 += CB) {
+        uint32_t ci[CB];
+        double x[CB];
+#pragma unroll
+        for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
+        if (nchunk) {  // a zero entry reads no limbs at all
+          pack_chunks<CB>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+        } else {
+#pragma unroll
+          for (int c = 0; c < CB; c++) x[c] = 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < CB; c++) {
+          const double y = ci[c] < nchunk ? sg * x[c] : 0.0;
+          v[u * Q0::GS + j0 + c] = y;
+          v[u * Q0::GS + j0 + c + Q0::GS / 2] = y;
+        }
       }
-      v[u * Q0::GS + j] = x;
-    }
-  fwd_pass<LGN, 0>(v, tid, tw, p);
+    fwd_pass<LGN, 0, 1>(v, tid, tw, p);
+  } else {
+#pragma unroll
+    for (int u = 0; u < Q0::NG; u++)
+#pragma unroll
+      for (int j0 = 0; j0 < Q0::GS; j0 += CB) {
+        uint32_t ci[CB];
+        double x[CB];
+#pragma unroll
+        for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
+        if (nchunk) {
+          pack_chunks<CB>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+        } else {
+#pragma unroll
+          for (int c = 0; c < CB; c++) x[c] = 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < CB; c++) v[u * Q0::GS + j0 + c] = ci[c] < nchunk ? sg * x[c] : 0.0;
+      }
+    fwd_pass<LGN, 0, 0>(v, tid, tw, p);
+  }
   fwd_rest<LGN, 1>(v, sh, tid, tw, p, pinv);
   using QL = Pass<LGN, S::NP - 1>;
   double* out = spec + tk.spec_off;
@@ -282,7 +346,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
   if (wg < 0) return;
   const int ti = wg * S::G + sub;
   if (ti >= ntasks) return;
-  const RfInvTask tk = tasks[ti];
+  const RfInvTask tk = S::G == 1 ? rf_wave_uniform_load(tasks + ti) : tasks[ti];
   double* sh = sh_all + sub * S::LDSN;
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
@@ -290,18 +354,33 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
   double v[S::E];
   // Pointwise product in coalesced order (consecutive lanes, consecutive
   // frequencies) into LDS; pass 0 then reads its groups of 2^R consecutive
-  // frequencies from there.
+  // frequencies from there.  Loads are issued eight residues at a time per
+  // operand pair so that many are in flight.
+  constexpr int EB = S::E < 8 ? S::E : 8;
 #pragma unroll
-  for (int e = 0; e < S::E; e++) {
-    const int i = tid + S::T * e;
-    double acc = 0.0;
+  for (int e0 = 0; e0 < S::E; e0 += EB) {
+    double acc[EB];
+#pragma unroll
+    for (int e = 0; e < EB; e++) acc[e] = 0.0;
+    const double* pa = spec + tk.a_off;
+    const double* pb = spec + tk.b_off;
     for (int l = 0; l < tk.d; l++) {
-      const double a = spec[tk.a_off + (size_t)l * tk.a_stride + i];
-      const double b = spec[tk.b_off + (size_t)l * tk.b_stride + i];
-      acc += rf_mulmod_var(a, b, p, pinv);  // |term| <= 0.6 p
-      if ((l & 7) == 7) acc = rf_reduce(acc, p, pinv);
+      double a[EB], b[EB];
+#pragma unroll
+      for (int e = 0; e < EB; e++) {
+        a[e] = pa[tid + S::T * (e0 + e)];
+        b[e] = pb[tid + S::T * (e0 + e)];
+      }
+#pragma unroll
+      for (int e = 0; e < EB; e++) {
+        acc[e] += rf_mulmod_var(a[e], b[e], p, pinv);  // |term| <= 0.6 p
+        if ((l & 7) == 7) acc[e] = rf_reduce(acc[e], p, pinv);
+      }
+      pa += tk.a_stride;
+      pb += tk.b_stride;
     }
-    v[e] = rf_reduce(acc, p, pinv);
+#pragma unroll
+    for (int e = 0; e < EB; e++) v[e0 + e] = rf_reduce(acc[e], p, pinv);
   }
   if constexpr (S::T > 1) {
 #pragma unroll

@@ -141,14 +141,14 @@ int rf_plan_fixed(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t,
   size_t bits = LA + LB + (size_t)ceil_log2((uint64_t)d) + 1;
   pl.rn = (bits + 63) / 64;
   // Cost estimate in VALU instructions (full-rate, per wave lane): transforms
-  // at ~9.5 per butterfly, pack at ~3 per 24-bit piece, pointwise ~8 per term,
+  // at ~9 per butterfly, pack at ~4 per 32-bit word, pointwise ~8 per term,
   // Garner ~5 k^2 and base conversion ~6 k s per coefficient.  HBM bytes are
   // weighed at 39 instructions per 8 bytes (39.3 T lane-op/s vs 8 TB/s).
   double N = (double)((size_t)1 << lgN);
   double polys = (double)(r * d + d * d), outs = (double)(r * d);
-  double pieces = (double)((64 * t + RF_PIECE - 1) / RF_PIECE);
-  double ntt = N / 2 * lgN * 9.5;
-  double ops = polys * k * (ntt + (double)nA * (3 * pieces + 15)) +
+  double words = 2.0 * t;
+  double ntt = N / 2 * lgN * 9.0;
+  double ops = polys * k * (ntt + (double)nA * (4 * words + 20)) +
                outs * k * (ntt + N * 8 * d) + outs * (double)nout * (5.0 * k * k + 6.0 * k * pl.s);
   double bytes = 8 * N * k * (polys + outs) * 2;
   double c = ops > bytes * 39 / 8 ? ops : bytes * 39 / 8;
@@ -195,12 +195,18 @@ void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
     c->ci[j][1] = (double)inv / (double)p;
     for (int v = 0; v <= RF_LGN_MAX; v++) {
       uint64_t S = v == 0 ? 1 : rf_h_invmod(((uint64_t)1 << v) % p, p);
-      for (int m = 0; m < RF_MAXPIECES; m++) {
-        uint64_t K = rf_h_mulmod(rf_h_powmod(2, (uint64_t)RF_PIECE * m, p), S, p);
-        c->pk[j][v][m][0] = (double)(K >> 24);
-        c->pk[j][v][m][1] = (double)(K & 0xffffff);
+      for (int w = 0; w < RF_MAXWORDS; w++) {
+        uint64_t K = rf_h_mulmod(rf_h_powmod(2, (uint64_t)32 * w, p), S, p);
+        c->pk[j][v][w][0] = (double)(K & 0xffff);
+        c->pk[j][v][w][1] = (double)((K >> 16) & 0xffff);
+        c->pk[j][v][w][2] = (double)(K >> 32);
       }
     }
+    const uint64_t t32 = rf_h_powmod(2, 32, p);
+    c->pk2[j][0] = 65536.0;
+    c->pk2[j][1] = 65536.0 / (double)p;
+    c->pk2[j][2] = (double)t32;
+    c->pk2[j][3] = (double)t32 / (double)p;
   }
   for (int k = 0; k <= pr->k; k++) {
     uint64_t P[RF_MAXS + 1] = {1};

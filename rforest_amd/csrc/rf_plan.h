@@ -20,8 +20,7 @@
 #define RF_LGN_MIN 1
 #define RF_LGN_MAX 14     // largest LDS-resident transform (2^14 doubles = 128 KiB)
 #define RF_TMAX 8         // chunk width up to 8 limbs = 512 bits
-#define RF_PIECE 24       // pack splits a chunk into 24-bit pieces
-#define RF_MAXPIECES 22   // ceil(512 / 24)
+#define RF_MAXWORDS (2 * RF_TMAX)  // 32-bit words per chunk
 #define RF_MAXS 24        // CRT limbs per coefficient, >= ceil((48 RF_MAXP + 2) / 64)
 
 struct RfPrimes {
@@ -71,9 +70,11 @@ struct RfConsts {
   uint64_t pu[RF_MAXP];                  // the primes as integers
   uint64_t P[RF_MAXP + 1][RF_MAXS];      // P_k = p_0 ... p_{k-1}
   uint64_t Ph[RF_MAXP + 1][RF_MAXS];     // floor(P_k / 2)
-  // pack: pk[q][v][m] = (hi, lo) 24-bit halves of 2^(24 m) * S_v mod p_q, where
-  // S_0 = 1 and S_v = 2^-v mod p_q for v = lgN (B operands absorb 1/N).
-  double pk[RF_MAXP][RF_LGN_MAX + 1][RF_MAXPIECES][2];
+  // pack: pk[q][v][w] = the three 16-bit pieces (low first) of
+  // 2^(32 w) * S_v mod p_q, where S_0 = 1 and S_v = 2^-v mod p_q for v = lgN
+  // (B operands absorb 1/N); pk2[q] = (2^16, 2^16/p) and (2^32 mod p, ./p).
+  double pk[RF_MAXP][RF_LGN_MAX + 1][RF_MAXWORDS][3];
+  double pk2[RF_MAXP][4];
 };
 
 // Twiddle tables, per prime, as (w, fl(w/p)) pairs:
