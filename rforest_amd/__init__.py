@@ -206,8 +206,10 @@ def mpz_matrix_mult_fft(A, B, d):
 
 
 def mpz_rmatrix_mult_naive(A, r, B, d):
-    """Schoolbook product (src/include/hwmpz.h:117-124) in Python ints."""
-    return [sum(A[i * d + k] * B[k * d + j] for k in range(d)) for i in range(r) for j in range(d)]
+    """Schoolbook product with GMP mpz_mul / mpz_addmul
+    (src/include/hwmpz.h:117-124)."""
+    from .gmp import rmatrix_mult_schoolbook
+    return rmatrix_mult_schoolbook(A, r, B, d)
 
 
 def plan(la_bits, lb_bits, r, d):

@@ -228,7 +228,7 @@ uint64_t out_limbs_for(uint64_t la_bits, uint64_t lb_bits, int d) {
 struct Layout {
   uint64_t spec_n = 0, coef_n = 0;  // doubles
   uint32_t nblk = 0;                // CRT blocks in total
-  size_t nfwd = 0, ninv = 0, ncrt = 0, ncarry = 0;
+  size_t nfwd = 0, ninv = 0, nrow = 0, ncol = 0, ncrt = 0, ncarry = 0;
   size_t spec_b = 0, coef_b = 0, tasks_b = 0, pre_b = 0, fun_b = 0;
   size_t total() const { return spec_b + coef_b + tasks_b + pre_b + 2 * fun_b; }
 };
@@ -242,7 +242,8 @@ uint32_t crt_blocks(const RfPlan& pl) {
 Layout layout_of(std::vector<Product>& prods) {
   Layout L;
   // task counts per launch group, as build_pipeline groups them
-  std::map<int, size_t> fwd_n, inv_n, crt_n;
+  std::map<int, size_t> fwd_n, inv_n, col_n, crt_n;
+  size_t row_n = 0;
   for (auto& P : prods) {
     const RfPlan& pl = P.plan;
     P.spec_off = L.spec_n;
@@ -253,7 +254,14 @@ Layout layout_of(std::vector<Product>& prods) {
     L.spec_n += (uint64_t)(rd + dd) * pl.k * N;
     L.coef_n += (uint64_t)rd * pl.k * N;
     fwd_n[pl.lgN] += (size_t)(rd + dd) * pl.k;
-    inv_n[pl.lgN] += (size_t)rd * pl.k;
+    if (pl.lgN > RF_LGN_LDS) {
+      const size_t rows = (size_t)1 << (pl.lgN - RF_LGN_ROW);
+      row_n += (size_t)(rd + dd) * pl.k * rows;
+      inv_n[RF_LGN_ROW] += (size_t)rd * pl.k * rows;
+      col_n[pl.lgN] += (size_t)rd * pl.k;
+    } else {
+      inv_n[pl.lgN] += (size_t)rd * pl.k;
+    }
     crt_n[pl.k] += (size_t)rd * crt_blocks(pl);
     L.ncarry += (size_t)rd;
     L.nblk += (uint32_t)(rd * crt_blocks(pl));
@@ -263,6 +271,9 @@ Layout layout_of(std::vector<Product>& prods) {
   size_t tb = 0;
   for (auto& kv : fwd_n) tb += align256(kv.second * sizeof(RfFwdTask)), L.nfwd += kv.second;
   for (auto& kv : inv_n) tb += align256(kv.second * sizeof(RfInvTask)), L.ninv += kv.second;
+  for (auto& kv : col_n) tb += align256(kv.second * sizeof(RfRowTask)), L.ncol += kv.second;
+  tb += align256(row_n * sizeof(RfRowTask));
+  L.nrow = row_n;
   for (auto& kv : crt_n) tb += align256(kv.second * sizeof(RfCrtTask)), L.ncrt += kv.second;
   tb += align256(L.ncarry * sizeof(RfCarryTask));
   L.tasks_b = tb;
@@ -276,8 +287,18 @@ Layout layout_of(std::vector<Product>& prods) {
 // tables hold absolute device pointers for limbs and outputs and offsets
 // into the scratch region for spectra and coefficients, so one image serves
 // every launch of the same batch.
+enum Kind { K_FWD, K_FWD_COL, K_FWD_ROW, K_INV, K_INV_COL, K_CRT, K_CARRY, K_APPLY };
+int stage_of(int kind) {
+  switch (kind) {
+    case K_FWD: case K_FWD_COL: case K_FWD_ROW: return RF_STAGE_FWD;
+    case K_INV: case K_INV_COL: return RF_STAGE_INV;
+    case K_CRT: return RF_STAGE_CRT;
+    case K_CARRY: return RF_STAGE_CARRY;
+    default: return RF_STAGE_APPLY;
+  }
+}
 struct LaunchRec {
-  int stage;   // RF_STAGE_*
+  int kind;    // Kind
   int param;   // lgN for transforms, k for CRT
   size_t off;  // byte offset of its task table in the image
   int n;       // tasks
@@ -302,6 +323,8 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   pp.launches.clear();
   std::map<int, std::vector<RfFwdTask>> fwd;
   std::map<int, std::vector<RfInvTask>> inv;
+  std::map<int, std::vector<RfRowTask>> icol;
+  std::vector<RfRowTask> frow;
   std::map<int, std::vector<RfCrtTask>> crt;
   std::vector<RfCarryTask> carry;
   uint32_t nblk_total = 0, npre = 0;
@@ -327,18 +350,30 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
         fwd[pl.lgN].push_back(t);
       }
     }
+    // Longer transforms: 2^(lgN-13) rows of 2^13 residues each.
+    const bool split = pl.lgN > RF_LGN_LDS;
+    const uint32_t rows = split ? 1u << (pl.lgN - RF_LGN_ROW) : 1u;
+    const uint64_t rowlen = split ? (uint64_t)1 << RF_LGN_ROW : N;
+    if (split)
+      for (int e = 0; e < r * d + d * d; e++)
+        for (int q = 0; q < k; q++)
+          for (uint32_t h = 0; h < rows; h++) frow.push_back(RfRowTask{spec_of(e, q) + h * rowlen, h, (uint16_t)q, 0});
     for (int q = 0; q < k; q++)
       for (int i = 0; i < r; i++)
         for (int j = 0; j < d; j++) {
-          RfInvTask t{};
-          t.a_off = spec_of(i * d, q);
-          t.a_stride = (uint32_t)(k * N);
-          t.b_off = spec_of(r * d + j, q);
-          t.b_stride = (uint32_t)(d * k * N);
-          t.out_off = P.coef_off + ((uint64_t)(i * d + j) * k + q) * N;
-          t.q = (uint16_t)q;
-          t.d = (uint16_t)d;
-          inv[pl.lgN].push_back(t);
+          const uint64_t out_off = P.coef_off + ((uint64_t)(i * d + j) * k + q) * N;
+          for (uint32_t h = 0; h < rows; h++) {
+            RfInvTask t{};
+            t.a_off = spec_of(i * d, q) + h * rowlen;
+            t.a_stride = (uint32_t)(k * N);
+            t.b_off = spec_of(r * d + j, q) + h * rowlen;
+            t.b_stride = (uint32_t)(d * k * N);
+            t.out_off = out_off + h * rowlen;
+            t.q = (uint16_t)q;
+            t.d = (uint16_t)d;
+            inv[split ? RF_LGN_ROW : pl.lgN].push_back(t);
+          }
+          if (split) icol[pl.lgN].push_back(RfRowTask{out_off, 0, (uint16_t)q, 0});
         }
     const uint32_t nblk = crt_blocks(pl);
     for (int e = 0; e < r * d; e++) {
@@ -364,18 +399,25 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
       nblk_total += nblk;
     }
   }
+  // Stream order: every forward pass (column passes before the row pass),
+  // then every inverse pass (row passes before the column passes), CRT,
+  // block carries, carry application.
   for (auto& kv : fwd)
-    pp.launches.push_back({RF_STAGE_FWD, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
+    pp.launches.push_back({kv.first > RF_LGN_LDS ? K_FWD_COL : K_FWD, kv.first,
+                           append_tasks(pp.image, kv.second), (int)kv.second.size()});
+  if (!frow.empty()) pp.launches.push_back({K_FWD_ROW, RF_LGN_ROW, append_tasks(pp.image, frow), (int)frow.size()});
   for (auto& kv : inv)
-    pp.launches.push_back({RF_STAGE_INV, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
+    pp.launches.push_back({K_INV, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
+  for (auto& kv : icol)
+    pp.launches.push_back({K_INV_COL, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
   std::vector<LaunchRec> crt_l;
   for (auto& kv : crt) {
     size_t off = append_tasks(pp.image, kv.second);
-    pp.launches.push_back({RF_STAGE_CRT, kv.first, off, (int)kv.second.size()});
-    crt_l.push_back({RF_STAGE_APPLY, kv.first, off, (int)kv.second.size()});
+    pp.launches.push_back({K_CRT, kv.first, off, (int)kv.second.size()});
+    crt_l.push_back({K_APPLY, kv.first, off, (int)kv.second.size()});
   }
   if (!carry.empty())
-    pp.launches.push_back({RF_STAGE_CARRY, 0, append_tasks(pp.image, carry), (int)carry.size()});
+    pp.launches.push_back({K_CARRY, 0, append_tasks(pp.image, carry), (int)carry.size()});
   for (auto& l : crt_l) pp.launches.push_back(l);
   if (pp.image.size() > pp.L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
   return RF_OK;
@@ -406,22 +448,31 @@ ScratchPtrs scratch_ptrs(char* base, const Layout& L) {
 int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
   const ScratchPtrs S = scratch_ptrs(base, pp.L);
   for (const LaunchRec& l : pp.launches) {
-    ProfScope ps(prof ? l.stage : -1, st);
+    ProfScope ps(prof ? stage_of(l.kind) : -1, st);
     const char* t = S.tasks + l.off;
-    switch (l.stage) {
-      case RF_STAGE_FWD:
+    switch (l.kind) {
+      case K_FWD:
         HIPCHK(rf_launch_fwd(g.tables, l.param, (const RfFwdTask*)t, l.n, S.spec, st));
         break;
-      case RF_STAGE_INV:
+      case K_FWD_COL:
+        HIPCHK(rf_launch_fwd_col(g.tables, l.param, (const RfFwdTask*)t, l.n, S.spec, st));
+        break;
+      case K_FWD_ROW:
+        HIPCHK(rf_launch_fwd_row(g.tables, (const RfRowTask*)t, l.n, S.spec, st));
+        break;
+      case K_INV:
         HIPCHK(rf_launch_inv(g.tables, l.param, (const RfInvTask*)t, l.n, S.spec, S.coef, st));
         break;
-      case RF_STAGE_CRT:
+      case K_INV_COL:
+        HIPCHK(rf_launch_inv_col(g.tables, l.param, (const RfRowTask*)t, l.n, S.coef, st));
+        break;
+      case K_CRT:
         HIPCHK(rf_launch_crt(g.tables, l.param, (const RfCrtTask*)t, l.n, S.coef, S.pre, S.fun, st));
         break;
-      case RF_STAGE_CARRY:
+      case K_CARRY:
         HIPCHK(rf_launch_carry((const RfCarryTask*)t, l.n, S.fun, S.cin, st));
         break;
-      case RF_STAGE_APPLY:
+      case K_APPLY:
         HIPCHK(rf_launch_apply((const RfCrtTask*)t, l.n, S.pre, S.cin, st));
         break;
     }
@@ -913,8 +964,10 @@ int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs,
   const size_t N = (size_t)1 << lgN;
   size_t total = 0;
   for (int e = 0; e < n; e++) total = std::max(total, off[e] + nlimbs[e]);
+  const bool split = lgN > RF_LGN_LDS;
+  const uint32_t rows = split ? 1u << (lgN - RF_LGN_ROW) : 0u;
   const size_t in_b = align256(total * 8 + 8), spec_b = align256((size_t)n * N * 8),
-               task_b = align256(n * sizeof(RfFwdTask));
+               task_b = align256(n * sizeof(RfFwdTask)) + align256((size_t)n * rows * sizeof(RfRowTask));
   rc = ensure_arena(in_b + spec_b + task_b);
   if (rc) return rc;
   rc = ensure_stage(std::max(in_b + task_b, spec_b));
@@ -937,9 +990,19 @@ int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs,
     tk.scale = (uint8_t)scale;
     ht[e] = tk;
   }
+  RfRowTask* hr = (RfRowTask*)((char*)ht + align256(n * sizeof(RfFwdTask)));
+  RfRowTask* d_row = (RfRowTask*)((char*)d_task + align256(n * sizeof(RfFwdTask)));
+  for (int e = 0; e < n; e++)
+    for (uint32_t h = 0; h < rows; h++)
+      hr[(size_t)e * rows + h] = RfRowTask{(uint64_t)e * N + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, 0};
   HIPCHK(hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream));
-  HIPCHK(hipMemcpyAsync(d_task, ht, n * sizeof(RfFwdTask), hipMemcpyHostToDevice, g.stream));
-  HIPCHK(rf_launch_fwd(g.tables, lgN, d_task, n, d_spec, g.stream));
+  HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
+  if (split) {
+    HIPCHK(rf_launch_fwd_col(g.tables, lgN, d_task, n, d_spec, g.stream));
+    HIPCHK(rf_launch_fwd_row(g.tables, d_row, n * (int)rows, d_spec, g.stream));
+  } else {
+    HIPCHK(rf_launch_fwd(g.tables, lgN, d_task, n, d_spec, g.stream));
+  }
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(g.h_stage, d_spec, (size_t)n * N * 8, hipMemcpyDeviceToHost));
   canon_out((const double*)g.h_stage, (size_t)n * N, g.primes.p[q], out);
@@ -955,8 +1018,12 @@ int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int
     return fail(RF_ERR_ARG, "rf_stage_inv: bad arguments");
   const size_t N = (size_t)1 << lgN;
   const size_t np = (size_t)(r * d + d * d);
+  const bool split = lgN > RF_LGN_LDS;
+  const uint32_t rows = split ? 1u << (lgN - RF_LGN_ROW) : 1u;
+  const uint64_t rowlen = (uint64_t)N / rows;
+  const size_t ninv = (size_t)r * d * rows;
   const size_t spec_b = align256(np * N * 8), coef_b = align256((size_t)r * d * N * 8),
-               task_b = align256((size_t)r * d * sizeof(RfInvTask));
+               task_b = align256(ninv * sizeof(RfInvTask)) + align256((size_t)r * d * sizeof(RfRowTask));
   rc = ensure_arena(spec_b + coef_b + task_b);
   if (rc) return rc;
   rc = ensure_stage(std::max(spec_b + task_b, coef_b));
@@ -977,21 +1044,31 @@ int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int
   double* d_coef = (double*)(g.d_arena + spec_b);
   RfInvTask* d_task = (RfInvTask*)(g.d_arena + spec_b + coef_b);
   RfInvTask* ht = (RfInvTask*)(g.h_stage + spec_b);
+  RfRowTask* hc = (RfRowTask*)((char*)ht + align256(ninv * sizeof(RfInvTask)));
+  RfRowTask* d_col = (RfRowTask*)((char*)d_task + align256(ninv * sizeof(RfInvTask)));
   for (int i = 0; i < r; i++)
     for (int j = 0; j < d; j++) {
-      RfInvTask tk{};
-      tk.a_off = (uint64_t)(i * d) * N;
-      tk.a_stride = (uint32_t)N;
-      tk.b_off = (uint64_t)(r * d + j) * N;
-      tk.b_stride = (uint32_t)(d * N);
-      tk.out_off = (uint64_t)(i * d + j) * N;
-      tk.q = (uint16_t)q;
-      tk.d = (uint16_t)d;
-      ht[i * d + j] = tk;
+      for (uint32_t h = 0; h < rows; h++) {
+        RfInvTask tk{};
+        tk.a_off = (uint64_t)(i * d) * N + h * rowlen;
+        tk.a_stride = (uint32_t)N;
+        tk.b_off = (uint64_t)(r * d + j) * N + h * rowlen;
+        tk.b_stride = (uint32_t)(d * N);
+        tk.out_off = (uint64_t)(i * d + j) * N + h * rowlen;
+        tk.q = (uint16_t)q;
+        tk.d = (uint16_t)d;
+        ht[(size_t)(i * d + j) * rows + h] = tk;
+      }
+      hc[i * d + j] = RfRowTask{(uint64_t)(i * d + j) * N, 0, (uint16_t)q, 0};
     }
   HIPCHK(hipMemcpyAsync(d_spec, hs, np * N * 8, hipMemcpyHostToDevice, g.stream));
-  HIPCHK(hipMemcpyAsync(d_task, ht, (size_t)r * d * sizeof(RfInvTask), hipMemcpyHostToDevice, g.stream));
-  HIPCHK(rf_launch_inv(g.tables, lgN, d_task, r * d, d_spec, d_coef, g.stream));
+  HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
+  if (split) {
+    HIPCHK(rf_launch_inv(g.tables, RF_LGN_ROW, d_task, (int)ninv, d_spec, d_coef, g.stream));
+    HIPCHK(rf_launch_inv_col(g.tables, lgN, d_col, r * d, d_coef, g.stream));
+  } else {
+    HIPCHK(rf_launch_inv(g.tables, lgN, d_task, r * d, d_spec, d_coef, g.stream));
+  }
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(g.h_stage, d_coef, (size_t)r * d * N * 8, hipMemcpyDeviceToHost));
   canon_out((const double*)g.h_stage, (size_t)r * d * N, p, out);

@@ -37,6 +37,20 @@ struct RfInvTask {
   uint16_t d;
 };
 
+// Transforms longer than 2^RF_LGN_LDS points, N = 2^(13 + n1), view the
+// array as 2^n1 rows of 2^13 (i = row 2^13 + col).  Forward: a column pass
+// (pack + the first n1 stages, one column per thread) then a row pass (the
+// last 13 stages of each row in LDS).  Inverse: a row pass (pointwise product
+// + the first 13 stages per row, an RfInvTask per row) then a column pass
+// (the last n1 stages).  Both passes work in place on the same array.
+struct RfRowTask {
+  uint64_t off;       // first residue of the row (forward rows) or of the array (inverse columns)
+  uint32_t row;       // row index (forward rows; its bits precede the row's own in the twiddle index)
+  uint16_t q;
+  uint16_t pad;
+};
+#define RF_COL_THREADS 256
+
 // One CRT/recompose block of one C entry: RF_CRT_THREADS coefficients, one
 // per thread, covering the digits [digit0, digit0 + RF_CRT_THREADS - halo);
 // the first `halo` coefficients only feed those digits' overlaps, so
@@ -98,6 +112,13 @@ hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks,
                          double* spec, hipStream_t st);
 hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
                          const double* spec, double* coef, hipStream_t st);
+// Column and row passes of the longer transforms (lgN in RF_LGN_LDS+1 .. RF_LGN_MAX).
+hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+                             double* spec, hipStream_t st);
+hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int ntasks, double* spec,
+                             hipStream_t st);
+hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* tasks, int ntasks,
+                             double* coef, hipStream_t st);
 // Blocks with the same k must be launched together (the kernel is specialised on k).
 hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, int ntasks,
                          const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st);

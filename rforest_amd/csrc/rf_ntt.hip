@@ -88,10 +88,12 @@ __device__ __forceinline__ int inv_index(int g, int j) {
 
 // R forward stages on NG groups held in v[u * 2^R + j].  At stage L0 + s the
 // pair (j, j + 2^(R-1-s)) of group (H, lo) uses twiddle T[H 2^s + (j >> (R-s))]
-// (T[m] = omega^(rev(m)), shared prefix for all lengths).
-template <int LGN, int P, int S0 = 0>
+// (T[m] = omega^(rev(m)), shared prefix for all lengths).  A row of a longer
+// transform (ROW) prefixes its row index hb to the block index:
+// T[(hb 2^L0 + H) 2^s + (j >> (R-s))].
+template <int LGN, int P, int S0 = 0, bool ROW = false>
 __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __restrict__ tw,
-                                         double p) {
+                                         double p, int hb = 0) {
   using Q = Pass<LGN, P>;
   constexpr int NLO = LGN - Q::L0 - Q::R;
 #pragma unroll
@@ -100,11 +102,12 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __re
 #pragma unroll
     for (int u = 0; u < Q::NG; u++) {
       const int g = tid + Shape<LGN>::T * u;
-      const int H = g >> NLO;
+      const int H = (ROW ? (hb << Q::L0) : 0) + (g >> NLO);
 #pragma unroll
       for (int b = 0; b < (1 << s); b++) {
-        // in pass 0 (H = 0) block 0 uses T[0] = 1: no multiplication
-        const bool one = (P == 0) && (b == 0);
+        // in pass 0 of a whole transform (H = 0) block 0 uses T[0] = 1: no
+        // multiplication
+        const bool one = !ROW && (P == 0) && (b == 0);
         const double2 w = one ? make_double2(1.0, 0.0) : tw[(H << s) + b];
 #pragma unroll
         for (int jj = 0; jj < half; jj++) {
@@ -159,9 +162,9 @@ __device__ __forceinline__ void reduce_all(double* v, double p, double pinv) {
 
 // Passes 1 .. NP-1 of the forward transform: LDS exchange, optional
 // reduction, stages.
-template <int LGN, int P>
+template <int LGN, int P, bool ROW = false>
 __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const double2* tw,
-                                         double p, double pinv) {
+                                         double p, double pinv, int hb = 0) {
   if constexpr (P < Shape<LGN>::NP) {
     using Q = Pass<LGN, P>;
     using Qp = Pass<LGN, P - 1>;
@@ -178,8 +181,8 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const d
       for (int j = 0; j < Q::GS; j++)
         v[u * Q::GS + j] = sh[lds_pad(fwd_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
     __syncthreads();
-    fwd_pass<LGN, P>(v, tid, tw, p);
-    fwd_rest<LGN, P + 1>(v, sh, tid, tw, p, pinv);
+    fwd_pass<LGN, P, 0, ROW>(v, tid, tw, p, hb);
+    fwd_rest<LGN, P + 1, ROW>(v, sh, tid, tw, p, pinv, hb);
   }
 }
 
@@ -403,6 +406,152 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
       out[inv_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
 }
 
+// ---- transforms longer than one workgroup: N = 2^(RF_LGN_ROW + N1) ----
+
+// Forward column pass: thread `col` packs chunks row 2^13 + col (rows
+// 0 .. 2^N1 - 1) and runs the first N1 stages, whose pairs differ in the row
+// bits only; at stage s the pair (hi, hi + 2^(N1-1-s)) uses T[hi >> (N1-s)],
+// the same for every column, so the whole pass is one 2^N1-point transform
+// per column.  Output in place, reduced.
+template <int N1>
+__global__ __launch_bounds__(RF_COL_THREADS) void k_fwd_col(RfDevTables tb,
+                                                            const RfFwdTask* __restrict__ tasks,
+                                                            int ntasks, double* __restrict__ spec) {
+  constexpr int ROWS = 1 << N1, COLS = 1 << RF_LGN_ROW, BPT = COLS / RF_COL_THREADS;
+  constexpr uint32_t N = (uint32_t)ROWS * COLS;
+  const int w = xcd_task(blockIdx.x, gridDim.x, ntasks * BPT);
+  if (w < 0) return;
+  const RfFwdTask tk = rf_wave_uniform_load(tasks + w / BPT);
+  const int col = (w % BPT) * RF_COL_THREADS + (int)threadIdx.x;
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  const double(*pk)[3] = C->pk[tk.q][tk.scale];
+  const double* pk2 = C->pk2[tk.q];
+  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  const double sg = tk.sign < 0 ? -1.0 : 1.0;
+  const uint32_t nchunk = tk.sign != 0 ? tk.nchunk : 0u;
+  const bool half = nchunk <= N / 2;  // rows >= ROWS/2 are zero: stage 0 copies
+  constexpr int CB = ROWS / 2 < 8 ? ROWS / 2 : 8;
+  double v[ROWS];
+#pragma unroll
+  for (int r0 = 0; r0 < ROWS; r0 += CB) {
+    uint32_t ci[CB];
+    double x[CB];
+#pragma unroll
+    for (int c = 0; c < CB; c++) ci[c] = (uint32_t)(r0 + c) * COLS + (uint32_t)col;
+    if (nchunk && !(half && r0 >= ROWS / 2)) {
+      pack_chunks<CB>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+    } else {
+#pragma unroll
+      for (int c = 0; c < CB; c++) x[c] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < CB; c++) v[r0 + c] = ci[c] < nchunk ? sg * x[c] : 0.0;
+  }
+  if (half) {
+#pragma unroll
+    for (int j = 0; j < ROWS / 2; j++) v[j + ROWS / 2] = v[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < ROWS / 2; j++) {  // stage 0: T[0] = 1
+      const double x = v[j], y = v[j + ROWS / 2];
+      v[j] = x + y;
+      v[j + ROWS / 2] = x - y;
+    }
+  }
+#pragma unroll
+  for (int st = 1; st < N1; st++) {
+    const int hf = ROWS >> (st + 1);
+#pragma unroll
+    for (int b = 0; b < (1 << st); b++) {
+      const double2 wt = tw[b];
+#pragma unroll
+      for (int jj = 0; jj < hf; jj++) {
+        const int j0 = b * 2 * hf + jj, j1 = j0 + hf;
+        const double z = b == 0 ? v[j1] : rf_mulmod(v[j1], wt.x, wt.y, p);
+        const double x = v[j0];
+        v[j0] = x + z;
+        v[j1] = x - z;
+      }
+    }
+  }
+  double* out = spec + tk.spec_off;
+#pragma unroll
+  for (int j = 0; j < ROWS; j++) out[(size_t)j * COLS + col] = rf_reduce(v[j], p, pinv);
+}
+
+// Forward row pass: the last 13 stages of one row, in LDS, with the row index
+// prefixed to every twiddle index.  Output in place (bit-reversed overall).
+__global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTables tb,
+                                                                         const RfRowTask* __restrict__ tasks,
+                                                                         int ntasks, double* __restrict__ spec) {
+  constexpr int LGN = RF_LGN_ROW;
+  using S = Shape<LGN>;
+  using Q0 = Pass<LGN, 0>;
+  static_assert(S::G == 1, "one row per workgroup");
+  extern __shared__ double sh_all[];
+  const int tid = threadIdx.x;
+  const int wg = xcd_task(blockIdx.x, gridDim.x, ntasks);
+  if (wg < 0) return;
+  const RfRowTask tk = rf_wave_uniform_load(tasks + wg);
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  double* a = spec + tk.off;
+  double v[S::E];
+#pragma unroll
+  for (int u = 0; u < Q0::NG; u++)
+#pragma unroll
+    for (int j = 0; j < Q0::GS; j++) v[u * Q0::GS + j] = a[fwd_index<LGN, 0>(tid + S::T * u, j)];
+  fwd_pass<LGN, 0, 0, true>(v, tid, tw, p, (int)tk.row);
+  fwd_rest<LGN, 1, true>(v, sh_all, tid, tw, p, pinv, (int)tk.row);
+  using QL = Pass<LGN, S::NP - 1>;
+#pragma unroll
+  for (int u = 0; u < QL::NG; u++)
+#pragma unroll
+    for (int j = 0; j < QL::GS; j++)
+      a[fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+}
+
+// Inverse column pass: after the row pass (k_inv on each row) the last N1
+// stages L = 13 + s pair (hi, hi + 2^s) with twiddle
+// V[2^L + (hi mod 2^s) 2^13 + col].  Output in place, natural order, reduced.
+template <int N1>
+__global__ __launch_bounds__(RF_COL_THREADS) void k_inv_col(RfDevTables tb,
+                                                            const RfRowTask* __restrict__ tasks,
+                                                            int ntasks, double* __restrict__ coef) {
+  constexpr int ROWS = 1 << N1, COLS = 1 << RF_LGN_ROW, BPT = COLS / RF_COL_THREADS;
+  const int w = xcd_task(blockIdx.x, gridDim.x, ntasks * BPT);
+  if (w < 0) return;
+  const RfRowTask tk = rf_wave_uniform_load(tasks + w / BPT);
+  const int col = (w % BPT) * RF_COL_THREADS + (int)threadIdx.x;
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
+  double* a = coef + tk.off;
+  double v[ROWS];
+#pragma unroll
+  for (int j = 0; j < ROWS; j++) v[j] = a[(size_t)j * COLS + col];
+#pragma unroll
+  for (int st = 0; st < N1; st++) {
+    const int hf = 1 << st;
+#pragma unroll
+    for (int jj = 0; jj < hf; jj++) {
+      const double2 wt = tw[(1 << (RF_LGN_ROW + st)) + (jj << RF_LGN_ROW) + col];
+#pragma unroll
+      for (int b = 0; b < (ROWS >> (st + 1)); b++) {
+        const int j0 = b * 2 * hf + jj, j1 = j0 + hf;
+        const double z = rf_mulmod(v[j1], wt.x, wt.y, p);
+        const double x = v[j0];
+        v[j0] = x + z;
+        v[j1] = x - z;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < ROWS; j++) a[(size_t)j * COLS + col] = rf_reduce(v[j], p, pinv);
+}
+
 template <int LGN>
 hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntasks, double* spec,
                         hipStream_t st) {
@@ -455,6 +604,48 @@ hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks,
     return launch_inv_t<L>(tb, tasks, ntasks, spec, coef, st);
     RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
     RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
+#undef RF_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+                             double* spec, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  const dim3 grid(ntasks * ((1 << RF_LGN_ROW) / RF_COL_THREADS)), blk(RF_COL_THREADS);
+  switch (lgN - RF_LGN_ROW) {
+#define RF_CASE(N1) \
+  case N1:          \
+    hipLaunchKernelGGL(k_fwd_col<N1>, grid, blk, 0, st, tb, tasks, ntasks, spec); \
+    return hipGetLastError();
+    RF_CASE(2) RF_CASE(3) RF_CASE(4)
+#undef RF_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int ntasks, double* spec,
+                             hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  using S = Shape<RF_LGN_ROW>;
+  const int grid = (ntasks + 7) & ~7;
+  hipLaunchKernelGGL(k_fwd_row, dim3(grid), dim3(S::THREADS), S::LDSN * sizeof(double), st, tb, tasks,
+                     ntasks, spec);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* tasks, int ntasks,
+                             double* coef, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  const dim3 grid(ntasks * ((1 << RF_LGN_ROW) / RF_COL_THREADS)), blk(RF_COL_THREADS);
+  switch (lgN - RF_LGN_ROW) {
+#define RF_CASE(N1) \
+  case N1:          \
+    hipLaunchKernelGGL(k_inv_col<N1>, grid, blk, 0, st, tb, tasks, ntasks, coef); \
+    return hipGetLastError();
+    RF_CASE(2) RF_CASE(3) RF_CASE(4)
 #undef RF_CASE
     default:
       return hipErrorInvalidValue;

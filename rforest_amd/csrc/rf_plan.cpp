@@ -2,6 +2,8 @@
 #define RF_HOST_ONLY
 #include "rf_plan.h"
 
+#include <vector>
+
 #include <string.h>
 
 #include "rf_field.h"
@@ -151,8 +153,11 @@ int rf_plan_fixed(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t,
   double ops = polys * k * (ntt + (double)nA * (4 * words + 20)) +
                outs * k * (ntt + N * 8 * d) + outs * (double)nout * (5.0 * k * k + 6.0 * k * pl.s);
   double bytes = 8 * N * k * (polys + outs) * 2;
+  // Transforms longer than one workgroup's LDS take a column pass and a row
+  // pass: one more read and write of every residue, and a few more ops.
+  if (lgN > RF_LGN_LDS) bytes *= 2, ops *= 1.1;
   double c = ops > bytes * 39 / 8 ? ops : bytes * 39 / 8;
-  if (lgN == RF_LGN_MAX) c *= 1.25;  // one workgroup per CU at 128 KiB of LDS
+  if (lgN == RF_LGN_LDS) c *= 1.25;  // one workgroup per CU at 128 KiB of LDS
   pl.cost = c;
   *out = pl;
   return 0;
@@ -220,15 +225,18 @@ void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
 
 void rf_twiddles_init(const RfPrimes* pr, double* fwd, double* inv) {
   const int B = RF_LGN_MAX - 1;
+  std::vector<uint64_t> pw(RF_FWD_TW);
   for (int q = 0; q < pr->k; q++) {
     uint64_t p = pr->p[q];
     double dp = (double)p;
     uint64_t om = rf_h_powmod(pr->g[q], (uint64_t)1 << (RF_S - RF_LGN_MAX), p);  // order 2^LGN_MAX
+    pw[0] = 1;
+    for (int m = 1; m < RF_FWD_TW; m++) pw[m] = rf_h_mulmod(pw[m - 1], om, p);
     double* F = fwd + (size_t)q * RF_FWD_TW * 2;
     for (int m = 0; m < RF_FWD_TW; m++) {
       unsigned rev = 0;
       for (int i = 0; i < B; i++) rev |= ((m >> i) & 1u) << (B - 1 - i);
-      uint64_t w = rf_h_powmod(om, rev, p);
+      uint64_t w = pw[rev];
       F[2 * m] = (double)w;
       F[2 * m + 1] = (double)w / dp;
     }

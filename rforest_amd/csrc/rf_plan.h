@@ -18,7 +18,9 @@
 #define RF_PMAX ((((uint64_t)1) << 51) / 9)
 #define RF_S 24           // every prime is 1 mod 2^RF_S
 #define RF_LGN_MIN 1
-#define RF_LGN_MAX 14     // largest LDS-resident transform (2^14 doubles = 128 KiB)
+#define RF_LGN_LDS 14     // largest one-workgroup transform (2^14 doubles = 128 KiB of LDS)
+#define RF_LGN_ROW 13     // longer transforms: columns of 2^(lgN-13) in registers, rows of 2^13 in LDS
+#define RF_LGN_MAX 17     // largest transform (columns of at most 16 residues)
 #define RF_TMAX 8         // chunk width up to 8 limbs = 512 bits
 #define RF_MAXWORDS (2 * RF_TMAX)  // 32-bit words per chunk
 #define RF_MAXS 24        // CRT limbs per coefficient, >= ceil((48 RF_MAXP + 2) / 64)
@@ -52,7 +54,7 @@ int rf_floor_log2_P(const RfPrimes* pr, int k);
 
 // Chooses the cheapest proven plan for an r x d times d x d product whose A
 // entries have at most LA bits and B entries at most LB bits.  Returns 0, or
-// -1 if no plan within the LDS-resident transform sizes exists.
+// -1 if no plan with a transform of at most 2^RF_LGN_MAX points exists.
 int rf_plan_choose(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, RfPlan* out);
 
 // Fills a plan for a fixed (t, lgN); returns 0 or -1 if it cannot hold.

@@ -24,12 +24,16 @@ def libgmp():
             "clear": getattr(g, "__gmpz_clear"),
             "import": getattr(g, "__gmpz_import"),
             "neg": getattr(g, "__gmpz_neg"),
+            "mul": getattr(g, "__gmpz_mul"),
+            "addmul": getattr(g, "__gmpz_addmul"),
         }
         _g["init"].argtypes = [ctypes.POINTER(MPZ)]
         _g["clear"].argtypes = [ctypes.POINTER(MPZ)]
         _g["import"].argtypes = [ctypes.POINTER(MPZ), ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t,
                                  ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p]
         _g["neg"].argtypes = [ctypes.POINTER(MPZ), ctypes.POINTER(MPZ)]
+        for f in ("mul", "addmul"):
+            _g[f].argtypes = [ctypes.POINTER(MPZ), ctypes.POINTER(MPZ), ctypes.POINTER(MPZ)]
     return _g
 
 
@@ -88,3 +92,23 @@ class MpzArray:
             self.close()
         except Exception:
             pass
+
+
+def rmatrix_mult_schoolbook(A, r, B, d):
+    """C = A (r x d) * B (d x d) with GMP mpz_mul / mpz_addmul, the schoolbook
+    product of the reference's mpz_rmatrix_mult_naive
+    (src/include/hwmpz.h:117-124).  Lists of Python integers, row-major."""
+    g = libgmp()
+    mA, mB, mC = MpzArray(A), MpzArray(B), MpzArray.zeros(r * d)
+    try:
+        for i in range(r):
+            for j in range(d):
+                c = ctypes.byref(mC.arr[i * d + j])
+                g["mul"](c, ctypes.byref(mA.arr[i * d]), ctypes.byref(mB.arr[j]))
+                for l in range(1, d):
+                    g["addmul"](c, ctypes.byref(mA.arr[i * d + l]), ctypes.byref(mB.arr[l * d + j]))
+        return mC.values()
+    finally:
+        mA.close()
+        mB.close()
+        mC.close()

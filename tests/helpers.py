@@ -61,6 +61,15 @@ def sha(x):
 
 
 def naive(A, r, B, d):
+    """Schoolbook product: Python integers for small entries, GMP mpz_mul /
+    mpz_addmul for large ones (the two agree: test_abi checks it)."""
+    if max((abs(x).bit_length() for x in list(A) + list(B)), default=0) > 200000:
+        from rforest_amd.gmp import rmatrix_mult_schoolbook
+        return rmatrix_mult_schoolbook(A, r, B, d)
+    return naive_py(A, r, B, d)
+
+
+def naive_py(A, r, B, d):
     return [sum(A[i * d + k] * B[k * d + j] for k in range(d)) for i in range(r) for j in range(d)]
 
 

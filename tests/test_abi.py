@@ -66,10 +66,11 @@ def _bound_ok(t, lgN, k, la, lb, d):
 
 @pytest.mark.parametrize("la,lb,r,d", [(1 << 20, 1 << 20, 2, 2), (1 << 20, 1 << 20, 4, 4), (4096, 4096, 2, 2),
                                        (1 << 10, 1 << 10, 2, 2), (1 << 18, 1 << 17, 2, 2), (64, 64, 1, 1),
-                                       (4 << 20, 1 << 20, 3, 3), (1, 1, 1, 1), (3000, 50000, 2, 5)])
+                                       (4 << 20, 1 << 20, 3, 3), (1, 1, 1, 1), (3000, 50000, 2, 5),
+                                       (1 << 24, 1 << 24, 4, 4), (5 << 20, 5 << 20, 2, 2), (1 << 25, 1 << 24, 2, 2)])
 def test_planner_bound_is_proven_and_minimal(la, lb, r, d):
     t, lgN, k = rf.plan(la, lb, r, d)
-    assert 1 <= t <= 8 and 1 <= lgN <= 14
+    assert 1 <= t <= 8 and 1 <= lgN <= 17
     assert _bound_ok(t, lgN, k, la, lb, d)
     if k > 1:  # one prime fewer must not satisfy the exact bound
         b = 64 * t
@@ -85,10 +86,30 @@ def test_planner_headline_plan():
     assert rf.plan(1 << 20, 1 << 20, 2, 2) == (4, 13, 11)
 
 
+def test_long_transform_plans():
+    # entries beyond 4 Mbit need more than 2^14 points: column + row passes
+    for la, r in [(1 << 24, 2), (1 << 24, 4), (5 << 20, 2)]:
+        t, lgN, k = rf.plan(la, la, r, r)
+        assert 14 < lgN <= 17 and _bound_ok(t, lgN, k, la, la, r)
+
+
 def test_unsupported_sizes_are_reported():
+    # 2^26-bit entries would need 2^18 points at the widest chunks
     with pytest.raises(rf.RfError) as e:
-        rf.plan(1 << 24, 1 << 24, 2, 2)
+        rf.plan(1 << 26, 1 << 26, 2, 2)
     assert e.value.code == rf.RF_ERR_UNSUPPORTED
+
+
+def test_gmp_schoolbook_matches_python_integers():
+    import random
+    from helpers import naive_py
+    from rforest_amd.gmp import rmatrix_mult_schoolbook
+    rng = random.Random(9)
+    for bits, r, d in [(1, 1, 1), (64, 2, 2), (5000, 3, 2), (100000, 2, 4)]:
+        A = [rng.getrandbits(bits) * rng.choice([-1, 1]) for _ in range(r * d)]
+        B = [rng.getrandbits(bits) * rng.choice([-1, 1]) for _ in range(d * d)]
+        A[0] = 0
+        assert rmatrix_mult_schoolbook(A, r, B, d) == naive_py(A, r, B, d)
 
 
 def test_out_limbs_formula():

@@ -57,6 +57,17 @@ def test_plain_gpu_case_4x4_seed_1002():
     assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == naive(A, r, B, d)
 
 
+@pytest.mark.parametrize("bits,r,d", [(5 << 20, 2, 2), (1 << 24, 2, 2), (3 << 21, 3, 3)])
+def test_long_transform_products(bits, r, d):
+    """Entries whose plans need more than 2^14 points (column + row passes),
+    against GMP schoolbook."""
+    rng = random.Random(bits + r)
+    t, lgN, k = rf.plan(bits, bits, r, d)
+    assert lgN > 14
+    A, B = _rand_product(rng, bits, r, d)
+    assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == naive(A, r, B, d)
+
+
 def test_zero_and_ragged_entries():
     rng = random.Random(11)
     A = [0, rng.getrandbits(9000), -rng.getrandbits(10), 0]
@@ -108,7 +119,7 @@ def test_dropin_mpz_abi():
 
 def test_unsupported_size_returns_error():
     rng = random.Random(1)
-    big = [rng.getrandbits(1 << 24) for _ in range(4)]
+    big = [rng.getrandbits(1 << 26) for _ in range(4)]
     mA = MpzArray(big)
     mC = MpzArray.zeros(4)
     assert rf.lib().mpz_rmatrix_gpu_fft_suitable(mA.ptr(), 2, mA.ptr(), 2) == 0

@@ -63,6 +63,41 @@ def test_fwd_stage_full_length_input():
         assert got[0] == _expect_fwd(ents[0][0], -1, 2, N, lgN, P[0], 0)
 
 
+@pytest.mark.parametrize("lgN", [15, 16, 17])
+def test_fwd_stage_long(lgN):
+    """Column pass + row pass transforms (beyond one workgroup's LDS): half
+    filled (the column pass's copy stage), full length, and a short entry
+    (whole rows of zeros)."""
+    rng = random.Random(lgN)
+    P = rf.primes(32)
+    N = 1 << lgN
+    for t, nchunk, scale in ((1, N // 2, 0), (8, N // 2, lgN), (2, N, 0), (3, 1000, lgN)):
+        ents = [([rng.getrandbits(64) for _ in range(nchunk * t - 1)], 1),
+                ([rng.getrandbits(64) for _ in range(nchunk * t)], -1)]
+        for q in (0, 23):
+            got = stages.fwd(ents, t, nchunk, lgN, q, scale)
+            for e, (lv, s) in enumerate(ents):
+                assert got[e] == _expect_fwd(lv, s, t, nchunk, lgN, P[q], scale), (t, nchunk, q, e)
+
+
+@pytest.mark.parametrize("lgN", [15, 16, 17])
+@pytest.mark.parametrize("r,d", [(1, 1), (2, 2), (1, 3)])
+def test_inv_stage_long(lgN, r, d):
+    rng = random.Random(lgN * 100 + r * 10 + d)
+    P = rf.primes(32)
+    N = 1 << lgN
+    q = rng.randrange(32)
+    p = P[q]
+    w = oracle.root(p, 24, lgN)
+    A = [[rng.randrange(p) for _ in range(N)] for _ in range(r * d)]
+    B = [[rng.randrange(p) for _ in range(N)] for _ in range(d * d)]
+    A[0][: N // 2] = [p - 1] * (N // 2)
+    got = stages.inv(A, B, r, d, lgN, q)
+    C = oracle.pointwise(sum(A, []), sum(B, []), r, d, N, p)
+    for e in range(r * d):
+        assert got[e] == oracle.ntt_inv(C[e * N:(e + 1) * N], lgN, p, w, False)
+
+
 @pytest.mark.parametrize("lgN", [1, 2, 3, 6, 10, 13, 14])
 @pytest.mark.parametrize("r,d", [(1, 1), (2, 2), (3, 2), (4, 4), (1, 9)])
 def test_inv_stage(lgN, r, d):
