@@ -97,11 +97,14 @@ typedef struct rf_dev_out {
   uint64_t cap;          /* capacity in limbs */
 } rf_dev_out;
 
+typedef struct rf_operand rf_operand;
+
 typedef struct rf_dev_product {
   int32_t r, d;
-  const rf_dev_entry* A; /* host array of r*d descriptors */
+  const rf_dev_entry* A; /* host array of r*d descriptors (ignored when left is set) */
   const rf_dev_entry* B; /* host array of d*d descriptors */
   const rf_dev_out* C;   /* host array of r*d descriptors */
+  const rf_operand* left; /* NULL, or a resident left operand used as A (same r, d) */
 } rf_dev_product;
 
 /* Output limbs (two's complement, sign included) written for every C entry of
@@ -125,6 +128,26 @@ int rf_batch_create(const rf_dev_product* prods, int nprod, rf_batch** out);
    hipGraph and replayed.  Completion: rf_device_sync or the stream. */
 int rf_batch_launch(rf_batch* b, void* stream, int use_graph);
 void rf_batch_destroy(rf_batch* b);
+
+/* ---------------------------------------------------------------------------
+ * Resident operands: a left matrix transformed once and kept in the NTT
+ * domain in device memory, reused by every product it enters.  The
+ * reference's counterpart is an array of mpzfft_t filled by mpzfft_fft and
+ * combined by mpzfft_matrix_mul (src/include/mpzfft.h:52-118,
+ * src/lib/mpzfft.c:10-23); here the plan (chunk width, transform length,
+ * primes) is fixed at creation for right operands of up to b_max_bits bits
+ * per entry, and a product whose right operand does not fit that plan is
+ * refused with RF_ERR_ARG.
+ * ------------------------------------------------------------------------- */
+/* From r*d device entries (the limbs are only read during the call). */
+int rf_operand_create_device(const rf_dev_entry* A, int r, int d, uint64_t b_max_bits, rf_operand** out);
+/* From r*d host mpz_t entries. */
+int rf_operand_create(mpz_t* A, int r, int d, uint64_t b_max_bits, rf_operand** out);
+/* C (r x d) = left (r x d) * B (d x d) on mpz_t, like mpz_rmatrix_mult_fft. */
+int rf_operand_mult(const rf_operand* left, mpz_t* C, mpz_t* B);
+/* The fixed plan: chunk width 64*t bits, transform 2^lgN, k primes. */
+int rf_operand_plan(const rf_operand* op, int* t, int* lgN, int* k);
+void rf_operand_destroy(rf_operand* op);
 
 /* ---------------------------------------------------------------------------
  * Device and pinned memory (src/include/hwmpz.h:38-46 declares the same

@@ -51,7 +51,7 @@ class DevOut(ctypes.Structure):
 class DevProduct(ctypes.Structure):
     _fields_ = [("r", ctypes.c_int32), ("d", ctypes.c_int32),
                 ("A", ctypes.POINTER(DevEntry)), ("B", ctypes.POINTER(DevEntry)),
-                ("C", ctypes.POINTER(DevOut))]
+                ("C", ctypes.POINTER(DevOut)), ("left", ctypes.c_void_p)]
 
 
 def lib():
@@ -111,6 +111,18 @@ def lib():
     L.rf_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     L.rf_batch_destroy.restype = None
     L.rf_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.rf_operand_create_device.restype = ctypes.c_int
+    L.rf_operand_create_device.argtypes = [ctypes.POINTER(DevEntry), ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                           ctypes.POINTER(ctypes.c_void_p)]
+    L.rf_operand_create.restype = ctypes.c_int
+    L.rf_operand_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_void_p)]
+    L.rf_operand_mult.restype = ctypes.c_int
+    L.rf_operand_mult.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.rf_operand_plan.restype = ctypes.c_int
+    L.rf_operand_plan.argtypes = [ctypes.c_void_p, _intp, _intp, _intp]
+    L.rf_operand_destroy.restype = None
+    L.rf_operand_destroy.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -250,4 +262,4 @@ def profile(enable=None, reset=False):
     return {STAGE_NAMES[i]: (ms[i], int(n[i])) for i in range(5)}
 
 
-from .device import DeviceBatch, DeviceBuffer, device_sync  # noqa: E402,F401
+from .device import DeviceBatch, DeviceBuffer, ResidentOperand, device_sync  # noqa: E402,F401

@@ -30,6 +30,16 @@ int hw_disable_fft = 0;
 int mpzfft_threads = 1;
 }
 
+// A resident left operand: r*d spectra, [entry][prime][N] like a batch's
+// spectra, computed once under a plan fixed at creation.
+struct rf_operand {
+  int r = 0, d = 0;
+  RfPlan plan{};
+  uint64_t la_bits = 0;     // largest entry, bits (0: the operand is zero)
+  std::vector<int> signs;   // per entry
+  double* spec = nullptr;   // device, r*d*k*N doubles
+};
+
 namespace {
 
 struct Ctx {
@@ -200,6 +210,7 @@ struct Entry {
 };
 struct Product {
   int r, d;
+  const rf_operand* left = nullptr;  // resident A: its spectra replace A's forward transforms
   std::vector<Entry> A, B;
   std::vector<uint64_t*> out;  // device pointers, plan.rn limbs each
   RfPlan plan;
@@ -210,9 +221,22 @@ int plan_product(Product& P) {
   uint64_t LA = 0, LB = 0;
   for (auto& e : P.A) LA = std::max(LA, e.sign ? e.bits : 0);
   for (auto& e : P.B) LB = std::max(LB, e.sign ? e.bits : 0);
+  if (P.left) {
+    const rf_operand& op = *P.left;
+    if (op.r != P.r || op.d != P.d) return fail(RF_ERR_ARG, "resident operand is %dx%d, product %dx%d", op.r, op.d, P.r, P.d);
+    if (op.la_bits == 0 || LB == 0) {  // a zero side: the product is zero
+      rf_plan_fixed(&g.primes, 0, 0, P.r, P.d, 1, RF_LGN_MIN, &P.plan);
+      return RF_OK;
+    }
+    const RfPlan& f = op.plan;
+    if (rf_plan_fixed_k(&g.primes, op.la_bits, LB, P.r, P.d, f.t, f.lgN, f.k, &P.plan) != 0 || P.plan.k != f.k)
+      return fail(RF_ERR_ARG, "right operand of %llu bits does not fit the resident operand's plan",
+                  (unsigned long long)LB);
+    return RF_OK;
+  }
   if (rf_plan_choose(&g.primes, LA, LB, P.r, P.d, &P.plan) != 0)
     return fail(RF_ERR_UNSUPPORTED,
-                "no LDS-resident transform plan for entries of %llu x %llu bits (r=%d, d=%d)",
+                "no transform plan for entries of %llu x %llu bits (r=%d, d=%d)",
                 (unsigned long long)LA, (unsigned long long)LB, P.r, P.d);
   return RF_OK;
 }
@@ -251,12 +275,13 @@ Layout layout_of(std::vector<Product>& prods) {
     if (pl.k == 0) continue;
     const uint64_t N = (uint64_t)1 << pl.lgN;
     const int rd = P.r * P.d, dd = P.d * P.d;
-    L.spec_n += (uint64_t)(rd + dd) * pl.k * N;
+    const int npoly = P.left ? dd : rd + dd;  // polys transformed here
+    L.spec_n += (uint64_t)npoly * pl.k * N;
     L.coef_n += (uint64_t)rd * pl.k * N;
-    fwd_n[pl.lgN] += (size_t)(rd + dd) * pl.k;
+    fwd_n[pl.lgN] += (size_t)npoly * pl.k;
     if (pl.lgN > RF_LGN_LDS) {
       const size_t rows = (size_t)1 << (pl.lgN - RF_LGN_ROW);
-      row_n += (size_t)(rd + dd) * pl.k * rows;
+      row_n += (size_t)npoly * pl.k * rows;
       inv_n[RF_LGN_ROW] += (size_t)rd * pl.k * rows;
       col_n[pl.lgN] += (size_t)rd * pl.k;
     } else {
@@ -270,6 +295,8 @@ Layout layout_of(std::vector<Product>& prods) {
   L.coef_b = align256(L.coef_n * 8);
   size_t tb = 0;
   for (auto& kv : fwd_n) tb += align256(kv.second * sizeof(RfFwdTask)), L.nfwd += kv.second;
+  // inverse launches split further by A base (one per resident operand)
+  tb += 256 * (prods.size() + 1);
   for (auto& kv : inv_n) tb += align256(kv.second * sizeof(RfInvTask)), L.ninv += kv.second;
   for (auto& kv : col_n) tb += align256(kv.second * sizeof(RfRowTask)), L.ncol += kv.second;
   tb += align256(row_n * sizeof(RfRowTask));
@@ -302,6 +329,7 @@ struct LaunchRec {
   int param;   // lgN for transforms, k for CRT
   size_t off;  // byte offset of its task table in the image
   int n;       // tasks
+  const double* abase = nullptr;  // K_INV: A spectra base (nullptr: the batch's spectra)
 };
 struct Pipeline {
   Layout L;
@@ -322,7 +350,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   pp.image.clear();
   pp.launches.clear();
   std::map<int, std::vector<RfFwdTask>> fwd;
-  std::map<int, std::vector<RfInvTask>> inv;
+  std::map<std::pair<int, const double*>, std::vector<RfInvTask>> inv;
   std::map<int, std::vector<RfRowTask>> icol;
   std::vector<RfRowTask> frow;
   std::map<int, std::vector<RfCrtTask>> crt;
@@ -333,8 +361,18 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
     if (pl.k == 0) continue;
     const uint64_t N = (uint64_t)1 << pl.lgN;
     const int r = P.r, d = P.d, k = pl.k;
-    auto spec_of = [&](int poly, int q) { return P.spec_off + ((uint64_t)poly * k + q) * N; };
-    for (int e = 0; e < r * d + d * d; e++) {
+    // poly e < r*d is A's entry e, the rest are B's; with a resident A the
+    // batch's spectra hold B only and A's offsets count from its own base
+    const int rd = r * d;
+    auto spec_of = [&](int poly, int q) {
+      const int pos = P.left ? poly - rd : poly;
+      return P.spec_off + ((uint64_t)pos * k + q) * N;
+    };
+    auto aspec_of = [&](int poly, int q) {
+      return P.left ? ((uint64_t)poly * k + q) * N : spec_of(poly, q);
+    };
+    const double* abase = P.left ? P.left->spec : nullptr;
+    for (int e = P.left ? rd : 0; e < r * d + d * d; e++) {
       const Entry& E = e < r * d ? P.A[e] : P.B[e - r * d];
       const bool isB = e >= r * d;
       for (int q = 0; q < k; q++) {
@@ -355,7 +393,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
     const uint32_t rows = split ? 1u << (pl.lgN - RF_LGN_ROW) : 1u;
     const uint64_t rowlen = split ? (uint64_t)1 << RF_LGN_ROW : N;
     if (split)
-      for (int e = 0; e < r * d + d * d; e++)
+      for (int e = P.left ? rd : 0; e < r * d + d * d; e++)
         for (int q = 0; q < k; q++)
           for (uint32_t h = 0; h < rows; h++) frow.push_back(RfRowTask{spec_of(e, q) + h * rowlen, h, (uint16_t)q, 0});
     for (int q = 0; q < k; q++)
@@ -364,14 +402,14 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
           const uint64_t out_off = P.coef_off + ((uint64_t)(i * d + j) * k + q) * N;
           for (uint32_t h = 0; h < rows; h++) {
             RfInvTask t{};
-            t.a_off = spec_of(i * d, q) + h * rowlen;
+            t.a_off = aspec_of(i * d, q) + h * rowlen;
             t.a_stride = (uint32_t)(k * N);
             t.b_off = spec_of(r * d + j, q) + h * rowlen;
             t.b_stride = (uint32_t)(d * k * N);
             t.out_off = out_off + h * rowlen;
             t.q = (uint16_t)q;
             t.d = (uint16_t)d;
-            inv[split ? RF_LGN_ROW : pl.lgN].push_back(t);
+            inv[{split ? RF_LGN_ROW : pl.lgN, abase}].push_back(t);
           }
           if (split) icol[pl.lgN].push_back(RfRowTask{out_off, 0, (uint16_t)q, 0});
         }
@@ -407,7 +445,8 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
                            append_tasks(pp.image, kv.second), (int)kv.second.size()});
   if (!frow.empty()) pp.launches.push_back({K_FWD_ROW, RF_LGN_ROW, append_tasks(pp.image, frow), (int)frow.size()});
   for (auto& kv : inv)
-    pp.launches.push_back({K_INV, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
+    pp.launches.push_back({K_INV, kv.first.first, append_tasks(pp.image, kv.second), (int)kv.second.size(),
+                           kv.first.second});
   for (auto& kv : icol)
     pp.launches.push_back({K_INV_COL, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
   std::vector<LaunchRec> crt_l;
@@ -461,7 +500,8 @@ int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
         HIPCHK(rf_launch_fwd_row(g.tables, (const RfRowTask*)t, l.n, S.spec, st));
         break;
       case K_INV:
-        HIPCHK(rf_launch_inv(g.tables, l.param, (const RfInvTask*)t, l.n, S.spec, S.coef, st));
+        HIPCHK(rf_launch_inv(g.tables, l.param, (const RfInvTask*)t, l.n, l.abase ? l.abase : S.spec, S.spec,
+                             S.coef, st));
         break;
       case K_INV_COL:
         HIPCHK(rf_launch_inv_col(g.tables, l.param, (const RfRowTask*)t, l.n, S.coef, st));
@@ -510,13 +550,14 @@ uint64_t bitlen_limbs(const uint64_t* x, uint64_t n) {
 // two's complement, rn limbs per C entry, in out_tc.
 int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, const int* as,
                  const uint64_t* const* bp, const uint64_t* bn, const int* bs,
-                 std::vector<std::vector<uint64_t>>& out_tc) {
+                 std::vector<std::vector<uint64_t>>& out_tc, const rf_operand* left = nullptr) {
   if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
   Product P;
   P.r = r;
   P.d = d;
+  P.left = left;  // when set, A's arrays are not read
   uint64_t total_limbs = 0;
-  for (int e = 0; e < r * d + d * d; e++) {
+  for (int e = left ? r * d : 0; e < r * d + d * d; e++) {
     const bool isA = e < r * d;
     const int ei = isA ? e : e - r * d;
     const uint64_t* p = isA ? ap[ei] : bp[ei];
@@ -547,7 +588,7 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
   HIPCHK(hipStreamSynchronize(g.stream));
   uint64_t* hs = (uint64_t*)g.h_stage;
   Product& Q = prods[0];
-  for (int e = 0; e < r * d + d * d; e++) {
+  for (int e = left ? r * d : 0; e < r * d + d * d; e++) {
     const bool isA = e < r * d;
     const int ei = isA ? e : e - r * d;
     const Entry& E = isA ? Q.A[ei] : Q.B[ei];
@@ -587,11 +628,11 @@ void tc_to_signmag(std::vector<uint64_t>& x, bool& neg) {
   while (!x.empty() && x.back() == 0) x.pop_back();
 }
 
-int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d) {
+int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* left = nullptr) {
   std::vector<const uint64_t*> ap(r * d), bp(d * d);
   std::vector<uint64_t> an(r * d), bn(d * d);
   std::vector<int> as(r * d), bs(d * d);
-  for (int e = 0; e < r * d; e++) {
+  for (int e = 0; e < (left ? 0 : r * d); e++) {
     ap[e] = (const uint64_t*)A[e]->_mp_d;
     an[e] = mpz_size(A[e]);
     as[e] = mpz_sgn(A[e]);
@@ -602,7 +643,7 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d) {
     bs[e] = mpz_sgn(B[e]);
   }
   std::vector<std::vector<uint64_t>> out;
-  int rc = host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), out);
+  int rc = host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), out, left);
   if (rc) return rc;
   for (int e = 0; e < r * d; e++) {
     bool neg = false;
@@ -712,8 +753,10 @@ int products_from_dev(const rf_dev_product* prods, int nprod, std::vector<Produc
     Product& P = ps[i];
     P.r = dp.r;
     P.d = dp.d;
-    if (P.r <= 0 || P.d <= 0 || !dp.A || !dp.B || !dp.C) return fail(RF_ERR_ARG, "product %d: bad descriptor", i);
-    for (int e = 0; e < P.r * P.d + P.d * P.d; e++) {
+    P.left = dp.left;
+    if (P.r <= 0 || P.d <= 0 || (!dp.A && !dp.left) || !dp.B || !dp.C)
+      return fail(RF_ERR_ARG, "product %d: bad descriptor", i);
+    for (int e = dp.left ? P.r * P.d : 0; e < P.r * P.d + P.d * P.d; e++) {
       const rf_dev_entry& x = e < P.r * P.d ? dp.A[e] : dp.B[e - P.r * P.d];
       Entry E{};
       E.dptr = x.limbs;
@@ -836,6 +879,160 @@ void rf_batch_destroy(rf_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->scratch) (void)hipFree(b->scratch);
   delete b;
+}
+
+}  // extern "C"
+
+namespace {
+// Forward transforms of a left operand's entries (device limbs in A[e].dptr)
+// into op->spec; task tables go to the arena at byte offset `base`.
+int operand_transform(rf_operand* op, const std::vector<Entry>& A, size_t base) {
+  const RfPlan& pl = op->plan;
+  const int k = pl.k, lgN = pl.lgN;
+  const uint64_t N = (uint64_t)1 << lgN;
+  const bool split = lgN > RF_LGN_LDS;
+  const uint32_t rows = split ? 1u << (lgN - RF_LGN_ROW) : 0u;
+  std::vector<RfFwdTask> ft;
+  std::vector<RfRowTask> rt;
+  for (size_t e = 0; e < A.size(); e++)
+    for (int q = 0; q < k; q++) {
+      RfFwdTask t{};
+      t.limbs = A[e].dptr;
+      t.spec_off = ((uint64_t)e * k + q) * N;
+      t.nlimbs = (uint32_t)A[e].nlimbs;
+      t.nchunk = A[e].sign ? (uint32_t)std::min<uint64_t>((A[e].nlimbs + pl.t - 1) / pl.t, N) : 0;
+      t.sign = A[e].sign;
+      t.q = (uint16_t)q;
+      t.t = (uint8_t)pl.t;
+      t.scale = 0;
+      ft.push_back(t);
+      for (uint32_t h = 0; h < rows; h++) rt.push_back(RfRowTask{t.spec_off + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, 0});
+    }
+  const size_t fb = align256(ft.size() * sizeof(RfFwdTask)), rb = align256(rt.size() * sizeof(RfRowTask));
+  int rc = ensure_arena(base + fb + rb);
+  if (rc) return rc;
+  rc = ensure_stage(fb + rb);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  memcpy(g.h_stage, ft.data(), ft.size() * sizeof(RfFwdTask));
+  if (!rt.empty()) memcpy(g.h_stage + fb, rt.data(), rt.size() * sizeof(RfRowTask));
+  RfFwdTask* d_ft = (RfFwdTask*)(g.d_arena + base);
+  RfRowTask* d_rt = (RfRowTask*)(g.d_arena + base + fb);
+  HIPCHK(hipMemcpyAsync(d_ft, g.h_stage, fb + rb, hipMemcpyHostToDevice, g.stream));
+  if (split) {
+    HIPCHK(rf_launch_fwd_col(g.tables, lgN, d_ft, (int)ft.size(), op->spec, g.stream));
+    HIPCHK(rf_launch_fwd_row(g.tables, d_rt, (int)rt.size(), op->spec, g.stream));
+  } else {
+    HIPCHK(rf_launch_fwd(g.tables, lgN, d_ft, (int)ft.size(), op->spec, g.stream));
+  }
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return RF_OK;
+}
+
+// Plans and allocates an operand for entries A (bits/sign filled), then
+// transforms them.
+int operand_make(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out) {
+  rf_operand* op = new rf_operand();
+  op->r = r;
+  op->d = d;
+  for (auto& e : A) {
+    op->la_bits = std::max(op->la_bits, e.sign ? e.bits : 0);
+    op->signs.push_back(e.sign);
+  }
+  int rc = RF_OK;
+  if (op->la_bits && b_max_bits) {
+    if (rf_plan_choose(&g.primes, op->la_bits, b_max_bits, r, d, &op->plan) != 0)
+      rc = fail(RF_ERR_UNSUPPORTED, "no transform plan for a %dx%d operand of %llu bits and right entries of %llu bits",
+                r, d, (unsigned long long)op->la_bits, (unsigned long long)b_max_bits);
+    if (!rc) {
+      const size_t bytes = (size_t)r * d * op->plan.k * ((size_t)8 << op->plan.lgN);
+      hipError_t e = hipMalloc((void**)&op->spec, bytes);
+      if (e != hipSuccess) rc = fail(RF_ERR_NOMEM, "operand spectra of %zu bytes: %s", bytes, hipGetErrorString(e));
+    }
+    if (!rc) rc = operand_transform(op, A, base);
+  } else {
+    op->la_bits = 0;  // a zero operand (or zero right operands): every product is zero
+  }
+  if (rc) {
+    if (op->spec) (void)hipFree(op->spec);
+    delete op;
+    return rc;
+  }
+  *out = op;
+  return RF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rf_operand_create_device(const rf_dev_entry* A, int r, int d, uint64_t b_max_bits, rf_operand** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!out) return fail(RF_ERR_ARG, "rf_operand_create_device: null output");
+  *out = nullptr;
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (r <= 0 || d <= 0 || !A) return fail(RF_ERR_ARG, "bad operand r=%d d=%d", r, d);
+  std::vector<Entry> E(r * d);
+  for (int e = 0; e < r * d; e++) {
+    E[e].dptr = A[e].limbs;
+    E[e].nlimbs = A[e].nlimbs;
+    E[e].sign = A[e].nlimbs ? (A[e].sign > 0 ? 1 : (A[e].sign < 0 ? -1 : 0)) : 0;
+    E[e].bits = E[e].sign ? 64 * A[e].nlimbs : 0;
+  }
+  return operand_make(E, r, d, b_max_bits, 0, out);
+}
+
+int rf_operand_create(mpz_t* A, int r, int d, uint64_t b_max_bits, rf_operand** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!out) return fail(RF_ERR_ARG, "rf_operand_create: null output");
+  *out = nullptr;
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (r <= 0 || d <= 0 || !A) return fail(RF_ERR_ARG, "bad operand r=%d d=%d", r, d);
+  std::vector<Entry> E(r * d);
+  uint64_t total = 0;
+  for (int e = 0; e < r * d; e++) {
+    E[e].nlimbs = mpz_size(A[e]);
+    E[e].sign = mpz_sgn(A[e]);
+    E[e].bits = E[e].sign ? mpz_sizeinbase(A[e], 2) : 0;
+    E[e].limb_off = total;
+    total += E[e].nlimbs;
+  }
+  const size_t in_b = align256(total * 8 + 8);
+  int rc = ensure_arena(in_b);
+  if (rc) return rc;
+  rc = ensure_stage(in_b);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  uint64_t* hs = (uint64_t*)g.h_stage;
+  for (int e = 0; e < r * d; e++)
+    if (E[e].nlimbs) memcpy(hs + E[e].limb_off, A[e]->_mp_d, E[e].nlimbs * 8);
+  uint64_t* d_in = (uint64_t*)g.d_arena;
+  if (total) HIPCHK(hipMemcpyAsync(d_in, hs, total * 8, hipMemcpyHostToDevice, g.stream));
+  for (auto& e : E) e.dptr = d_in + e.limb_off;
+  // the transform reuses the staging buffer only after synchronising the stream
+  return operand_make(E, r, d, b_max_bits, in_b, out);
+}
+
+int rf_operand_mult(const rf_operand* left, mpz_t* C, mpz_t* B) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (!left) return fail(RF_ERR_ARG, "rf_operand_mult: null operand");
+  return mpz_product(C, nullptr, left->r, B, left->d, left);
+}
+
+int rf_operand_plan(const rf_operand* op, int* t, int* lgN, int* k) {
+  if (!op) return fail(RF_ERR_ARG, "rf_operand_plan: null operand");
+  *t = op->plan.t;
+  *lgN = op->plan.lgN;
+  *k = op->plan.k;
+  return RF_OK;
+}
+
+void rf_operand_destroy(rf_operand* op) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!op) return;
+  if (g.ready) (void)hipDeviceSynchronize();
+  if (op->spec) (void)hipFree(op->spec);
+  delete op;
 }
 
 int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k) {
@@ -1064,10 +1261,10 @@ int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int
   HIPCHK(hipMemcpyAsync(d_spec, hs, np * N * 8, hipMemcpyHostToDevice, g.stream));
   HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
   if (split) {
-    HIPCHK(rf_launch_inv(g.tables, RF_LGN_ROW, d_task, (int)ninv, d_spec, d_coef, g.stream));
+    HIPCHK(rf_launch_inv(g.tables, RF_LGN_ROW, d_task, (int)ninv, d_spec, d_spec, d_coef, g.stream));
     HIPCHK(rf_launch_inv_col(g.tables, lgN, d_col, r * d, d_coef, g.stream));
   } else {
-    HIPCHK(rf_launch_inv(g.tables, lgN, d_task, r * d, d_spec, d_coef, g.stream));
+    HIPCHK(rf_launch_inv(g.tables, lgN, d_task, r * d, d_spec, d_spec, d_coef, g.stream));
   }
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(g.h_stage, d_coef, (size_t)r * d * N * 8, hipMemcpyDeviceToHost));

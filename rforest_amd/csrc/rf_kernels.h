@@ -110,8 +110,10 @@ struct RfDevTables {
 
 hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
                          double* spec, hipStream_t st);
+// a_off of the tasks counts from aspec (the batch's spectra or a resident
+// operand's), b_off from bspec.
 hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
-                         const double* spec, double* coef, hipStream_t st);
+                         const double* aspec, const double* bspec, double* coef, hipStream_t st);
 // Column and row passes of the longer transforms (lgN in RF_LGN_LDS+1 .. RF_LGN_MAX).
 hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
                              double* spec, hipStream_t st);

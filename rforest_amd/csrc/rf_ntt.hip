@@ -339,7 +339,8 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_fwd(RfDevTables tb,
 template <int LGN>
 __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
                                                              const RfInvTask* __restrict__ tasks,
-                                                             int ntasks, const double* __restrict__ spec,
+                                                             int ntasks, const double* __restrict__ aspec,
+                                                             const double* __restrict__ bspec,
                                                              double* __restrict__ coef) {
   using S = Shape<LGN>;
   using Q0 = Pass<LGN, 0>;
@@ -365,8 +366,8 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
     double acc[EB];
 #pragma unroll
     for (int e = 0; e < EB; e++) acc[e] = 0.0;
-    const double* pa = spec + tk.a_off;
-    const double* pb = spec + tk.b_off;
+    const double* pa = aspec + tk.a_off;
+    const double* pb = bspec + tk.b_off;
     for (int l = 0; l < tk.d; l++) {
       double a[EB], b[EB];
 #pragma unroll
@@ -567,14 +568,15 @@ hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntask
 
 template <int LGN>
 hipError_t launch_inv_t(const RfDevTables& tb, const RfInvTask* tasks, int ntasks,
-                        const double* spec, double* coef, hipStream_t st) {
+                        const double* aspec, const double* bspec, double* coef, hipStream_t st) {
   using S = Shape<LGN>;
   const int wgs = (ntasks + S::G - 1) / S::G;
   const int grid = (wgs + 7) & ~7;
   const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_inv<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_inv<LGN>, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec, coef);
+  hipLaunchKernelGGL(k_inv<LGN>, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, aspec, bspec,
+                     coef);
   return hipGetLastError();
 }
 
@@ -596,12 +598,12 @@ hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks,
 }
 
 hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
-                         const double* spec, double* coef, hipStream_t st) {
+                         const double* aspec, const double* bspec, double* coef, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   switch (lgN) {
 #define RF_CASE(L) \
   case L:          \
-    return launch_inv_t<L>(tb, tasks, ntasks, spec, coef, st);
+    return launch_inv_t<L>(tb, tasks, ntasks, aspec, bspec, coef, st);
     RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
     RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
 #undef RF_CASE

@@ -108,6 +108,11 @@ bool rf_bound_holds(const RfPrimes* pr, int k, int b, uint64_t f) {
 
 int rf_plan_fixed(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t, int lgN,
                   RfPlan* out) {
+  return rf_plan_fixed_k(pr, LA, LB, r, d, t, lgN, 0, out);
+}
+
+int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t, int lgN, int kmin,
+                    RfPlan* out) {
   const size_t b = 64 * (size_t)t;
   RfPlan pl;
   memset(&pl, 0, sizeof pl);
@@ -132,7 +137,8 @@ int rf_plan_fixed(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t,
       k = kk;
       break;
     }
-  if (k == 0) return -1;
+  if (k == 0 || kmin > pr->k) return -1;
+  if (k < kmin) k = kmin;  // more primes than the bound needs are always valid
   pl.k = k;
   pl.nout = (int)nout;
   // Coefficients lie in (-P/2, P/2) with P < 2^(48 k): 48 k + 1 bits of two's

@@ -60,6 +60,9 @@ int rf_plan_choose(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, RfPla
 // Fills a plan for a fixed (t, lgN); returns 0 or -1 if it cannot hold.
 int rf_plan_fixed(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t, int lgN,
                   RfPlan* out);
+// The same with at least kmin primes (a resident operand's spectra fix k).
+int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t, int lgN, int kmin,
+                    RfPlan* out);
 
 // ---- constant tables shared by all kernels (one copy in device memory) ----
 struct RfConsts {

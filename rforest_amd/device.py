@@ -46,9 +46,76 @@ def device_sync():
     check(lib().rf_device_sync())
 
 
+def _upload_entries(values):
+    """Python ints -> (DeviceBuffer of their magnitudes, [(limb offset,
+    limbs, sign)])."""
+    import numpy as np
+    chunks, ents, pos = [], [], 0
+    for x in values:
+        x = int(x)
+        n = (abs(x).bit_length() + 63) // 64
+        if n:
+            chunks.append(np.frombuffer(abs(x).to_bytes(8 * n, "little"), dtype=np.uint64))
+        ents.append((pos, n, (x > 0) - (x < 0)))
+        pos += n
+    host = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.uint64)
+    buf = DeviceBuffer(host.nbytes)
+    buf.upload(host)
+    return buf, ents
+
+
+class ResidentOperand:
+    """A left operand (r x d, Python ints) transformed once into the NTT
+    domain in device memory (rf_operand_create_device), planned for right
+    operands of up to b_max_bits bits per entry.  Pass it as A in a
+    DeviceBatch product, or call mult() for a host product."""
+
+    def __init__(self, A, r, d, b_max_bits):
+        self.r, self.d = r, d
+        buf, ents = _upload_entries(A)
+        E = (DevEntry * (r * d))()
+        for e, (off, n, s) in enumerate(ents):
+            E[e].limbs = buf.ptr + 8 * off
+            E[e].nlimbs = n
+            E[e].sign = s
+        self.la_limbs = max([n for (_, n, s) in ents if s] or [0])
+        h = ctypes.c_void_p()
+        check(lib().rf_operand_create_device(E, r, d, int(b_max_bits), ctypes.byref(h)))
+        self.handle = h
+        buf.free()  # the spectra are all the operand keeps
+
+    def plan(self):
+        t, lg, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().rf_operand_plan(self.handle, ctypes.byref(t), ctypes.byref(lg), ctypes.byref(k)))
+        return t.value, lg.value, k.value
+
+    def mult(self, B):
+        """C = A * B through rf_operand_mult on GMP mpz_t."""
+        from .gmp import MpzArray
+        mB, mC = MpzArray(B), MpzArray.zeros(self.r * self.d)
+        try:
+            check(lib().rf_operand_mult(self.handle, mC.ptr(), mB.ptr()))
+            return mC.values()
+        finally:
+            mB.close()
+            mC.close()
+
+    def release(self):
+        if self.handle is not None:
+            lib().rf_operand_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 class DeviceBatch:
-    """Uploads a list of products [(r, d, A, B), ...] (Python ints) once; run()
-    enqueues the whole pipeline over all of them; results() reads back."""
+    """Uploads a list of products [(r, d, A, B), ...] (Python ints; A may be a
+    ResidentOperand) once; run() enqueues the whole pipeline over all of
+    them; results() reads back."""
 
     def __init__(self, products):
         import numpy as np
@@ -58,21 +125,23 @@ class DeviceBatch:
         pos = 0
         plan_in = []
         for (r, d, A, B) in products:
+            left = A if isinstance(A, ResidentOperand) else None
             ents = []
-            for x in list(A) + list(B):
+            for x in ([] if left else list(A)) + list(B):
                 x = int(x)
                 n = (abs(x).bit_length() + 63) // 64
                 if n:
                     chunks.append(np.frombuffer(abs(x).to_bytes(8 * n, "little"), dtype=np.uint64))
                 ents.append((pos, n, (x > 0) - (x < 0)))
                 pos += n
-            la = max([n for (_, n, s) in ents[:r * d] if s] or [0])
-            lb = max([n for (_, n, s) in ents[r * d:] if s] or [0])
-            plan_in.append((r, d, ents, int(L.rf_out_limbs(la, lb, d))))
+            na = 0 if left else r * d
+            la = left.la_limbs if left else max([n for (_, n, s) in ents[:na] if s] or [0])
+            lb = max([n for (_, n, s) in ents[na:] if s] or [0])
+            plan_in.append((r, d, ents, int(L.rf_out_limbs(la, lb, d)), left))
         host = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.uint64)
         self.limbs = DeviceBuffer(host.nbytes)
         self.limbs.upload(host)
-        total_out = sum(max(rn, 1) * r * d for (r, d, _, rn) in plan_in)
+        total_out = sum(max(rn, 1) * r * d for (r, d, _, rn, _) in plan_in)
         self.out = DeviceBuffer(8 * max(total_out, 1))
         self.total_out = total_out
         base, obase = self.limbs.ptr, self.out.ptr
@@ -80,12 +149,13 @@ class DeviceBatch:
         prods = (DevProduct * len(products))()
         opos = 0
         self.layout = []
-        for pi, (r, d, ents, rn) in enumerate(plan_in):
+        for pi, (r, d, ents, rn, left) in enumerate(plan_in):
             A = (DevEntry * (r * d))()
             B = (DevEntry * (d * d))()
             C = (DevOut * (r * d))()
+            na = 0 if left else r * d
             for e, (off, n, s) in enumerate(ents):
-                tgt = A[e] if e < r * d else B[e - r * d]
+                tgt = A[e] if e < na else B[e - na]
                 tgt.limbs = base + 8 * off
                 tgt.nlimbs = n
                 tgt.sign = s
@@ -100,6 +170,7 @@ class DeviceBatch:
             prods[pi].A = ctypes.cast(A, ctypes.POINTER(DevEntry))
             prods[pi].B = ctypes.cast(B, ctypes.POINTER(DevEntry))
             prods[pi].C = ctypes.cast(C, ctypes.POINTER(DevOut))
+            prods[pi].left = left.handle.value if left else None
             self._keep += [A, B, C]
         self.prods = prods
 

@@ -117,3 +117,13 @@ def tree_level(n=1024, seed=1003, lo=10, hi=17):
         out.append((2, 2, ents[:4], ents[4:]))
     rng.close()
     return out
+
+
+def resident_case(seed=1004, left_bits=1 << 22, right_bits=1 << 20, n_right=16, r=3):
+    """One r x r left matrix of uniform left_bits-bit entries and n_right
+    right r x r matrices of uniform right_bits-bit entries, one generator."""
+    rng = GmpRandom(seed)
+    A = [rng.urandomb(left_bits) for _ in range(r * r)]
+    Bs = [[rng.urandomb(right_bits) for _ in range(r * r)] for _ in range(n_right)]
+    rng.close()
+    return A, Bs
