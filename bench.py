@@ -85,6 +85,90 @@ def cpu_baseline(products, gpu_results, budget_s):
             "gpu_results_identical": mismatches == 0}
 
 
+def tree_level_section(rf, workloads, reps):
+    """1024 independent 2x2 . 2x2 products of uneven sizes (entries
+    log-uniform in [2^10, 2^18) bits, each matrix within 2x, random signs):
+    one prepared batch replayed from its graph, against the plain path (each
+    product alone through the one-product pipeline, and the host mpz
+    drop-in on a sample)."""
+    prods = workloads.tree_level(1024, seed=1003)
+    batch = rf.DeviceBatch(prods)
+    batch.prepare()
+    batch.launch(graph=True)
+    rf.device_sync()
+    res = batch.results()
+    q = (1 << 61) - 1
+    bad = modcheck(prods, res, q)
+    exact = all(res[i] == rf.mpz_rmatrix_mult_naive(prods[i][2], 2, prods[i][3], 2) for i in range(0, 1024, 97))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.launch(graph=True)
+    rf.device_sync()
+    batched = 1024 * reps / (time.perf_counter() - t0)
+    batch.run_one(0)
+    t0 = time.perf_counter()
+    for i in range(1024):
+        batch.run_one(i)
+    plain_dev = 1024 / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    ns = 128
+    for i in range(ns):
+        rf.mpz_rmatrix_mult_fft(prods[i][2], 2, prods[i][3], 2)
+    plain_host = ns / (time.perf_counter() - t0)
+    batch.release()
+    return {"workload": "1024 products 2x2 . 2x2, entries log-uniform in [2^10, 2^18) bits (matrix within 2x), "
+                        "random signs, GMP seed 1003",
+            "batched_products_per_s": round(batched, 1),
+            "plain_device_products_per_s": round(plain_dev, 1),
+            "plain_dropin_products_per_s": round(plain_host, 1),
+            "speedup_vs_plain_device": round(batched / plain_dev, 2),
+            "speedup_vs_plain_dropin": round(batched / plain_host, 2),
+            "checks": {"mod_2^61-1_mismatches": bad, "exact_sample": exact}}
+
+
+def resident_section(rf, workloads, reps):
+    """One 3x3 left matrix of 4,194,304-bit entries times 16 right 3x3
+    matrices of 1,048,576-bit entries: the left operand transformed once and
+    kept in the NTT domain, against transforming it in every product."""
+    A, Bs = workloads.resident_case(1004)
+    plain = rf.DeviceBatch([(3, 3, A, B) for B in Bs])
+    plain.prepare()
+    plain.launch(graph=True)
+    rf.device_sync()
+    want = plain.results()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain.launch(graph=True)
+    rf.device_sync()
+    t_plain = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    op = rf.ResidentOperand(A, 3, 3, 1 << 20)
+    t_create = time.perf_counter() - t0
+    res = rf.DeviceBatch([(3, 3, op, B) for B in Bs])
+    res.prepare()
+    res.launch(graph=True)
+    rf.device_sync()
+    got = res.results()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res.launch(graph=True)
+    rf.device_sync()
+    t_res = (time.perf_counter() - t0) / reps
+    exact = got[0] == rf.mpz_rmatrix_mult_naive(A, 3, Bs[0], 3)
+    t, lgN, k = op.plan()
+    res.release()
+    plain.release()
+    op.release()
+    return {"workload": "3x3 left of 4,194,304-bit entries x 16 right 3x3 of 1,048,576-bit entries, GMP seed 1004",
+            "plan": {"chunk_bits": 64 * t, "ntt_length": 1 << lgN, "primes": k},
+            "resident_products_per_s": round(16 / t_res, 1),
+            "recompute_products_per_s": round(16 / t_plain, 1),
+            "speedup": round(t_plain / t_res, 2),
+            "operand_create_ms": round(t_create * 1e3, 3),
+            "transforms_per_16": {"resident": 9 + 16 * (9 + 9), "recompute": 16 * (9 + 9 + 9)},
+            "checks": {"identical_to_recompute": got == want, "exact_product0": exact}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +179,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host mpz drop-in (PCIe-inclusive, one product per call)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline only (skip the tree-level and resident-operand sections)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -179,6 +265,13 @@ def main():
             rf.mpz_rmatrix_mult_fft(products[i][2], 2, products[i][3], 2)
         dropin = round(nd / (time.perf_counter() - t1), 2)
 
+    extra = {}
+    if not args.no_extra:
+        log("tree level: batched vs plain")
+        extra["tree_level"] = tree_level_section(rf, workloads, max(3, args.steps // 4))
+        log("resident operand")
+        extra["resident_operand"] = resident_section(rf, workloads, max(3, args.steps // 4))
+
     cpu = None
     if not args.no_cpu:
         log("timing the CPU baseline")
@@ -211,6 +304,7 @@ def main():
         "cpu_baseline": cpu,
         "checks": {"exact_product0": exact0, "mod_2^61-1_mismatches": bad},
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -180,6 +180,11 @@ class DeviceBatch:
         """One-shot: plans, builds task tables and runs (rf_batch_mult_device)."""
         check(lib().rf_batch_mult_device(self.prods, len(self.products), None, 0 if sync else 1))
 
+    def run_one(self, i, sync=True):
+        """Product i alone, one-shot (the plain per-product path)."""
+        ptr = ctypes.cast(ctypes.byref(self.prods, i * ctypes.sizeof(DevProduct)), ctypes.POINTER(DevProduct))
+        check(lib().rf_batch_mult_device(ptr, 1, None, 0 if sync else 1))
+
     def prepare(self):
         """Fixes plans, task tables and scratch once (rf_batch_create)."""
         if self.handle is None:
