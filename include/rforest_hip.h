@@ -150,6 +150,88 @@ int rf_operand_plan(const rf_operand* op, int* t, int* lgN, int* k);
 void rf_operand_destroy(rf_operand* op);
 
 /* ---------------------------------------------------------------------------
+ * Request server (no reference counterpart; the reference serves one
+ * synchronous call at a time).  Continuous batching: a worker thread admits
+ * queued requests in arrival order whenever a stream slot is free and the
+ * request's device memory (inputs, outputs and its share of the batch's
+ * spectra, coefficients and task tables) fits in the HBM budget, runs each
+ * admitted group as one pipeline on its slot's stream, and retires requests
+ * as their batch's event completes; batches on different slots overlap on
+ * the device.  Each result is the exact product whatever else shares its
+ * batch.
+ * ------------------------------------------------------------------------- */
+#define RF_IN_HOST 0        /* entry limbs are host pointers, copied at admission */
+#define RF_IN_SYNTH 1       /* entries generated at admission from the request's seed
+                               (the synthetic stream of rf_util.h; load generation) */
+#define RF_OUT_KEEP 1       /* keep the result in device memory until rf_server_release */
+#define RF_OUT_HOST 2       /* copy the result to out_host (r*d*out_cap limbs, two's complement) */
+#define RF_OUT_CHECKSUM 4   /* compute the result's 64-bit checksum (rf_util.h) */
+
+typedef struct rf_serve_entry {
+  const uint64_t* limbs; /* RF_IN_HOST: ceil(bits / 64) magnitude limbs; RF_IN_SYNTH: unused */
+  uint64_t bits;         /* magnitude bit length (RF_IN_SYNTH: exact) */
+  int32_t sign;          /* -1, 0, +1 */
+  int32_t pad;
+} rf_serve_entry;
+
+typedef struct rf_request {
+  int32_t r, d;
+  int32_t input;               /* RF_IN_* */
+  int32_t output;              /* RF_OUT_* flags */
+  const rf_serve_entry* A;     /* r*d, copied at submission */
+  const rf_serve_entry* B;     /* d*d */
+  uint64_t seed;               /* RF_IN_SYNTH */
+  uint64_t* out_host;          /* RF_OUT_HOST */
+  uint64_t out_cap;            /* RF_OUT_HOST: limbs per C entry */
+  uint64_t tag;                /* returned in the completion */
+} rf_request;
+
+typedef struct rf_completion {
+  uint64_t id, tag;
+  int32_t status;              /* RF_OK or an RF_ERR_* code */
+  int32_t pad;
+  uint64_t out_limbs;          /* limbs per C entry of the result */
+  uint64_t checksum;           /* RF_OUT_CHECKSUM */
+  double t_submit, t_admit, t_done;  /* seconds on the server clock */
+} rf_completion;
+
+typedef struct rf_server_config {
+  uint64_t hbm_budget;         /* bytes of device memory the server may use */
+  int32_t streams;             /* batches in flight at once (default 3) */
+  int32_t max_batch;           /* products per batch (default 256) */
+  uint64_t max_batch_bytes;    /* device bytes per batch (default budget / streams) */
+} rf_server_config;
+
+typedef struct rf_server_stats {
+  uint64_t submitted, completed, batches, products_in_batches;
+  uint64_t peak_bytes, budget;
+} rf_server_stats;
+
+typedef struct rf_server rf_server;
+int rf_server_create(const rf_server_config* cfg, rf_server** out);
+/* Queues one request (thread-safe, returns at once).  RF_ERR_CAPACITY if the
+   request alone cannot fit the budget, RF_ERR_UNSUPPORTED if it has no plan. */
+int rf_server_submit(rf_server* s, const rf_request* req, uint64_t* id);
+/* Moves up to max completions into out; waits up to wait_ms for the first.
+   Returns the number moved, or a negative RF_ERR_* code. */
+int rf_server_poll(rf_server* s, rf_completion* out, int max, int wait_ms);
+/* Copies a completed RF_OUT_KEEP result (r*d entries of out_limbs limbs,
+   two's complement) to host memory; cap is the room in limbs. */
+int rf_server_read(rf_server* s, uint64_t id, uint64_t* dst, uint64_t cap);
+/* Frees a completed RF_OUT_KEEP result. */
+int rf_server_release(rf_server* s, uint64_t id);
+int rf_server_stats_get(rf_server* s, rf_server_stats* st);
+/* Seconds since the server was created (the clock of the completions). */
+double rf_server_clock(rf_server* s);
+/* Finishes every queued and running request, then frees the server. */
+int rf_server_destroy(rf_server* s);
+
+/* The synthetic stream and checksum of rf_util.h on device memory (for
+   checking the server's results against other paths). */
+int rf_gen_entry_device(uint64_t* dst, uint64_t bits, uint64_t seed, uint32_t entry);
+int rf_checksum_device(const uint64_t* limbs, uint64_t n, uint32_t entry, uint64_t* sum);
+
+/* ---------------------------------------------------------------------------
  * Device and pinned memory (src/include/hwmpz.h:38-46 declares the same
  * helpers for the reference's CUDA path; callers of rf_batch_mult_device use
  * them to place entries in HBM without another runtime in the process)

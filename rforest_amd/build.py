@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librforest_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rf_ntt.hip", "rf_crt.hip", "rf_engine.cpp", "rf_plan.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f)
+           for f in ("rf_ntt.hip", "rf_crt.hip", "rf_util.hip", "rf_engine.cpp", "rf_serve.cpp", "rf_plan.cpp")]
 OBJDIR = os.path.join(ROOT, "_build")
 GMP_PREFIX = os.environ.get("RF_GMP_PREFIX", "/opt/conda")
 

@@ -22,6 +22,7 @@
 
 #include "../../include/rforest_hip.h"
 #include "../../include/rforest_hip_stages.h"
+#include "rf_internal.h"
 #include "rf_kernels.h"
 #include "rf_plan.h"
 
@@ -30,17 +31,8 @@ int hw_disable_fft = 0;
 int mpzfft_threads = 1;
 }
 
-// A resident left operand: r*d spectra, [entry][prime][N] like a batch's
-// spectra, computed once under a plan fixed at creation.
-struct rf_operand {
-  int r = 0, d = 0;
-  RfPlan plan{};
-  uint64_t la_bits = 0;     // largest entry, bits (0: the operand is zero)
-  std::vector<int> signs;   // per entry
-  double* spec = nullptr;   // device, r*d*k*N doubles
-};
 
-namespace {
+namespace rfi {
 
 struct Ctx {
   bool ready = false;       // GPU state initialised
@@ -80,15 +72,7 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(x)                                                                     \
-  do {                                                                                \
-    hipError_t e_ = (x);                                                              \
-    if (e_ != hipSuccess)                                                             \
-      return fail(RF_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
-                  __LINE__);                                                          \
-  } while (0)
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int host_init_locked() {
   if (g.host_ready) return RF_OK;
@@ -200,22 +184,6 @@ void prof_collect() {
   g.pending.clear();
 }
 
-// ---- batch description shared by all entry points ----
-struct Entry {
-  uint64_t limb_off;     // host entries: offset in the staged limb buffer
-  const uint64_t* dptr;  // device pointer to the magnitude limbs
-  uint64_t nlimbs;
-  uint64_t bits;  // upper bound on the magnitude's bit length
-  int sign;
-};
-struct Product {
-  int r, d;
-  const rf_operand* left = nullptr;  // resident A: its spectra replace A's forward transforms
-  std::vector<Entry> A, B;
-  std::vector<uint64_t*> out;  // device pointers, plan.rn limbs each
-  RfPlan plan;
-  uint64_t spec_off, coef_off;  // in doubles, within their regions
-};
 
 int plan_product(Product& P) {
   uint64_t LA = 0, LB = 0;
@@ -248,14 +216,6 @@ uint64_t out_limbs_for(uint64_t la_bits, uint64_t lb_bits, int d) {
   return (la_bits + lb_bits + cl + 1 + 63) / 64;
 }
 
-// Scratch layout of one pipeline run (after the caller's `base` bytes).
-struct Layout {
-  uint64_t spec_n = 0, coef_n = 0;  // doubles
-  uint32_t nblk = 0;                // CRT blocks in total
-  size_t nfwd = 0, ninv = 0, nrow = 0, ncol = 0, ncrt = 0, ncarry = 0;
-  size_t spec_b = 0, coef_b = 0, tasks_b = 0, pre_b = 0, fun_b = 0;
-  size_t total() const { return spec_b + coef_b + tasks_b + pre_b + 2 * fun_b; }
-};
 
 uint32_t crt_blocks(const RfPlan& pl) {
   const uint64_t ndig = (pl.rn + pl.t - 1) / pl.t;
@@ -314,7 +274,7 @@ Layout layout_of(std::vector<Product>& prods) {
 // tables hold absolute device pointers for limbs and outputs and offsets
 // into the scratch region for spectra and coefficients, so one image serves
 // every launch of the same batch.
-enum Kind { K_FWD, K_FWD_COL, K_FWD_ROW, K_INV, K_INV_COL, K_CRT, K_CARRY, K_APPLY };
+
 int stage_of(int kind) {
   switch (kind) {
     case K_FWD: case K_FWD_COL: case K_FWD_ROW: return RF_STAGE_FWD;
@@ -324,19 +284,6 @@ int stage_of(int kind) {
     default: return RF_STAGE_APPLY;
   }
 }
-struct LaunchRec {
-  int kind;    // Kind
-  int param;   // lgN for transforms, k for CRT
-  size_t off;  // byte offset of its task table in the image
-  int n;       // tasks
-  const double* abase = nullptr;  // K_INV: A spectra base (nullptr: the batch's spectra)
-};
-struct Pipeline {
-  Layout L;
-  std::vector<char> image;
-  std::vector<LaunchRec> launches;
-};
-
 template <typename T>
 size_t append_tasks(std::vector<char>& img, const std::vector<T>& v) {
   size_t off = align256(img.size());
@@ -462,14 +409,6 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   return RF_OK;
 }
 
-struct ScratchPtrs {
-  double* spec;
-  double* coef;
-  char* tasks;
-  uint8_t* pre;
-  uint8_t* fun;
-  int8_t* cin;
-};
 
 ScratchPtrs scratch_ptrs(char* base, const Layout& L) {
   ScratchPtrs s;
@@ -482,8 +421,6 @@ ScratchPtrs scratch_ptrs(char* base, const Layout& L) {
   return s;
 }
 
-// Enqueues every kernel of a built pipeline whose task image already sits in
-// the scratch region at `base`.  No host synchronisation, no allocation.
 int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
   const ScratchPtrs S = scratch_ptrs(base, pp.L);
   for (const LaunchRec& l : pp.launches) {
@@ -660,7 +597,14 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* le
   return RF_OK;
 }
 
-}  // namespace
+std::mutex& engine_mutex() { return g_mu; }
+bool engine_ready() { return g.ready; }
+const RfDevTables& engine_tables() { return g.tables; }
+const RfPrimes& engine_primes() { return g.primes; }
+
+}  // namespace rfi
+
+using namespace rfi;
 
 extern "C" {
 
@@ -744,7 +688,7 @@ uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d) {
 
 }  // extern "C"
 
-namespace {
+namespace rfi {
 int products_from_dev(const rf_dev_product* prods, int nprod, std::vector<Product>& ps) {
   if (nprod < 0 || (nprod > 0 && !prods)) return fail(RF_ERR_ARG, "bad product list");
   ps.assign(nprod, Product());

@@ -1,0 +1,104 @@
+// rf_internal.h -- engine internals shared by the library's host sources
+// (rf_engine.cpp: one-shot calls, batches, operands; rf_serve.cpp: the
+// request server).  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/rforest_hip.h"
+#include "rf_kernels.h"
+#include "rf_plan.h"
+
+// A resident left operand: r*d spectra, [entry][prime][N] like a batch's
+// spectra, computed once under a plan fixed at creation.
+struct rf_operand {
+  int r = 0, d = 0;
+  RfPlan plan{};
+  uint64_t la_bits = 0;     // largest entry, bits (0: the operand is zero)
+  std::vector<int> signs;   // per entry
+  double* spec = nullptr;   // device, r*d*k*N doubles
+};
+
+namespace rfi {
+
+int fail(int code, const char* fmt, ...);
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      return fail(RF_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                          \
+  } while (0)
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ---- batch description shared by all entry points ----
+struct Entry {
+  uint64_t limb_off;     // host entries: offset in the staged limb buffer
+  const uint64_t* dptr;  // device pointer to the magnitude limbs
+  uint64_t nlimbs;
+  uint64_t bits;  // upper bound on the magnitude's bit length
+  int sign;
+};
+struct Product {
+  int r, d;
+  const rf_operand* left = nullptr;  // resident A: its spectra replace A's forward transforms
+  std::vector<Entry> A, B;
+  std::vector<uint64_t*> out;  // device pointers, plan.rn limbs each
+  RfPlan plan;
+  uint64_t spec_off, coef_off;  // in doubles, within their regions
+};
+
+// Scratch layout of one pipeline run (after the caller's `base` bytes).
+struct Layout {
+  uint64_t spec_n = 0, coef_n = 0;  // doubles
+  uint32_t nblk = 0;                // CRT blocks in total
+  size_t nfwd = 0, ninv = 0, nrow = 0, ncol = 0, ncrt = 0, ncarry = 0;
+  size_t spec_b = 0, coef_b = 0, tasks_b = 0, pre_b = 0, fun_b = 0;
+  size_t total() const { return spec_b + coef_b + tasks_b + pre_b + 2 * fun_b; }
+};
+
+enum Kind { K_FWD, K_FWD_COL, K_FWD_ROW, K_INV, K_INV_COL, K_CRT, K_CARRY, K_APPLY };
+int stage_of(int kind);
+struct LaunchRec {
+  int kind;    // Kind
+  int param;   // lgN for transforms, k for CRT
+  size_t off;  // byte offset of its task table in the image
+  int n;       // tasks
+  const double* abase = nullptr;  // K_INV: A spectra base (nullptr: the batch's spectra)
+};
+struct Pipeline {
+  Layout L;
+  std::vector<char> image;
+  std::vector<LaunchRec> launches;
+};
+
+struct ScratchPtrs {
+  double* spec;
+  double* coef;
+  char* tasks;
+  uint8_t* pre;
+  uint8_t* fun;
+  int8_t* cin;
+};
+
+// The library's global lock, its readiness and its device tables.
+std::mutex& engine_mutex();
+bool engine_ready();
+const RfDevTables& engine_tables();
+const RfPrimes& engine_primes();
+
+int plan_product(Product& P);
+Layout layout_of(std::vector<Product>& prods);
+int build_pipeline(std::vector<Product>& prods, Pipeline& pp);
+ScratchPtrs scratch_ptrs(char* base, const Layout& L);
+// Enqueues every kernel of a built pipeline whose task image already sits in
+// the scratch region at `base`.  No host synchronisation, no allocation.
+int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof);
+uint64_t bitlen_limbs(const uint64_t* x, uint64_t n);
+void tc_to_signmag(std::vector<uint64_t>& x, bool& neg);
+
+}  // namespace rfi

@@ -1,0 +1,54 @@
+// rf_util.h -- synthetic limb generation and result checksums (definitions
+// shared by the device kernels and the host code that restates them).
+//
+// Synthetic entry e of a request with seed s, b bits (b >= 1):
+//   key   = mix64(s ^ ((e + 1) * 0xD1B54A32D192ED03))
+//   limb i = mix64(key + (i + 1) * 0x9E3779B97F4A7C15), i < n = ceil(b / 64);
+//   the top limb keeps its low b - 64 (n - 1) bits and has the highest of
+//   them set, so the entry has exactly b bits.
+// mix64 is the splitmix64 finaliser (Steele, Lea, Flood 2014).
+//
+// Checksum of a result: the sum mod 2^64, over C entries e and limbs i of its
+// rn-limb two's complement form, of mix64(limb + ((e << 40) + i + 1) * 0x9E3779B97F4A7C15).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RF_UTIL_THREADS 256
+#define RF_UTIL_LIMBS 2048  // limbs per block
+
+__host__ __device__ inline uint64_t rf_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t rf_gen_key(uint64_t seed, uint32_t entry) {
+  return rf_mix64(seed ^ ((uint64_t)(entry + 1) * 0xD1B54A32D192ED03ull));
+}
+__host__ __device__ inline uint64_t rf_gen_limb(uint64_t key, uint64_t i) {
+  return rf_mix64(key + (i + 1) * 0x9E3779B97F4A7C15ull);
+}
+__host__ __device__ inline uint64_t rf_sum_term(uint64_t limb, uint64_t entry_key, uint64_t i) {
+  return rf_mix64(limb + (entry_key + i + 1) * 0x9E3779B97F4A7C15ull);
+}
+
+// A span: k_gen writes `bits` bits of entry key `key` at ptr; k_checksum
+// sums `bits` limbs (a limb count there) at ptr with entry_key = key
+// (= e << 40) into sums[slot].
+struct RfSpan {
+  uint64_t* ptr;
+  uint64_t bits;
+  uint64_t key;
+  uint32_t slot;
+  uint32_t pad;
+};
+struct RfBlock {
+  uint32_t span;
+  uint32_t pad;
+  uint64_t first;  // first limb of the block
+};
+
+hipError_t rf_launch_gen(const RfSpan* spans, const RfBlock* blocks, int nblocks, hipStream_t st);
+hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nblocks, uint64_t* sums,
+                              hipStream_t st);
+hipError_t rf_launch_fill(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t st);

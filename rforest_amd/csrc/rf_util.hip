@@ -1,0 +1,89 @@
+// rf_util.hip -- data kernels beside the product pipeline (gfx950).
+//
+//   k_gen       fills entries with the synthetic limb stream of a seeded
+//               request (load generation for the request server)
+//   k_checksum  64-bit checksums of result limbs, summed per request
+//   k_fill      writes a constant over a u64 range (clears checksum slots
+//               without a memset node, safe under graph capture)
+//
+// All three are flat, coalesced streams over lists of spans; a span is one
+// entry (k_gen) or one C entry (k_checksum), split by the host into blocks
+// of RF_UTIL_LIMBS limbs.
+#include "rf_util.h"
+
+namespace {
+
+__global__ __launch_bounds__(RF_UTIL_THREADS) void k_gen(const RfSpan* __restrict__ spans,
+                                                         const RfBlock* __restrict__ blocks, int nblocks) {
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  const RfBlock bk = blocks[b];
+  const RfSpan sp = spans[bk.span];
+  const uint64_t n = (sp.bits + 63) / 64;
+  const uint32_t top_bits = (uint32_t)(sp.bits - 64 * (n - 1));  // 1 .. 64
+#pragma unroll
+  for (int u = 0; u < RF_UTIL_LIMBS / RF_UTIL_THREADS; u++) {
+    const uint64_t i = bk.first + (uint64_t)u * RF_UTIL_THREADS + threadIdx.x;
+    if (i >= n) break;
+    uint64_t v = rf_gen_limb(sp.key, i);
+    if (i == n - 1) {
+      if (top_bits < 64) v &= (((uint64_t)1) << top_bits) - 1;
+      v |= ((uint64_t)1) << (top_bits - 1);
+    }
+    sp.ptr[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(RF_UTIL_THREADS) void k_checksum(const RfSpan* __restrict__ spans,
+                                                              const RfBlock* __restrict__ blocks, int nblocks,
+                                                              unsigned long long* __restrict__ sums) {
+  __shared__ uint64_t part[RF_UTIL_THREADS / 64];
+  const int b = blockIdx.x;
+  if (b >= nblocks) return;
+  const RfBlock bk = blocks[b];
+  const RfSpan sp = spans[bk.span];
+  const uint64_t n = sp.bits;  // limbs
+  uint64_t h = 0;
+#pragma unroll
+  for (int u = 0; u < RF_UTIL_LIMBS / RF_UTIL_THREADS; u++) {
+    const uint64_t i = bk.first + (uint64_t)u * RF_UTIL_THREADS + threadIdx.x;
+    if (i < n) h += rf_sum_term(sp.ptr[i], sp.key, i);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+#pragma unroll
+    for (int w = 0; w < RF_UTIL_THREADS / 64; w++) t += part[w];
+    atomicAdd(sums + sp.slot, (unsigned long long)t);
+  }
+}
+
+__global__ void k_fill(uint64_t* __restrict__ dst, uint64_t n, uint64_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
+}  // namespace
+
+hipError_t rf_launch_gen(const RfSpan* spans, const RfBlock* blocks, int nblocks, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gen, dim3(nblocks), dim3(RF_UTIL_THREADS), 0, st, spans, blocks, nblocks);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nblocks, uint64_t* sums,
+                              hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_checksum, dim3(nblocks), dim3(RF_UTIL_THREADS), 0, st, spans, blocks, nblocks,
+                     (unsigned long long*)sums);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_fill(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, n, v);
+  return hipGetLastError();
+}
