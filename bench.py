@@ -11,6 +11,7 @@ reference CPU path timed on a bounded sample of the same products.
 """
 import argparse
 import json
+import math
 import os
 import random
 import sys
@@ -169,6 +170,95 @@ def resident_section(rf, workloads, reps):
             "checks": {"identical_to_recompute": got == want, "exact_product0": exact}}
 
 
+def _pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(math.ceil(q / 100.0 * len(xs))) - 1)] if xs else None
+
+
+def _run_stream(rf, serve, reqs, cfg, rate=None):
+    """Submits reqs to a server built with cfg (all at once, or open loop with
+    the stream's exponential gaps scaled to `rate` per second); returns
+    (completions by tag, wall seconds from first submission to last
+    completion, server stats)."""
+    srv = serve.Server(**cfg)
+    t0 = srv.clock()
+    due = 0.0
+    comps = []
+    for i, (r, bits, signs, seed, gap) in enumerate(reqs):
+        if rate:
+            due += gap / rate
+            while srv.clock() - t0 < due:
+                comps += srv.poll(4096, 0)
+        srv.submit_synth(r, r, bits, signs, seed, tag=i)
+    comps += srv.wait_all(len(reqs) - len(comps))
+    t1 = max(c["t_done"] for c in comps)
+    st = srv.stats()
+    srv.close()
+    return {c["tag"]: c for c in comps}, t1 - t0, st
+
+
+def serving_section(rf, workloads, n_req, n_plain, n_dropin, n_open):
+    """The mixed request stream through the request server (continuous
+    batching, 64 GB HBM budget, synthetic entries generated in device memory
+    at admission, results checksummed on the device), against the plain
+    per-request path: the same server with one stream slot and one product
+    per batch, and the host mpz drop-in on a prefix."""
+    from rforest_amd import serve
+    reqs = workloads.served_stream(n_req, seed=1005)
+    budget = 64 * 10**9
+    served_cfg = dict(hbm_budget=budget, streams=3, max_batch=256)
+    plain_cfg = dict(hbm_budget=budget, streams=1, max_batch=1)
+    sat, t_sat, st = _run_stream(rf, serve, reqs, served_cfg)
+    bad = sum(1 for c in sat.values() if c["status"] != rf.RF_OK)
+    plain, t_plain, _ = _run_stream(rf, serve, reqs[:n_plain], plain_cfg)
+    same = all(plain[i]["checksum"] == sat[i]["checksum"] for i in plain)
+    served_rate = len(reqs) / t_sat
+    plain_rate = n_plain / t_plain
+    # exactness of a few small requests against GMP on host-generated entries
+    exact = True
+    small = [i for i in range(n_plain) if reqs[i][1][0] < 200000][:4]
+    for i in small:
+        r, bits, signs, seed, _ = reqs[i]
+        vals = [serve.gen_value(b, sg, seed, e) for e, (b, sg) in enumerate(zip(bits, signs))]
+        C = rf.mpz_rmatrix_mult_naive(vals[:r * r], r, vals[r * r:], r)
+        exact = exact and serve.checksum_values(C, sat[i]["out_limbs"]) == sat[i]["checksum"]
+    # the host mpz drop-in (PCIe and mpz conversion included) on a prefix
+    from rforest_amd.gmp import MpzArray
+    arrs = []
+    for (r, bits, signs, seed, _) in reqs[:n_dropin]:
+        vals = [serve.gen_value(b, sg, seed, e) for e, (b, sg) in enumerate(zip(bits, signs))]
+        arrs.append((r, MpzArray(vals[:r * r]), MpzArray(vals[r * r:]), MpzArray.zeros(r * r)))
+    w = MpzArray.zeros(1)
+    t0 = time.perf_counter()
+    for (r, mA, mB, mC) in arrs:
+        rf.lib().mpz_rmatrix_mult_fft(mC.ptr(), mA.ptr(), r, mB.ptr(), r, w.ptr())
+    dropin_rate = n_dropin / (time.perf_counter() - t0)
+    for a in arrs:
+        for m in a[1:]:
+            m.close()
+    # open loop at 0.7 x the plain capacity: latency of served vs plain
+    rate = 0.7 * plain_rate
+    op, _, _ = _run_stream(rf, serve, reqs[:n_open], served_cfg, rate)
+    pp, _, _ = _run_stream(rf, serve, reqs[:n_open], plain_cfg, rate)
+    lat = [c["t_done"] - c["t_submit"] for c in op.values()]
+    plat = [c["t_done"] - c["t_submit"] for c in pp.values()]
+    return {"workload": f"{n_req} requests, r in {{2,3,4}} w.p. {{0.6,0.3,0.1}}, entries log-uniform in [2^12, 2^24] bits, "
+                        "random signs, GMP seed 1005; entries generated in HBM at admission; 64 GB HBM budget",
+            "served_products_per_s": round(served_rate, 1),
+            "plain_products_per_s": round(plain_rate, 1),
+            "plain_sample": f"first {n_plain} requests, same server with one slot and one product per batch",
+            "dropin_products_per_s": round(dropin_rate, 1),
+            "dropin_sample": f"first {n_dropin} requests, host mpz_rmatrix_mult_fft (PCIe included)",
+            "speedup_vs_plain": round(served_rate / plain_rate, 2),
+            "speedup_vs_dropin": round(served_rate / dropin_rate, 2),
+            "batches": st["batches"], "mean_batch": round(st["products_in_batches"] / max(st["batches"], 1), 1),
+            "peak_pool_gb": round(st["peak_bytes"] / 1e9, 2),
+            "open_loop": {"rate_per_s": round(rate, 1), "requests": n_open,
+                          "p50_ms": round(_pct(lat, 50) * 1e3, 3), "p99_ms": round(_pct(lat, 99) * 1e3, 3),
+                          "plain_p50_ms": round(_pct(plat, 50) * 1e3, 3), "plain_p99_ms": round(_pct(plat, 99) * 1e3, 3)},
+            "checks": {"failed_requests": bad, "mix_equals_alone": same, "exact_sample": exact}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +269,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host mpz drop-in (PCIe-inclusive, one product per call)")
+    ap.add_argument("--requests", type=int, default=20000, help="requests in the served stream")
     ap.add_argument("--no-extra", action="store_true",
                     help="headline only (skip the tree-level and resident-operand sections)")
     args = ap.parse_args()
@@ -271,6 +362,9 @@ def main():
         extra["tree_level"] = tree_level_section(rf, workloads, max(3, args.steps // 4))
         log("resident operand")
         extra["resident_operand"] = resident_section(rf, workloads, max(3, args.steps // 4))
+        log("serving: mixed request stream")
+        extra["serving"] = serving_section(rf, workloads, args.requests, min(2000, args.requests), 50,
+                                           min(5000, args.requests))
 
     cpu = None
     if not args.no_cpu:
@@ -278,7 +372,7 @@ def main():
         cpu = cpu_baseline(products, results, args.cpu_seconds)
 
     out = {
-        "metric": "r=2 1-Mbit matmuls/sec",
+        "metric": "r=2 1-Mbit matmuls/sec; mixed-stream p50/p99 latency, batched vs plain",
         "value": round(value, 2),
         "unit": "products/s",
         "n_gpus": 1,

@@ -205,7 +205,9 @@ class Server:
 
     def poll(self, max_n=4096, wait_ms=0):
         L = _bind()
-        buf = (Completion * max_n)()
+        if getattr(self, "_buf", None) is None or len(self._buf) < max_n:
+            self._buf = (Completion * max_n)()
+        buf = self._buf
         n = L.rf_server_poll(self.h, buf, max_n, int(wait_ms))
         if n < 0:
             raise RfError(n, L.rf_last_error().decode())

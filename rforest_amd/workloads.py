@@ -127,3 +127,24 @@ def resident_case(seed=1004, left_bits=1 << 22, right_bits=1 << 20, n_right=16, 
     Bs = [[rng.urandomb(right_bits) for _ in range(r * r)] for _ in range(n_right)]
     rng.close()
     return A, Bs
+
+
+def served_stream(n=20000, seed=1005, lo=12, hi=24):
+    """The mixed request stream: per request r in {2, 3, 4} with
+    probabilities {0.6, 0.3, 0.1}, entry bit length L log-uniform in
+    [2^lo, 2^hi] (all 2 r^2 entries of L bits), random signs, an
+    exponential inter-arrival gap of mean 1 (scale by 1/rate), and its own
+    seed seed * 2^20 + id for the synthetic entry stream.  Returns a list of
+    (r, bits, signs, request_seed, gap)."""
+    rng = GmpRandom(seed)
+    out = []
+    for i in range(n):
+        u = rng.uniform()
+        r = 2 if u < 0.6 else (3 if u < 0.9 else 4)
+        L = rng.log_uniform(lo, hi)
+        ents = 2 * r * r
+        signs = [(-1 if rng.urandomb(1) else 1) for _ in range(ents)]
+        gap = -math.log(1.0 - rng.uniform())
+        out.append((r, [L] * ents, signs, seed * (1 << 20) + i, gap))
+    rng.close()
+    return out
