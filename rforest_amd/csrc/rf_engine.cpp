@@ -217,6 +217,10 @@ uint64_t out_limbs_for(uint64_t la_bits, uint64_t lb_bits, int d) {
 }
 
 
+// Forward launch groups: one per transform length, and per chunk width
+// where the kernels are specialised on it.
+int fwd_key(int lgN, int t) { return (lgN << 4) | ((lgN >= RF_FWD_TSPEC || lgN > RF_LGN_LDS) ? t : 0); }
+
 uint32_t crt_blocks(const RfPlan& pl) {
   const uint64_t ndig = (pl.rn + pl.t - 1) / pl.t;
   const uint64_t D = RF_CRT_THREADS - rf_crt_halo(pl.k, pl.t);
@@ -238,7 +242,7 @@ Layout layout_of(std::vector<Product>& prods) {
     const int npoly = P.left ? dd : rd + dd;  // polys transformed here
     L.spec_n += (uint64_t)npoly * pl.k * N;
     L.coef_n += (uint64_t)rd * pl.k * N;
-    fwd_n[pl.lgN] += (size_t)npoly * pl.k;
+    fwd_n[fwd_key(pl.lgN, pl.t)] += (size_t)npoly * pl.k;
     if (pl.lgN > RF_LGN_LDS) {
       const size_t rows = (size_t)1 << (pl.lgN - RF_LGN_ROW);
       row_n += (size_t)npoly * pl.k * rows;
@@ -332,7 +336,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
         t.q = (uint16_t)q;
         t.t = (uint8_t)pl.t;
         t.scale = isB ? (uint8_t)pl.lgN : 0;
-        fwd[pl.lgN].push_back(t);
+        fwd[fwd_key(pl.lgN, pl.t)].push_back(t);
       }
     }
     // Longer transforms: 2^(lgN-13) rows of 2^13 residues each.
@@ -388,7 +392,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   // then every inverse pass (row passes before the column passes), CRT,
   // block carries, carry application.
   for (auto& kv : fwd)
-    pp.launches.push_back({kv.first > RF_LGN_LDS ? K_FWD_COL : K_FWD, kv.first,
+    pp.launches.push_back({(kv.first >> 4) > RF_LGN_LDS ? K_FWD_COL : K_FWD, kv.first,
                            append_tasks(pp.image, kv.second), (int)kv.second.size()});
   if (!frow.empty()) pp.launches.push_back({K_FWD_ROW, RF_LGN_ROW, append_tasks(pp.image, frow), (int)frow.size()});
   for (auto& kv : inv)
@@ -427,11 +431,11 @@ int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
     ProfScope ps(prof ? stage_of(l.kind) : -1, st);
     const char* t = S.tasks + l.off;
     switch (l.kind) {
-      case K_FWD:
-        HIPCHK(rf_launch_fwd(g.tables, l.param, (const RfFwdTask*)t, l.n, S.spec, st));
+      case K_FWD:  // param: fwd_key
+        HIPCHK(rf_launch_fwd(g.tables, l.param >> 4, l.param & 15, (const RfFwdTask*)t, l.n, S.spec, st));
         break;
       case K_FWD_COL:
-        HIPCHK(rf_launch_fwd_col(g.tables, l.param, (const RfFwdTask*)t, l.n, S.spec, st));
+        HIPCHK(rf_launch_fwd_col(g.tables, l.param >> 4, l.param & 15, (const RfFwdTask*)t, l.n, S.spec, st));
         break;
       case K_FWD_ROW:
         HIPCHK(rf_launch_fwd_row(g.tables, (const RfRowTask*)t, l.n, S.spec, st));
@@ -505,7 +509,7 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
     E.nlimbs = (E.bits + 63) / 64;
     E.sign = E.bits ? (s < 0 ? -1 : 1) : 0;
     E.limb_off = total_limbs;
-    total_limbs += E.nlimbs;
+    total_limbs += (E.nlimbs + 1) & ~(uint64_t)1;  // entries start 16-byte aligned
     (isA ? P.A : P.B).push_back(E);
   }
   int rc = plan_product(P);
@@ -864,10 +868,10 @@ int operand_transform(rf_operand* op, const std::vector<Entry>& A, size_t base) 
   RfRowTask* d_rt = (RfRowTask*)(g.d_arena + base + fb);
   HIPCHK(hipMemcpyAsync(d_ft, g.h_stage, fb + rb, hipMemcpyHostToDevice, g.stream));
   if (split) {
-    HIPCHK(rf_launch_fwd_col(g.tables, lgN, d_ft, (int)ft.size(), op->spec, g.stream));
+    HIPCHK(rf_launch_fwd_col(g.tables, lgN, pl.t, d_ft, (int)ft.size(), op->spec, g.stream));
     HIPCHK(rf_launch_fwd_row(g.tables, d_rt, (int)rt.size(), op->spec, g.stream));
   } else {
-    HIPCHK(rf_launch_fwd(g.tables, lgN, d_ft, (int)ft.size(), op->spec, g.stream));
+    HIPCHK(rf_launch_fwd(g.tables, lgN, pl.t, d_ft, (int)ft.size(), op->spec, g.stream));
   }
   HIPCHK(hipStreamSynchronize(g.stream));
   return RF_OK;
@@ -938,7 +942,7 @@ int rf_operand_create(mpz_t* A, int r, int d, uint64_t b_max_bits, rf_operand** 
     E[e].sign = mpz_sgn(A[e]);
     E[e].bits = E[e].sign ? mpz_sizeinbase(A[e], 2) : 0;
     E[e].limb_off = total;
-    total += E[e].nlimbs;
+    total += (E[e].nlimbs + 1) & ~(uint64_t)1;  // entries start 16-byte aligned
   }
   const size_t in_b = align256(total * 8 + 8);
   int rc = ensure_arena(in_b);
@@ -1139,10 +1143,10 @@ int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs,
   HIPCHK(hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream));
   HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
   if (split) {
-    HIPCHK(rf_launch_fwd_col(g.tables, lgN, d_task, n, d_spec, g.stream));
+    HIPCHK(rf_launch_fwd_col(g.tables, lgN, t, d_task, n, d_spec, g.stream));
     HIPCHK(rf_launch_fwd_row(g.tables, d_row, n * (int)rows, d_spec, g.stream));
   } else {
-    HIPCHK(rf_launch_fwd(g.tables, lgN, d_task, n, d_spec, g.stream));
+    HIPCHK(rf_launch_fwd(g.tables, lgN, t, d_task, n, d_spec, g.stream));
   }
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(g.h_stage, d_spec, (size_t)n * N * 8, hipMemcpyDeviceToHost));

@@ -108,14 +108,17 @@ struct RfDevTables {
   const double* tw_inv;  // RF_MAXP * RF_INV_TW * 2
 };
 
-hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+// Every task of one forward launch has the same chunk width t (the kernels
+// for lengths from 2^RF_FWD_TSPEC on are specialised on it).
+#define RF_FWD_TSPEC 11
+hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, int t, const RfFwdTask* tasks, int ntasks,
                          double* spec, hipStream_t st);
 // a_off of the tasks counts from aspec (the batch's spectra or a resident
 // operand's), b_off from bspec.
 hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks, int ntasks,
                          const double* aspec, const double* bspec, double* coef, hipStream_t st);
 // Column and row passes of the longer transforms (lgN in RF_LGN_LDS+1 .. RF_LGN_MAX).
-hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, int t, const RfFwdTask* tasks, int ntasks,
                              double* spec, hipStream_t st);
 hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int ntasks, double* spec,
                              hipStream_t st);

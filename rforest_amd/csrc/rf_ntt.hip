@@ -49,6 +49,10 @@ struct Shape {
   static constexpr int NP = (LGN + 3) / 4;             // passes
   static constexpr int LDSN = N;                       // doubles of LDS per transform
   static constexpr int THREADS = T * G;
+  // waves per SIMD the register allocation must allow (HIP's second
+  // launch bound): four, as many as two 64 KiB transforms in LDS leave room
+  // for, so at most 128 VGPRs
+  static constexpr int MINB = 4;
 };
 
 template <int LGN, int P>
@@ -217,14 +221,19 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const d
 // terms are summed, so the three accumulators stay below 2^52: exact.  Then
 // s0 + s1 2^16 + s2 2^32 mod p by two rf_mulmod (inputs < 2^52, outputs
 // <= 1.5 p) and one reduction of a value below 2^53.  Limb l of all NCH
-// chunks is loaded before any is used, so NCH loads are in flight; loads past
-// the entry's end read limb 0 and are discarded, so control flow depends only
-// on the uniform t.
+// chunks is loaded before any is used, so NCH loads are in flight.  When
+// every chunk of the wave lies inside the entry the loads need no bounds
+// checks; otherwise loads past the entry's end read limb 0 and are
+// discarded.  Control flow depends only on the uniform t and that test.
 template <int NCH>
 __device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
                                             const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
                                             const double* pk2, double p, double pinv,
                                             double (&out)[NCH]) {
+  uint32_t cmax = 0;
+#pragma unroll
+  for (int c = 0; c < NCH; c++) cmax = cmax > ci[c] ? cmax : ci[c];
+  const bool full = __builtin_amdgcn_ballot_w64((uint64_t)(cmax + 1) * (uint32_t)t > nlimbs) == 0;
   double s0[NCH], s1[NCH], s2[NCH];
 #pragma unroll
   for (int c = 0; c < NCH; c++) s0[c] = s1[c] = s2[c] = 0.0;
@@ -232,11 +241,16 @@ __device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, 
   for (int l = 0; l < RF_TMAX; l++) {
     if (l < t) {
       uint64_t x[NCH];
+      if (full) {
 #pragma unroll
-      for (int c = 0; c < NCH; c++) {
-        const uint32_t idx = ci[c] * (uint32_t)t + (uint32_t)l;
-        const uint64_t y = limbs[idx < nlimbs ? idx : 0];
-        x[c] = idx < nlimbs ? y : 0;
+        for (int c = 0; c < NCH; c++) x[c] = limbs[ci[c] * (uint32_t)t + (uint32_t)l];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+          const uint32_t idx = ci[c] * (uint32_t)t + (uint32_t)l;
+          const uint64_t y = limbs[idx < nlimbs ? idx : 0];
+          x[c] = idx < nlimbs ? y : 0;
+        }
       }
       const double a0 = pk[2 * l][0], a1 = pk[2 * l][1], a2 = pk[2 * l][2];
       const double b0 = pk[2 * l + 1][0], b1 = pk[2 * l + 1][1], b2 = pk[2 * l + 1][2];
@@ -257,8 +271,92 @@ __device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, 
   }
 }
 
-template <int LGN>
-__global__ __launch_bounds__(Shape<LGN>::THREADS) void k_fwd(RfDevTables tb,
+// Packs with a compile-time chunk width T: every limb of a sub-batch of SB
+// chunks (at most 16 limbs) is loaded before any is used.  pack_full16: every
+// chunk inside the entry, T even and the entry 16-byte aligned, so each chunk
+// is T/2 aligned 16-byte loads.  pack_any8: 8-byte loads with the bounds
+// checks of pack_chunks.  Same arithmetic and bounds as pack_chunks.
+template <int T>
+__device__ __forceinline__ double pack_combine(const uint64_t (&x)[T], const double (*pk)[3],
+                                               const double* pk2, double p, double pinv) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+  for (int l = 0; l < T; l++) {
+    const double lo = (double)(uint32_t)x[l];
+    const double hi = (double)(uint32_t)(x[l] >> 32);
+    s0 = __builtin_fma(hi, pk[2 * l + 1][0], __builtin_fma(lo, pk[2 * l][0], s0));
+    s1 = __builtin_fma(hi, pk[2 * l + 1][1], __builtin_fma(lo, pk[2 * l][1], s1));
+    s2 = __builtin_fma(hi, pk[2 * l + 1][2], __builtin_fma(lo, pk[2 * l][2], s2));
+  }
+  const double r = rf_mulmod(s1, pk2[0], pk2[1], p) + rf_mulmod(s2, pk2[2], pk2[3], p) + s0;
+  return rf_reduce(r, p, pinv);
+}
+
+template <int NCH, int T>
+__device__ __forceinline__ void pack_full16(const uint64_t* __restrict__ limbs, const uint32_t (&ci)[NCH],
+                                            const double (*pk)[3], const double* pk2, double p, double pinv,
+                                            double (&out)[NCH]) {
+  constexpr int SB = NCH * T <= 16 ? NCH : (16 / T > 0 ? 16 / T : 1);
+#pragma unroll
+  for (int c0 = 0; c0 < NCH; c0 += SB) {
+    uint64_t x[SB][T];
+#pragma unroll
+    for (int c = 0; c < SB; c++) {
+      const ulonglong2* src = reinterpret_cast<const ulonglong2*>(limbs + (size_t)ci[c0 + c] * T);
+#pragma unroll
+      for (int l = 0; l < T / 2; l++) {
+        const ulonglong2 y = src[l];
+        x[c][2 * l] = y.x;
+        x[c][2 * l + 1] = y.y;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < SB; c++) out[c0 + c] = pack_combine<T>(x[c], pk, pk2, p, pinv);
+  }
+}
+
+template <int NCH, int T>
+__device__ __forceinline__ void pack_any8(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+                                          const uint32_t (&ci)[NCH], const double (*pk)[3], const double* pk2,
+                                          double p, double pinv, double (&out)[NCH]) {
+  constexpr int SB = NCH * T <= 16 ? NCH : (16 / T > 0 ? 16 / T : 1);
+#pragma unroll
+  for (int c0 = 0; c0 < NCH; c0 += SB) {
+    uint64_t x[SB][T];
+#pragma unroll
+    for (int c = 0; c < SB; c++)
+#pragma unroll
+      for (int l = 0; l < T; l++) {
+        const uint32_t idx = ci[c0 + c] * (uint32_t)T + (uint32_t)l;
+        const uint64_t y = limbs[idx < nlimbs ? idx : 0];
+        x[c][l] = idx < nlimbs ? y : 0;
+      }
+#pragma unroll
+    for (int c = 0; c < SB; c++) out[c0 + c] = pack_combine<T>(x[c], pk, pk2, p, pinv);
+  }
+}
+
+// T > 0: chunk width fixed at compile time (t == T); T == 0: run-time t.
+template <int NCH, int T>
+__device__ __forceinline__ void pack_dispatch(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+                                              const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
+                                              const double* pk2, double p, double pinv, double (&out)[NCH]) {
+  if constexpr (T == 0) {
+    pack_chunks<NCH>(limbs, nlimbs, ci, t, pk, pk2, p, pinv, out);
+  } else {
+    uint32_t cmax = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) cmax = cmax > ci[c] ? cmax : ci[c];
+    const bool full = __builtin_amdgcn_ballot_w64((uint64_t)(cmax + 1) * T > nlimbs) == 0;
+    if (T % 2 == 0 && full && ((uintptr_t)limbs & 15) == 0)
+      pack_full16<NCH, T>(limbs, ci, pk, pk2, p, pinv, out);
+    else
+      pack_any8<NCH, T>(limbs, nlimbs, ci, pk, pk2, p, pinv, out);
+  }
+}
+
+template <int LGN, int T>
+__global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(RfDevTables tb,
                                                              const RfFwdTask* __restrict__ tasks,
                                                              int ntasks, double* __restrict__ spec) {
   using S = Shape<LGN>;
@@ -293,7 +391,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_fwd(RfDevTables tb,
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {  // a zero entry reads no limbs at all
-          pack_chunks<CB>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -316,7 +414,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_fwd(RfDevTables tb,
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {
-          pack_chunks<CB>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -336,8 +434,69 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_fwd(RfDevTables tb,
       out[fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
 }
 
+// Pointwise product sum_l a_l b_l of the thread's E frequencies
+// tid + T e (coalesced: consecutive lanes, consecutive frequencies).  D > 0
+// fixes the number of terms so that all loads of a batch are issued before
+// any is used (|sum| <= 2.4 p needs no intermediate reduction); D == 0 takes
+// d at run time and reduces every eighth term (|term| <= 0.6 p).
+template <int LGN, int D>
+__device__ __forceinline__ void pointwise(double* v, int tid, const double* __restrict__ pa,
+                                          const double* __restrict__ pb, uint32_t as, uint32_t bs, int d,
+                                          double p, double pinv) {
+  using S = Shape<LGN>;
+  if constexpr (D > 0) {
+    constexpr int EB0 = D == 1 ? 16 : (D == 2 ? 8 : 4);
+    constexpr int EB = S::E < EB0 ? S::E : EB0;
+#pragma unroll
+    for (int e0 = 0; e0 < S::E; e0 += EB) {
+      double a[D][EB], b[D][EB];
+#pragma unroll
+      for (int l = 0; l < D; l++)
+#pragma unroll
+        for (int e = 0; e < EB; e++) {
+          a[l][e] = pa[(size_t)l * as + tid + S::T * (e0 + e)];
+          b[l][e] = pb[(size_t)l * bs + tid + S::T * (e0 + e)];
+        }
+#pragma unroll
+      for (int e = 0; e < EB; e++) {
+        double acc = rf_mulmod_var(a[0][e], b[0][e], p, pinv);
+#pragma unroll
+        for (int l = 1; l < D; l++) acc += rf_mulmod_var(a[l][e], b[l][e], p, pinv);
+        v[e0 + e] = rf_reduce(acc, p, pinv);
+      }
+    }
+  } else {
+    constexpr int EB = S::E < 8 ? S::E : 8;
+#pragma unroll
+    for (int e0 = 0; e0 < S::E; e0 += EB) {
+      double acc[EB];
+#pragma unroll
+      for (int e = 0; e < EB; e++) acc[e] = 0.0;
+      const double* qa = pa;
+      const double* qb = pb;
+      for (int l = 0; l < d; l++) {
+        double a[EB], b[EB];
+#pragma unroll
+        for (int e = 0; e < EB; e++) {
+          a[e] = qa[tid + S::T * (e0 + e)];
+          b[e] = qb[tid + S::T * (e0 + e)];
+        }
+#pragma unroll
+        for (int e = 0; e < EB; e++) {
+          acc[e] += rf_mulmod_var(a[e], b[e], p, pinv);  // |term| <= 0.6 p
+          if ((l & 7) == 7) acc[e] = rf_reduce(acc[e], p, pinv);
+        }
+        qa += as;
+        qb += bs;
+      }
+#pragma unroll
+      for (int e = 0; e < EB; e++) v[e0 + e] = rf_reduce(acc[e], p, pinv);
+    }
+  }
+}
+
 template <int LGN>
-__global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
+__global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(RfDevTables tb,
                                                              const RfInvTask* __restrict__ tasks,
                                                              int ntasks, const double* __restrict__ aspec,
                                                              const double* __restrict__ bspec,
@@ -360,31 +519,14 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
   // frequencies) into LDS; pass 0 then reads its groups of 2^R consecutive
   // frequencies from there.  Loads are issued eight residues at a time per
   // operand pair so that many are in flight.
-  constexpr int EB = S::E < 8 ? S::E : 8;
-#pragma unroll
-  for (int e0 = 0; e0 < S::E; e0 += EB) {
-    double acc[EB];
-#pragma unroll
-    for (int e = 0; e < EB; e++) acc[e] = 0.0;
-    const double* pa = aspec + tk.a_off;
-    const double* pb = bspec + tk.b_off;
-    for (int l = 0; l < tk.d; l++) {
-      double a[EB], b[EB];
-#pragma unroll
-      for (int e = 0; e < EB; e++) {
-        a[e] = pa[tid + S::T * (e0 + e)];
-        b[e] = pb[tid + S::T * (e0 + e)];
-      }
-#pragma unroll
-      for (int e = 0; e < EB; e++) {
-        acc[e] += rf_mulmod_var(a[e], b[e], p, pinv);  // |term| <= 0.6 p
-        if ((l & 7) == 7) acc[e] = rf_reduce(acc[e], p, pinv);
-      }
-      pa += tk.a_stride;
-      pb += tk.b_stride;
-    }
-#pragma unroll
-    for (int e = 0; e < EB; e++) v[e0 + e] = rf_reduce(acc[e], p, pinv);
+  const double* pa = aspec + tk.a_off;
+  const double* pb = bspec + tk.b_off;
+  switch (tk.d) {
+    case 1: pointwise<LGN, 1>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 1, p, pinv); break;
+    case 2: pointwise<LGN, 2>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 2, p, pinv); break;
+    case 3: pointwise<LGN, 3>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 3, p, pinv); break;
+    case 4: pointwise<LGN, 4>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 4, p, pinv); break;
+    default: pointwise<LGN, 0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, tk.d, p, pinv); break;
   }
   if constexpr (S::T > 1) {
 #pragma unroll
@@ -414,8 +556,8 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS) void k_inv(RfDevTables tb,
 // bits only; at stage s the pair (hi, hi + 2^(N1-1-s)) uses T[hi >> (N1-s)],
 // the same for every column, so the whole pass is one 2^N1-point transform
 // per column.  Output in place, reduced.
-template <int N1>
-__global__ __launch_bounds__(RF_COL_THREADS) void k_fwd_col(RfDevTables tb,
+template <int N1, int T>
+__global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
                                                             const RfFwdTask* __restrict__ tasks,
                                                             int ntasks, double* __restrict__ spec) {
   constexpr int ROWS = 1 << N1, COLS = 1 << RF_LGN_ROW, BPT = COLS / RF_COL_THREADS;
@@ -441,7 +583,7 @@ __global__ __launch_bounds__(RF_COL_THREADS) void k_fwd_col(RfDevTables tb,
 #pragma unroll
     for (int c = 0; c < CB; c++) ci[c] = (uint32_t)(r0 + c) * COLS + (uint32_t)col;
     if (nchunk && !(half && r0 >= ROWS / 2)) {
-      pack_chunks<CB>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+      pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
     } else {
 #pragma unroll
       for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -518,7 +660,7 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
 // stages L = 13 + s pair (hi, hi + 2^s) with twiddle
 // V[2^L + (hi mod 2^s) 2^13 + col].  Output in place, natural order, reduced.
 template <int N1>
-__global__ __launch_bounds__(RF_COL_THREADS) void k_inv_col(RfDevTables tb,
+__global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
                                                             const RfRowTask* __restrict__ tasks,
                                                             int ntasks, double* __restrict__ coef) {
   constexpr int ROWS = 1 << N1, COLS = 1 << RF_LGN_ROW, BPT = COLS / RF_COL_THREADS;
@@ -553,7 +695,7 @@ __global__ __launch_bounds__(RF_COL_THREADS) void k_inv_col(RfDevTables tb,
   for (int j = 0; j < ROWS; j++) a[(size_t)j * COLS + col] = rf_reduce(v[j], p, pinv);
 }
 
-template <int LGN>
+template <int LGN, int T>
 hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntasks, double* spec,
                         hipStream_t st) {
   using S = Shape<LGN>;
@@ -561,9 +703,30 @@ hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntask
   const int grid = (wgs + 7) & ~7;
   const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
   if (lds > 65536)
-    (void)hipFuncSetAttribute((const void*)k_fwd<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_fwd<LGN>, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec);
+    (void)hipFuncSetAttribute((const void*)k_fwd<LGN, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_fwd<LGN, T>), dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec);
   return hipGetLastError();
+}
+
+// Transform lengths from 2^RF_FWD_TSPEC on get one kernel per chunk width.
+template <int LGN>
+hipError_t launch_fwd_l(const RfDevTables& tb, int t, const RfFwdTask* tasks, int ntasks, double* spec,
+                        hipStream_t st) {
+  if constexpr (LGN < RF_FWD_TSPEC) {
+    return launch_fwd_t<LGN, 0>(tb, tasks, ntasks, spec, st);
+  } else {
+    switch (t) {
+      case 1: return launch_fwd_t<LGN, 1>(tb, tasks, ntasks, spec, st);
+      case 2: return launch_fwd_t<LGN, 2>(tb, tasks, ntasks, spec, st);
+      case 3: return launch_fwd_t<LGN, 3>(tb, tasks, ntasks, spec, st);
+      case 4: return launch_fwd_t<LGN, 4>(tb, tasks, ntasks, spec, st);
+      case 5: return launch_fwd_t<LGN, 5>(tb, tasks, ntasks, spec, st);
+      case 6: return launch_fwd_t<LGN, 6>(tb, tasks, ntasks, spec, st);
+      case 7: return launch_fwd_t<LGN, 7>(tb, tasks, ntasks, spec, st);
+      case 8: return launch_fwd_t<LGN, 8>(tb, tasks, ntasks, spec, st);
+      default: return hipErrorInvalidValue;
+    }
+  }
 }
 
 template <int LGN>
@@ -582,13 +745,13 @@ hipError_t launch_inv_t(const RfDevTables& tb, const RfInvTask* tasks, int ntask
 
 }  // namespace
 
-hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, int t, const RfFwdTask* tasks, int ntasks,
                          double* spec, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   switch (lgN) {
 #define RF_CASE(L) \
   case L:          \
-    return launch_fwd_t<L>(tb, tasks, ntasks, spec, st);
+    return launch_fwd_l<L>(tb, t, tasks, ntasks, spec, st);
     RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7)
     RF_CASE(8) RF_CASE(9) RF_CASE(10) RF_CASE(11) RF_CASE(12) RF_CASE(13) RF_CASE(14)
 #undef RF_CASE
@@ -612,19 +775,30 @@ hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks,
   }
 }
 
-hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, const RfFwdTask* tasks, int ntasks,
+template <int N1>
+hipError_t launch_fwd_col_n(int t, dim3 grid, dim3 blk, const RfDevTables& tb, const RfFwdTask* tasks, int ntasks,
+                            double* spec, hipStream_t st) {
+  switch (t) {
+#define RF_TC(TT)                                                                             \
+  case TT:                                                                                    \
+    hipLaunchKernelGGL((k_fwd_col<N1, TT>), grid, blk, 0, st, tb, tasks, ntasks, spec); \
+    return hipGetLastError();
+    RF_TC(1) RF_TC(2) RF_TC(3) RF_TC(4) RF_TC(5) RF_TC(6) RF_TC(7) RF_TC(8)
+#undef RF_TC
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, int t, const RfFwdTask* tasks, int ntasks,
                              double* spec, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   const dim3 grid(ntasks * ((1 << RF_LGN_ROW) / RF_COL_THREADS)), blk(RF_COL_THREADS);
   switch (lgN - RF_LGN_ROW) {
-#define RF_CASE(N1) \
-  case N1:          \
-    hipLaunchKernelGGL(k_fwd_col<N1>, grid, blk, 0, st, tb, tasks, ntasks, spec); \
-    return hipGetLastError();
-    RF_CASE(2) RF_CASE(3) RF_CASE(4)
-#undef RF_CASE
-    default:
-      return hipErrorInvalidValue;
+    case 2: return launch_fwd_col_n<2>(t, grid, blk, tb, tasks, ntasks, spec, st);
+    case 3: return launch_fwd_col_n<3>(t, grid, blk, tb, tasks, ntasks, spec, st);
+    case 4: return launch_fwd_col_n<4>(t, grid, blk, tb, tasks, ntasks, spec, st);
+    default: return hipErrorInvalidValue;
   }
 }
 

@@ -134,6 +134,9 @@ class DeviceBatch:
                     chunks.append(np.frombuffer(abs(x).to_bytes(8 * n, "little"), dtype=np.uint64))
                 ents.append((pos, n, (x > 0) - (x < 0)))
                 pos += n
+                if n % 2:  # entries start 16-byte aligned
+                    chunks.append(np.zeros(1, dtype=np.uint64))
+                    pos += 1
             na = 0 if left else r * d
             la = left.la_limbs if left else max([n for (_, n, s) in ents[:na] if s] or [0])
             lb = max([n for (_, n, s) in ents[na:] if s] or [0])
