@@ -34,10 +34,26 @@ __device__ __forceinline__ int xcd_task(int bid, int grid, int ntasks) {
   return t < ntasks ? t : -1;
 }
 
-// LDS slot of residue i: the low four index bits are XORed with the next four.
-// For every pass's access pattern each 16-lane group of a ds_read_b64 /
-// ds_write_b64 then covers 16 distinct 8-byte slots modulo 16 (consecutive
-// runs, stride-2 pairs, and groups spread by 32 or 16 residues alike).
+// LDS slot of residue i in an exchange.  Both sides of an exchange address
+// residue i; the slot permutes i within aligned blocks of 32 so that each
+// 32-lane half of a ds_read_b64 / ds_write_b64 covers 32 distinct slots
+// modulo 32 (all 64 banks).  SW = 0: identity, enough when both sides' lanes
+// run over 32 consecutive residues.  SW = 1: i ^ h(i >> 5) with
+// h(m) = 2 (m mod 16) + (m mod 2), which also spreads residues two apart
+// (i = 2 lane + c) and pairs 32 apart (i = 32 (lane / 2) + lane mod 2 + c).
+template <int SW>
+__device__ __forceinline__ int lds_slot(int i) {
+  if constexpr (SW == 0) {
+    return i;
+  } else {
+    const int m = i >> 5;
+    return i ^ (((m & 15) << 1) | (m & 1));
+  }
+}
+
+// Slot for the pointwise-to-first-pass transposition (coalesced writes,
+// groups of 16 consecutive residues read): the low four index bits XORed
+// with the next four.
 __device__ __forceinline__ int lds_pad(int i) { return i ^ ((i >> 4) & 15); }
 
 template <int LGN>
@@ -46,7 +62,7 @@ struct Shape {
   static constexpr int E = LGN >= 4 ? 16 : N;          // residues per thread
   static constexpr int T = N / E;                      // threads per transform
   static constexpr int G = T >= 64 ? 1 : 64 / T;       // transforms per workgroup
-  static constexpr int NP = (LGN + 3) / 4;             // passes
+  static constexpr int NP = (LGN + 3) / 4;             // forward passes
   static constexpr int LDSN = N;                       // doubles of LDS per transform
   static constexpr int THREADS = T * G;
   // waves per SIMD the register allocation must allow (HIP's second
@@ -55,22 +71,38 @@ struct Shape {
   static constexpr int MINB = 4;
 };
 
-template <int LGN, int P>
+// Pass P of a transform whose first pass runs R0 stages and every later one
+// four (the last fewer).  Forward transforms use R0 = 4; inverse transforms
+// start with the LGN mod 4 (or 4) leftover stages so that their groups of
+// 2^R0 consecutive residues come straight from global memory.
+template <int LGN, int P, int R0 = 4>
 struct Pass {
-  static constexpr int L0 = 4 * P;                           // first stage of the pass
-  static constexpr int R = (LGN - L0) < 4 ? (LGN - L0) : 4;  // stages in the pass
+  static constexpr int L0 = P == 0 ? 0 : R0 + 4 * (P - 1);   // first stage of the pass
+  static constexpr int R = P == 0 ? (R0 < LGN ? R0 : LGN)
+                                  : ((LGN - L0) < 4 ? (LGN - L0) : 4);  // stages in the pass
   static constexpr int NG = Shape<LGN>::E >> R;              // groups per thread
   static constexpr int GS = 1 << R;                          // group size
   // Reduce before this pass if the stages since the last reduction would
   // exceed eight.  Passes start after a reduction at P = 0.
-  static constexpr int SINCE = (P == 0) ? 0 : (Pass<LGN, P - 1>::SINCE_AFTER);
+  static constexpr int SINCE = (P == 0) ? 0 : (Pass<LGN, P - 1, R0>::SINCE_AFTER);
   static constexpr bool REDUCE_BEFORE = SINCE + R > 8;
   static constexpr int SINCE_AFTER = (REDUCE_BEFORE ? 0 : SINCE) + R;
 };
-template <int LGN>
-struct Pass<LGN, -1> {
+template <int LGN, int R0>
+struct Pass<LGN, -1, R0> {
   static constexpr int SINCE_AFTER = 0;
 };
+template <int LGN, int R0>
+constexpr int npasses() {
+  return LGN <= R0 ? 1 : 1 + (LGN - R0 + 3) / 4;
+}
+// First inverse pass: the leftover stages when they are one or two (groups
+// of two or four residues loaded directly), otherwise four after an LDS
+// transposition of the pointwise products.
+template <int LGN>
+constexpr int inv_r0() {
+  return (Shape<LGN>::E == 16 && LGN >= 5 && ((LGN - 1) % 4) < 2) ? ((LGN - 1) % 4) + 1 : 4;
+}
 
 // Forward group indexing: stages L0 .. L0+R-1 act on index bits
 // LGN-1-L0 .. LGN-L0-R.  Group g = (H, lo): i = H 2^(LGN-L0) + j 2^NLO + lo.
@@ -83,9 +115,9 @@ __device__ __forceinline__ int fwd_index(int g, int j) {
 }
 
 // Inverse group indexing: stages L0 .. L0+R-1 act on index bits L0 .. L0+R-1.
-template <int LGN, int P>
+template <int LGN, int P, int R0 = inv_r0<LGN>()>
 __device__ __forceinline__ int inv_index(int g, int j) {
-  using Q = Pass<LGN, P>;
+  using Q = Pass<LGN, P, R0>;
   const int hi = g >> Q::L0, lo = g & ((1 << Q::L0) - 1);
   return (hi << (Q::L0 + Q::R)) | (j << Q::L0) | lo;
 }
@@ -128,10 +160,10 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __re
 
 // R inverse (decimation-in-time) stages: at stage L = L0 + s the pair
 // (j, j + 2^s) uses V[2^L + (j mod 2^s) 2^L0 + lo] = omega_{2^(L+1)}^-(i mod 2^L).
-template <int LGN, int P>
+template <int LGN, int P, int R0 = inv_r0<LGN>()>
 __device__ __forceinline__ void inv_pass(double* v, int tid, const double2* __restrict__ tw,
                                          double p) {
-  using Q = Pass<LGN, P>;
+  using Q = Pass<LGN, P, R0>;
 #pragma unroll
   for (int s = 0; s < Q::R; s++) {
     const int half = 1 << s;
@@ -164,6 +196,30 @@ __device__ __forceinline__ void reduce_all(double* v, double p, double pinv) {
   for (int e = 0; e < E; e++) v[e] = rf_reduce(v[e], p, pinv);
 }
 
+// Reduced residues of the last forward pass to out (in place order).  When
+// its groups are consecutive residues (no index bits below its stages) they
+// go out as 16-byte pairs.
+template <int LGN>
+__device__ __forceinline__ void fwd_store(double* __restrict__ out, const double* v, int tid, double p,
+                                          double pinv) {
+  using S = Shape<LGN>;
+  using QL = Pass<LGN, S::NP - 1>;
+  constexpr int NLO = LGN - QL::L0 - QL::R;
+#pragma unroll
+  for (int u = 0; u < QL::NG; u++) {
+    if constexpr (NLO == 0 && QL::GS >= 2) {
+#pragma unroll
+      for (int j = 0; j < QL::GS; j += 2)
+        *reinterpret_cast<double2*>(out + fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)) =
+            make_double2(rf_reduce(v[u * QL::GS + j], p, pinv), rf_reduce(v[u * QL::GS + j + 1], p, pinv));
+    } else {
+#pragma unroll
+      for (int j = 0; j < QL::GS; j++)
+        out[fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+    }
+  }
+}
+
 // Passes 1 .. NP-1 of the forward transform: LDS exchange, optional
 // reduction, stages.
 template <int LGN, int P, bool ROW = false>
@@ -173,17 +229,20 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const d
     using Q = Pass<LGN, P>;
     using Qp = Pass<LGN, P - 1>;
     if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
+    // lanes run over consecutive residues on both sides when both passes
+    // keep at least five low index bits below their stages
+    constexpr int SW = (Shape<LGN>::T >= 64 && LGN - Qp::L0 - Qp::R >= 5 && LGN - Q::L0 - Q::R >= 5) ? 0 : 1;
 #pragma unroll
     for (int u = 0; u < Qp::NG; u++)
 #pragma unroll
       for (int j = 0; j < Qp::GS; j++)
-        sh[lds_pad(fwd_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
+        sh[lds_slot<SW>(fwd_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < Q::NG; u++)
 #pragma unroll
       for (int j = 0; j < Q::GS; j++)
-        v[u * Q::GS + j] = sh[lds_pad(fwd_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
+        v[u * Q::GS + j] = sh[lds_slot<SW>(fwd_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
     __syncthreads();
     fwd_pass<LGN, P, 0, ROW>(v, tid, tw, p, hb);
     fwd_rest<LGN, P + 1, ROW>(v, sh, tid, tw, p, pinv, hb);
@@ -193,21 +252,23 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const d
 template <int LGN, int P>
 __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const double2* tw,
                                          double p, double pinv) {
-  if constexpr (P < Shape<LGN>::NP) {
-    using Q = Pass<LGN, P>;
-    using Qp = Pass<LGN, P - 1>;
+  constexpr int R0 = inv_r0<LGN>();
+  if constexpr (P < npasses<LGN, R0>()) {
+    using Q = Pass<LGN, P, R0>;
+    using Qp = Pass<LGN, P - 1, R0>;
     if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
+    constexpr int SW = (Shape<LGN>::T >= 64 && Qp::L0 >= 5 && Q::L0 >= 5) ? 0 : 1;
 #pragma unroll
     for (int u = 0; u < Qp::NG; u++)
 #pragma unroll
       for (int j = 0; j < Qp::GS; j++)
-        sh[lds_pad(inv_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
+        sh[lds_slot<SW>(inv_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < Q::NG; u++)
 #pragma unroll
       for (int j = 0; j < Q::GS; j++)
-        v[u * Q::GS + j] = sh[lds_pad(inv_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
+        v[u * Q::GS + j] = sh[lds_slot<SW>(inv_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
     __syncthreads();
     inv_pass<LGN, P>(v, tid, tw, p);
     inv_rest<LGN, P + 1>(v, sh, tid, tw, p, pinv);
@@ -425,13 +486,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
     fwd_pass<LGN, 0, 0>(v, tid, tw, p);
   }
   fwd_rest<LGN, 1>(v, sh, tid, tw, p, pinv);
-  using QL = Pass<LGN, S::NP - 1>;
-  double* out = spec + tk.spec_off;
-#pragma unroll
-  for (int u = 0; u < QL::NG; u++)
-#pragma unroll
-    for (int j = 0; j < QL::GS; j++)
-      out[fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+  fwd_store<LGN>(spec + tk.spec_off, v, tid, p, pinv);
 }
 
 // Pointwise product sum_l a_l b_l of the thread's E frequencies
@@ -495,6 +550,81 @@ __device__ __forceinline__ void pointwise(double* v, int tid, const double* __re
   }
 }
 
+// Pointwise product straight into the first inverse pass's layout: group u
+// of the thread holds the 2^R0 consecutive residues (g << R0) + j,
+// g = tid + T u, loaded as 16-byte pairs.  D as in pointwise.
+template <int LGN, int D, int R0>
+__device__ __forceinline__ void pointwise_grp(double* v, int tid, const double* __restrict__ pa,
+                                              const double* __restrict__ pb, uint32_t as, uint32_t bs, int d,
+                                              double p, double pinv) {
+  using S = Shape<LGN>;
+  constexpr int GS = 1 << R0, NG = S::E >> R0, PR = GS / 2;  // 16-byte pairs per group
+  constexpr int EB0 = (D == 1 || D == 0) ? 16 : (D == 2 ? 8 : 4);
+  constexpr int GB = (EB0 >> R0) > 0 ? (EB0 >> R0) : 1;      // groups per batch
+#pragma unroll
+  for (int u0 = 0; u0 < NG; u0 += GB) {
+    if constexpr (D > 0) {
+      double2 a[D][GB][PR], b[D][GB][PR];
+#pragma unroll
+      for (int l = 0; l < D; l++)
+#pragma unroll
+        for (int u = 0; u < GB; u++)
+#pragma unroll
+          for (int k = 0; k < PR; k++) {
+            const size_t i = (size_t)((tid + S::T * (u0 + u)) << R0) + 2 * k;
+            a[l][u][k] = *reinterpret_cast<const double2*>(pa + (size_t)l * as + i);
+            b[l][u][k] = *reinterpret_cast<const double2*>(pb + (size_t)l * bs + i);
+          }
+#pragma unroll
+      for (int u = 0; u < GB; u++)
+#pragma unroll
+        for (int k = 0; k < PR; k++) {
+          double x = rf_mulmod_var(a[0][u][k].x, b[0][u][k].x, p, pinv);
+          double y = rf_mulmod_var(a[0][u][k].y, b[0][u][k].y, p, pinv);
+#pragma unroll
+          for (int l = 1; l < D; l++) {
+            x += rf_mulmod_var(a[l][u][k].x, b[l][u][k].x, p, pinv);
+            y += rf_mulmod_var(a[l][u][k].y, b[l][u][k].y, p, pinv);
+          }
+          v[(u0 + u) * GS + 2 * k] = rf_reduce(x, p, pinv);
+          v[(u0 + u) * GS + 2 * k + 1] = rf_reduce(y, p, pinv);
+        }
+    } else {
+      double acc[GB][GS];
+#pragma unroll
+      for (int u = 0; u < GB; u++)
+#pragma unroll
+        for (int j = 0; j < GS; j++) acc[u][j] = 0.0;
+      for (int l = 0; l < d; l++) {
+        double2 a[GB][PR], b[GB][PR];
+#pragma unroll
+        for (int u = 0; u < GB; u++)
+#pragma unroll
+          for (int k = 0; k < PR; k++) {
+            const size_t i = (size_t)((tid + S::T * (u0 + u)) << R0) + 2 * k;
+            a[u][k] = *reinterpret_cast<const double2*>(pa + (size_t)l * as + i);
+            b[u][k] = *reinterpret_cast<const double2*>(pb + (size_t)l * bs + i);
+          }
+#pragma unroll
+        for (int u = 0; u < GB; u++)
+#pragma unroll
+          for (int k = 0; k < PR; k++) {
+            acc[u][2 * k] += rf_mulmod_var(a[u][k].x, b[u][k].x, p, pinv);  // |term| <= 0.6 p
+            acc[u][2 * k + 1] += rf_mulmod_var(a[u][k].y, b[u][k].y, p, pinv);
+            if ((l & 7) == 7) {
+              acc[u][2 * k] = rf_reduce(acc[u][2 * k], p, pinv);
+              acc[u][2 * k + 1] = rf_reduce(acc[u][2 * k + 1], p, pinv);
+            }
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < GB; u++)
+#pragma unroll
+        for (int j = 0; j < GS; j++) v[(u0 + u) * GS + j] = rf_reduce(acc[u][j], p, pinv);
+    }
+  }
+}
+
 template <int LGN>
 __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(RfDevTables tb,
                                                              const RfInvTask* __restrict__ tasks,
@@ -502,7 +632,6 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
                                                              const double* __restrict__ bspec,
                                                              double* __restrict__ coef) {
   using S = Shape<LGN>;
-  using Q0 = Pass<LGN, 0>;
   extern __shared__ double sh_all[];
   const int sub = threadIdx.x / S::T, tid = threadIdx.x % S::T;
   const int wg = xcd_task(blockIdx.x, gridDim.x, (ntasks + S::G - 1) / S::G);
@@ -515,38 +644,52 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
   const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
   double v[S::E];
-  // Pointwise product in coalesced order (consecutive lanes, consecutive
-  // frequencies) into LDS; pass 0 then reads its groups of 2^R consecutive
-  // frequencies from there.  Loads are issued eight residues at a time per
-  // operand pair so that many are in flight.
   const double* pa = aspec + tk.a_off;
   const double* pb = bspec + tk.b_off;
-  switch (tk.d) {
-    case 1: pointwise<LGN, 1>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 1, p, pinv); break;
-    case 2: pointwise<LGN, 2>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 2, p, pinv); break;
-    case 3: pointwise<LGN, 3>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 3, p, pinv); break;
-    case 4: pointwise<LGN, 4>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 4, p, pinv); break;
-    default: pointwise<LGN, 0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, tk.d, p, pinv); break;
-  }
-  if constexpr (S::T > 1) {
+  constexpr int R0 = inv_r0<LGN>();
+  if constexpr (R0 <= 2) {
+    // the first pass's groups of 2 or 4 consecutive frequencies come
+    // straight from global memory as 16-byte pairs
+    switch (tk.d) {
+      case 1: pointwise_grp<LGN, 1, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 1, p, pinv); break;
+      case 2: pointwise_grp<LGN, 2, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 2, p, pinv); break;
+      case 3: pointwise_grp<LGN, 3, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 3, p, pinv); break;
+      case 4: pointwise_grp<LGN, 4, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 4, p, pinv); break;
+      default: pointwise_grp<LGN, 0, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, tk.d, p, pinv); break;
+    }
+  } else {
+    // Pointwise product in coalesced order (consecutive lanes, consecutive
+    // frequencies) into LDS; pass 0 then reads its groups of 16 consecutive
+    // frequencies from there.
+    switch (tk.d) {
+      case 1: pointwise<LGN, 1>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 1, p, pinv); break;
+      case 2: pointwise<LGN, 2>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 2, p, pinv); break;
+      case 3: pointwise<LGN, 3>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 3, p, pinv); break;
+      case 4: pointwise<LGN, 4>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 4, p, pinv); break;
+      default: pointwise<LGN, 0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, tk.d, p, pinv); break;
+    }
+    if constexpr (S::T > 1) {
+      using Q0 = Pass<LGN, 0, R0>;
 #pragma unroll
-    for (int e = 0; e < S::E; e++) sh[lds_pad(tid + S::T * e)] = v[e];
-    __syncthreads();
+      for (int e = 0; e < S::E; e++) sh[lds_pad(tid + S::T * e)] = v[e];
+      __syncthreads();
 #pragma unroll
-    for (int u = 0; u < Q0::NG; u++)
+      for (int u = 0; u < Q0::NG; u++)
 #pragma unroll
-      for (int j = 0; j < Q0::GS; j++) v[u * Q0::GS + j] = sh[lds_pad(inv_index<LGN, 0>(tid + S::T * u, j))];
-    __syncthreads();
+        for (int j = 0; j < Q0::GS; j++) v[u * Q0::GS + j] = sh[lds_pad(inv_index<LGN, 0>(tid + S::T * u, j))];
+      __syncthreads();
+    }
   }
   inv_pass<LGN, 0>(v, tid, tw, p);
   inv_rest<LGN, 1>(v, sh, tid, tw, p, pinv);
-  using QL = Pass<LGN, S::NP - 1>;
+  constexpr int NPI = npasses<LGN, R0>();
+  using QL = Pass<LGN, NPI - 1, R0>;
   double* out = coef + tk.out_off;
 #pragma unroll
   for (int u = 0; u < QL::NG; u++)
 #pragma unroll
     for (int j = 0; j < QL::GS; j++)
-      out[inv_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+      out[inv_index<LGN, NPI - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
 }
 
 // ---- transforms longer than one workgroup: N = 2^(RF_LGN_ROW + N1) ----
@@ -648,12 +791,7 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
     for (int j = 0; j < Q0::GS; j++) v[u * Q0::GS + j] = a[fwd_index<LGN, 0>(tid + S::T * u, j)];
   fwd_pass<LGN, 0, 0, true>(v, tid, tw, p, (int)tk.row);
   fwd_rest<LGN, 1, true>(v, sh_all, tid, tw, p, pinv, (int)tk.row);
-  using QL = Pass<LGN, S::NP - 1>;
-#pragma unroll
-  for (int u = 0; u < QL::NG; u++)
-#pragma unroll
-    for (int j = 0; j < QL::GS; j++)
-      a[fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+  fwd_store<LGN>(a, v, tid, p, pinv);
 }
 
 // Inverse column pass: after the row pass (k_inv on each row) the last N1
