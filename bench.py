@@ -265,6 +265,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=64, help="products per step")
+    ap.add_argument("--lanes", type=int, default=1, help="overlapping pipelines per prepared batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dropin", action="store_true",
@@ -299,7 +300,7 @@ def main():
     log(f"plan: chunk {64 * t} bits, N = 2^{lgN}, k = {k} primes; batch {args.batch}")
 
     batch.run(sync=True)  # one-shot path once, then the prepared pipeline
-    batch.prepare()
+    batch.prepare(lanes=args.lanes)
     for _ in range(max(args.warmup, 1)):
         batch.launch(graph=True)
     rf.device_sync()
@@ -320,7 +321,10 @@ def main():
     ms_per_step = el / args.steps * 1e3
     value = args.batch * args.steps / el
 
-    # Per-kernel timing with HIP events on the library's stream.
+    # Per-kernel timing with HIP events on the library's stream, on one lane
+    # (kernels not overlapped, whole batch per launch).
+    batch.release()
+    batch.prepare(lanes=1)
     rf.profile(enable=True)
     rf.profile(reset=True)
     psteps = max(3, min(args.steps, 10))

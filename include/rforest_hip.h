@@ -123,6 +123,10 @@ int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, i
    written at the device addresses given at creation, on every launch. */
 typedef struct rf_batch rf_batch;
 int rf_batch_create(const rf_dev_product* prods, int nprod, rf_batch** out);
+/* The same with the products split into `lanes` (1..4) independent
+   pipelines, balanced by planned cost, each on its own stream so that their
+   kernels may overlap (rf_batch_create uses one lane). */
+int rf_batch_create_lanes(const rf_dev_product* prods, int nprod, int lanes, rf_batch** out);
 /* Enqueues the whole pipeline on stream (NULL: the library's stream) and
    returns; with use_graph != 0 the launch sequence is captured once into a
    hipGraph and replayed.  Completion: rf_device_sync or the stream. */
@@ -266,7 +270,7 @@ int rf_primes(uint64_t* out, int max);
 #define RF_STAGE_FWD 0
 #define RF_STAGE_INV 1
 #define RF_STAGE_CRT 2
-#define RF_STAGE_CARRY 3
+#define RF_STAGE_CARRY 3  /* kept for the slot layout: block carries are now part of RF_STAGE_APPLY */
 #define RF_STAGE_APPLY 4
 #define RF_NSTAGES 5
 void rf_profile_enable(int on);

@@ -105,6 +105,9 @@ def lib():
     L.rf_device_sync.restype = ctypes.c_int
     L.rf_memcpy_d2d.restype = ctypes.c_int
     L.rf_memcpy_d2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    L.rf_batch_create_lanes.restype = ctypes.c_int
+    L.rf_batch_create_lanes.argtypes = [ctypes.POINTER(DevProduct), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
     L.rf_batch_create.restype = ctypes.c_int
     L.rf_batch_create.argtypes = [ctypes.POINTER(DevProduct), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.rf_batch_launch.restype = ctypes.c_int

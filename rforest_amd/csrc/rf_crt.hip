@@ -2,8 +2,8 @@
 //
 //   k_crt    Garner CRT of each coefficient + digit sums of the overlapping
 //            coefficients + block scan of carry transfer functions
-//   k_carry  per C entry: the carry into each block from the block aggregates
-//   k_apply  adds the resolved carries into the output limbs
+//   k_apply  per block: the carry into it from the earlier blocks' aggregates,
+//            then the resolved carries added into the output limbs
 //
 // The carry-propagating sum of src/lib/recompose.c:11-107 and the serial
 // fix-up of src/lib/crt_recompose.c:103-153 become a segmented scan: every
@@ -240,29 +240,38 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
   if (tid == RF_CRT_THREADS - 1) fun[tk.fun_idx] = (uint8_t)code[tid];
 }
 
-__global__ void k_carry(const RfCarryTask* __restrict__ tasks, int ntasks,
-                        const uint8_t* __restrict__ fun, int8_t* __restrict__ cin) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ntasks) return;
-  const RfCarryTask tk = tasks[e];
-  int c = 0;
-  for (uint32_t b = 0; b < tk.nblk; b++) {
-    cin[tk.fun_idx + b] = (int8_t)c;
-    c = fn_apply(fun[tk.fun_idx + b], c);
-  }
-}
-
+// Adds each digit's resolved carry into the output limbs.  The carry into
+// the block is the composition, in order, of the aggregates of the entry's
+// earlier blocks applied to 0: the first wave's lanes each compose a
+// contiguous run of them, then an ordered shuffle tree combines the runs.
 __global__ __launch_bounds__(RF_CRT_THREADS) void k_apply(const RfCrtTask* __restrict__ tasks,
                                                          const uint8_t* __restrict__ pre,
-                                                         const int8_t* __restrict__ cin) {
+                                                         const uint8_t* __restrict__ fun) {
+  __shared__ int cin_sh;
   const int tid = threadIdx.x;
   const RfCrtTask tk = rf_wave_uniform_load(tasks + blockIdx.x);
   const int t = tk.t, H = tk.halo;
+  if (tid < 64) {
+    const uint32_t nprev = tk.digit0 / (uint32_t)(RF_CRT_THREADS - H);  // earlier blocks of the entry
+    const uint32_t first = tk.fun_idx - nprev;
+    const uint32_t cnt = (nprev + 63) / 64;
+    uint32_t f = FN_ID;
+    const uint32_t b1 = min(nprev, (uint32_t)(tid + 1) * cnt);
+    for (uint32_t b = (uint32_t)tid * cnt; b < b1; b++) f = fn_compose(fun[first + b], f);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t g = __shfl_down(f, off);
+      if ((tid & (2 * off - 1)) == 0) f = fn_compose(g, f);
+    }
+    if (tid == 0) cin_sh = fn_apply(f, 0);
+  }
+  __syncthreads();
+  const int cin = cin_sh;
   if (tid < H) return;
   const uint32_t i = tk.digit0 + (uint32_t)(tid - H);
   const uint32_t ndig = (tk.rn + t - 1) / t;
   if (i >= ndig) return;
-  const int c = fn_apply(pre[tk.pre_off + tid], cin[tk.fun_idx]);
+  const int c = fn_apply(pre[tk.pre_off + tid], cin);
   if (c == 0) return;
   uint64_t* dst = tk.out + (size_t)i * t;
   const uint32_t lim = min((uint32_t)t, tk.rn - i * t);
@@ -305,16 +314,9 @@ hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, i
   }
 }
 
-hipError_t rf_launch_carry(const RfCarryTask* tasks, int ntasks, const uint8_t* fun, int8_t* cin,
+hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre, const uint8_t* fun,
                            hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_carry, dim3((ntasks + 63) / 64), dim3(64), 0, st, tasks, ntasks, fun, cin);
-  return hipGetLastError();
-}
-
-hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre,
-                           const int8_t* cin, hipStream_t st) {
-  if (ntasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_apply, dim3(ntasks), dim3(RF_CRT_THREADS), 0, st, tasks, pre, cin);
+  hipLaunchKernelGGL(k_apply, dim3(ntasks), dim3(RF_CRT_THREADS), 0, st, tasks, pre, fun);
   return hipGetLastError();
 }

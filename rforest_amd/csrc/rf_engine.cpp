@@ -252,7 +252,6 @@ Layout layout_of(std::vector<Product>& prods) {
       inv_n[pl.lgN] += (size_t)rd * pl.k;
     }
     crt_n[pl.k] += (size_t)rd * crt_blocks(pl);
-    L.ncarry += (size_t)rd;
     L.nblk += (uint32_t)(rd * crt_blocks(pl));
   }
   L.spec_b = align256(L.spec_n * 8);
@@ -266,7 +265,6 @@ Layout layout_of(std::vector<Product>& prods) {
   tb += align256(row_n * sizeof(RfRowTask));
   L.nrow = row_n;
   for (auto& kv : crt_n) tb += align256(kv.second * sizeof(RfCrtTask)), L.ncrt += kv.second;
-  tb += align256(L.ncarry * sizeof(RfCarryTask));
   L.tasks_b = tb;
   L.pre_b = align256((size_t)L.nblk * RF_CRT_THREADS);
   L.fun_b = align256(L.nblk);
@@ -284,7 +282,6 @@ int stage_of(int kind) {
     case K_FWD: case K_FWD_COL: case K_FWD_ROW: return RF_STAGE_FWD;
     case K_INV: case K_INV_COL: return RF_STAGE_INV;
     case K_CRT: return RF_STAGE_CRT;
-    case K_CARRY: return RF_STAGE_CARRY;
     default: return RF_STAGE_APPLY;
   }
 }
@@ -305,7 +302,6 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   std::map<int, std::vector<RfRowTask>> icol;
   std::vector<RfRowTask> frow;
   std::map<int, std::vector<RfCrtTask>> crt;
-  std::vector<RfCarryTask> carry;
   uint32_t nblk_total = 0, npre = 0;
   for (auto& P : prods) {
     const RfPlan& pl = P.plan;
@@ -366,7 +362,6 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
         }
     const uint32_t nblk = crt_blocks(pl);
     for (int e = 0; e < r * d; e++) {
-      carry.push_back(RfCarryTask{nblk_total, nblk});
       for (uint32_t b = 0; b < nblk; b++) {
         RfCrtTask t{};
         t.coef_off = P.coef_off + (uint64_t)e * k * N;
@@ -390,7 +385,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   }
   // Stream order: every forward pass (column passes before the row pass),
   // then every inverse pass (row passes before the column passes), CRT,
-  // block carries, carry application.
+  // carry application.
   for (auto& kv : fwd)
     pp.launches.push_back({(kv.first >> 4) > RF_LGN_LDS ? K_FWD_COL : K_FWD, kv.first,
                            append_tasks(pp.image, kv.second), (int)kv.second.size()});
@@ -406,8 +401,6 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
     pp.launches.push_back({K_CRT, kv.first, off, (int)kv.second.size()});
     crt_l.push_back({K_APPLY, kv.first, off, (int)kv.second.size()});
   }
-  if (!carry.empty())
-    pp.launches.push_back({K_CARRY, 0, append_tasks(pp.image, carry), (int)carry.size()});
   for (auto& l : crt_l) pp.launches.push_back(l);
   if (pp.image.size() > pp.L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
   return RF_OK;
@@ -421,7 +414,6 @@ ScratchPtrs scratch_ptrs(char* base, const Layout& L) {
   s.tasks = base + L.spec_b + L.coef_b;
   s.pre = (uint8_t*)(s.tasks + L.tasks_b);
   s.fun = s.pre + L.pre_b;
-  s.cin = (int8_t*)(s.fun + L.fun_b);
   return s;
 }
 
@@ -450,11 +442,8 @@ int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
       case K_CRT:
         HIPCHK(rf_launch_crt(g.tables, l.param, (const RfCrtTask*)t, l.n, S.coef, S.pre, S.fun, st));
         break;
-      case K_CARRY:
-        HIPCHK(rf_launch_carry((const RfCarryTask*)t, l.n, S.fun, S.cin, st));
-        break;
       case K_APPLY:
-        HIPCHK(rf_launch_apply((const RfCrtTask*)t, l.n, S.pre, S.cin, st));
+        HIPCHK(rf_launch_apply((const RfCrtTask*)t, l.n, S.pre, S.fun, st));
         break;
     }
   }
@@ -726,14 +715,54 @@ int products_from_dev(const rf_dev_product* prods, int nprod, std::vector<Produc
 }
 }  // namespace
 
+// A prepared batch runs as `lanes` independent pipelines over disjoint
+// product sets (balanced by planned cost), each on its own stream and
+// scratch: the lanes' kernels overlap, so one lane's memory-bound phases
+// fill another's compute-bound ones and launch gaps are hidden.
+#define RF_MAX_LANES 4
 struct rf_batch {
-  std::vector<Product> prods;
-  Pipeline pp;
+  int lanes = 1;
+  std::vector<std::vector<Product>> prods;  // per lane
+  std::vector<Pipeline> pp;                 // per lane
+  std::vector<size_t> off;                  // per lane scratch offset
   char* scratch = nullptr;
+  hipStream_t side[RF_MAX_LANES] = {};
+  hipEvent_t fork = nullptr, join[RF_MAX_LANES] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
 };
+
+namespace {
+// Enqueues every lane: lane 0 on st, lane i on side[i], forked from and
+// joined back into st with events (also under stream capture).
+int batch_enqueue(rf_batch* b, hipStream_t st, bool prof) {
+  if (b->lanes > 1) HIPCHK(hipEventRecord(b->fork, st));
+  for (int l = 0; l < b->lanes; l++) {
+    hipStream_t ls = l == 0 ? st : b->side[l];
+    if (l > 0) HIPCHK(hipStreamWaitEvent(ls, b->fork, 0));
+    int rc = launch_pipeline(b->pp[l], b->scratch + b->off[l], ls, prof);
+    if (rc) return rc;
+    if (l > 0) {
+      HIPCHK(hipEventRecord(b->join[l], ls));
+      HIPCHK(hipStreamWaitEvent(st, b->join[l], 0));
+    }
+  }
+  return RF_OK;
+}
+
+void batch_free(rf_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->scratch) (void)hipFree(b->scratch);
+  for (int l = 1; l < RF_MAX_LANES; l++) {
+    if (b->side[l]) (void)hipStreamDestroy(b->side[l]);
+    if (b->join[l]) (void)hipEventDestroy(b->join[l]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  delete b;
+}
+}  // namespace
 
 extern "C" {
 
@@ -757,28 +786,65 @@ int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, i
 }
 
 int rf_batch_create(const rf_dev_product* prods, int nprod, rf_batch** out) {
+  return rf_batch_create_lanes(prods, nprod, 1, out);
+}
+
+int rf_batch_create_lanes(const rf_dev_product* prods, int nprod, int lanes, rf_batch** out) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!out) return fail(RF_ERR_ARG, "rf_batch_create: null output");
   *out = nullptr;
   if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (lanes < 1 || lanes > RF_MAX_LANES) return fail(RF_ERR_ARG, "lanes must be 1..%d", RF_MAX_LANES);
+  std::vector<Product> all;
+  int rc = products_from_dev(prods, nprod, all);
+  if (rc) return rc;
   rf_batch* b = new rf_batch();
-  int rc = products_from_dev(prods, nprod, b->prods);
-  if (!rc) {
-    b->pp.L = layout_of(b->prods);
-    rc = build_pipeline(b->prods, b->pp);
+  b->lanes = std::max(1, std::min<int>(lanes, (int)all.size()));
+  // greedy balance by planned cost, largest first, order kept within a lane
+  std::vector<size_t> order(all.size());
+  for (size_t i = 0; i < all.size(); i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return all[x].plan.cost * all[x].r * all[x].d > all[y].plan.cost * all[y].r * all[y].d; });
+  std::vector<std::vector<size_t>> pick(b->lanes);
+  std::vector<double> load(b->lanes, 0.0);
+  for (size_t i : order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    pick[l].push_back(i);
+    load[l] += all[i].plan.cost * all[i].r * all[i].d;
+  }
+  b->prods.resize(b->lanes);
+  b->pp.resize(b->lanes);
+  b->off.resize(b->lanes);
+  size_t total = 0;
+  for (int l = 0; l < b->lanes && !rc; l++) {
+    std::sort(pick[l].begin(), pick[l].end());
+    for (size_t i : pick[l]) b->prods[l].push_back(all[i]);
+    rc = build_pipeline(b->prods[l], b->pp[l]);
+    b->off[l] = total;
+    total += align256(b->pp[l].L.total());
   }
   if (!rc) {
-    hipError_t e = hipMalloc((void**)&b->scratch, std::max<size_t>(b->pp.L.total(), 256));
-    if (e != hipSuccess) rc = fail(RF_ERR_NOMEM, "batch scratch of %zu bytes: %s", b->pp.L.total(), hipGetErrorString(e));
+    hipError_t e = hipMalloc((void**)&b->scratch, std::max<size_t>(total, 256));
+    if (e != hipSuccess) rc = fail(RF_ERR_NOMEM, "batch scratch of %zu bytes: %s", total, hipGetErrorString(e));
   }
-  if (!rc && !b->pp.image.empty()) {
-    const ScratchPtrs S = scratch_ptrs(b->scratch, b->pp.L);
-    hipError_t e = hipMemcpy(S.tasks, b->pp.image.data(), b->pp.image.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) rc = fail(RF_ERR_HIP, "task upload: %s", hipGetErrorString(e));
+  for (int l = 0; l < b->lanes && !rc; l++) {
+    if (!b->pp[l].image.empty()) {
+      const ScratchPtrs S = scratch_ptrs(b->scratch + b->off[l], b->pp[l].L);
+      hipError_t e = hipMemcpy(S.tasks, b->pp[l].image.data(), b->pp[l].image.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = fail(RF_ERR_HIP, "task upload: %s", hipGetErrorString(e));
+    }
+    if (!rc && l > 0) {
+      hipError_t e = hipStreamCreateWithFlags(&b->side[l], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming);
+      if (e != hipSuccess) rc = fail(RF_ERR_HIP, "batch lane: %s", hipGetErrorString(e));
+    }
+  }
+  if (!rc && b->lanes > 1) {
+    hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e != hipSuccess) rc = fail(RF_ERR_HIP, "batch fork event: %s", hipGetErrorString(e));
   }
   if (rc) {
-    if (b->scratch) (void)hipFree(b->scratch);
-    delete b;
+    batch_free(b);
     return rc;
   }
   *out = b;
@@ -790,7 +856,7 @@ int rf_batch_launch(rf_batch* b, void* stream, int use_graph) {
   if (!b) return fail(RF_ERR_ARG, "rf_batch_launch: null batch");
   if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
   hipStream_t st = stream ? (hipStream_t)stream : g.stream;
-  if (!use_graph || g.prof) return launch_pipeline(b->pp, b->scratch, st, g.prof);
+  if (!use_graph || g.prof) return batch_enqueue(b, st, g.prof);
   if (b->exec && b->graph_stream != st) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
@@ -799,7 +865,7 @@ int rf_batch_launch(rf_batch* b, void* stream, int use_graph) {
   }
   if (!b->exec) {
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = launch_pipeline(b->pp, b->scratch, st, false);
+    int rc = batch_enqueue(b, st, false);
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamEndCapture(st, &gr);
     if (rc) {
@@ -823,10 +889,7 @@ void rf_batch_destroy(rf_batch* b) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!b) return;
   if (g.ready) (void)hipDeviceSynchronize();
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->scratch) (void)hipFree(b->scratch);
-  delete b;
+  batch_free(b);
 }
 
 }  // extern "C"
@@ -1231,9 +1294,9 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
   const int halo = rf_crt_halo(k, t);
   const uint64_t D = RF_CRT_THREADS - halo;
   const uint32_t nblk = (uint32_t)((ndig + D - 1) / D);
-  const size_t task_b = align256(nblk * sizeof(RfCrtTask)) + align256(sizeof(RfCarryTask));
+  const size_t task_b = align256(nblk * sizeof(RfCrtTask));
   const size_t pre_b = align256((size_t)nblk * RF_CRT_THREADS), fun_b = align256(nblk);
-  rc = ensure_arena(coef_b + out_b + task_b + pre_b + 2 * fun_b);
+  rc = ensure_arena(coef_b + out_b + task_b + pre_b + fun_b);
   if (rc) return rc;
   rc = ensure_stage(std::max(coef_b + task_b, out_b));
   if (rc) return rc;
@@ -1252,7 +1315,6 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
   char* d_task = A + coef_b + out_b;
   uint8_t* d_pre = (uint8_t*)(d_task + task_b);
   uint8_t* d_fun = d_pre + pre_b;
-  int8_t* d_cin = (int8_t*)(d_fun + fun_b);
   char* ht = g.h_stage + coef_b;
   for (uint32_t b = 0; b < nblk; b++) {
     RfCrtTask tk{};
@@ -1270,15 +1332,11 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
     tk.halo = (uint8_t)halo;
     ((RfCrtTask*)ht)[b] = tk;
   }
-  RfCarryTask ct{0, nblk};
-  memcpy(ht + align256(nblk * sizeof(RfCrtTask)), &ct, sizeof ct);
   HIPCHK(hipMemcpyAsync(d_coef, hs, (size_t)k * nout * 8, hipMemcpyHostToDevice, g.stream));
   HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
   const RfCrtTask* tasks = (const RfCrtTask*)d_task;
-  const RfCarryTask* ctask = (const RfCarryTask*)(d_task + align256(nblk * sizeof(RfCrtTask)));
   HIPCHK(rf_launch_crt(g.tables, k, tasks, (int)nblk, d_coef, d_pre, d_fun, g.stream));
-  HIPCHK(rf_launch_carry(ctask, 1, d_fun, d_cin, g.stream));
-  HIPCHK(rf_launch_apply(tasks, (int)nblk, d_pre, d_cin, g.stream));
+  HIPCHK(rf_launch_apply(tasks, (int)nblk, d_pre, d_fun, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(out, d_out, rn * 8, hipMemcpyDeviceToHost));
   return RF_OK;

@@ -56,12 +56,12 @@ struct Product {
 struct Layout {
   uint64_t spec_n = 0, coef_n = 0;  // doubles
   uint32_t nblk = 0;                // CRT blocks in total
-  size_t nfwd = 0, ninv = 0, nrow = 0, ncol = 0, ncrt = 0, ncarry = 0;
+  size_t nfwd = 0, ninv = 0, nrow = 0, ncol = 0, ncrt = 0;
   size_t spec_b = 0, coef_b = 0, tasks_b = 0, pre_b = 0, fun_b = 0;
-  size_t total() const { return spec_b + coef_b + tasks_b + pre_b + 2 * fun_b; }
+  size_t total() const { return spec_b + coef_b + tasks_b + pre_b + fun_b; }
 };
 
-enum Kind { K_FWD, K_FWD_COL, K_FWD_ROW, K_INV, K_INV_COL, K_CRT, K_CARRY, K_APPLY };
+enum Kind { K_FWD, K_FWD_COL, K_FWD_ROW, K_INV, K_INV_COL, K_CRT, K_APPLY };
 int stage_of(int kind);
 struct LaunchRec {
   int kind;    // Kind
@@ -82,7 +82,6 @@ struct ScratchPtrs {
   char* tasks;
   uint8_t* pre;
   uint8_t* fun;
-  int8_t* cin;
 };
 
 // The library's global lock, its readiness and its device tables.

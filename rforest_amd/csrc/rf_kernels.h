@@ -79,12 +79,6 @@ static inline int rf_crt_halo(int k, int t) {
   return (S + t - 1) / t + 2;
 }
 
-// Per C entry, for the block-carry pass.
-struct RfCarryTask {
-  uint32_t fun_idx;   // first block's index in fun / cin
-  uint32_t nblk;
-};
-
 // A descriptor read by a whole wave: loaded as 32-bit words and broadcast
 // from the first lane, so its fields live in scalar registers and branches
 // on them stay uniform.  Only for descriptors all lanes of the wave share.
@@ -127,7 +121,5 @@ hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* ta
 // Blocks with the same k must be launched together (the kernel is specialised on k).
 hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, int ntasks,
                          const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st);
-hipError_t rf_launch_carry(const RfCarryTask* tasks, int ntasks, const uint8_t* fun, int8_t* cin,
+hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre, const uint8_t* fun,
                            hipStream_t st);
-hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre,
-                           const int8_t* cin, hipStream_t st);

@@ -188,11 +188,12 @@ class DeviceBatch:
         ptr = ctypes.cast(ctypes.byref(self.prods, i * ctypes.sizeof(DevProduct)), ctypes.POINTER(DevProduct))
         check(lib().rf_batch_mult_device(ptr, 1, None, 0 if sync else 1))
 
-    def prepare(self):
-        """Fixes plans, task tables and scratch once (rf_batch_create)."""
+    def prepare(self, lanes=1):
+        """Fixes plans, task tables and scratch once (rf_batch_create_lanes):
+        the products run as `lanes` overlapping pipelines."""
         if self.handle is None:
             h = ctypes.c_void_p()
-            check(lib().rf_batch_create(self.prods, len(self.products), ctypes.byref(h)))
+            check(lib().rf_batch_create_lanes(self.prods, len(self.products), lanes, ctypes.byref(h)))
             self.handle = h
         return self
 

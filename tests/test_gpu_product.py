@@ -175,3 +175,22 @@ def test_prepared_batch_graph_replays_match_eager():
         rf.device_sync()
         assert batch.results() == eager
     batch.release()
+
+
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_prepared_batch_lanes_match_one_lane(lanes):
+    """Products split over several overlapping lanes (own streams, forked and
+    joined inside the captured graph) give the one-lane results."""
+    import numpy as np
+    prods = workloads.tree_level(20, seed=8) + workloads.headline(2, seed=42)
+    batch = rf.DeviceBatch(prods)
+    batch.run()
+    want = batch.results()
+    batch.prepare(lanes=lanes)
+    junk = np.full(max(batch.total_out, 1), 0x77, dtype=np.uint64)
+    for graph in (True, False, True):
+        batch.out.upload(junk)
+        batch.launch(graph=graph)
+        rf.device_sync()
+        assert batch.results() == want
+    batch.release()
