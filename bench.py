@@ -117,14 +117,33 @@ def tree_level_section(rf, workloads, reps):
         rf.mpz_rmatrix_mult_fft(prods[i][2], 2, prods[i][3], 2)
     plain_host = ns / (time.perf_counter() - t0)
     batch.release()
+    # the same level planned by the cost model alone (no measured plan table)
+    from rforest_amd import tune
+    nrules = tune.reset()
+    tune.clear()
+    mb = rf.DeviceBatch(prods)
+    mb.prepare()
+    mb.launch(graph=True)
+    rf.device_sync()
+    same_model = mb.results() == res
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mb.launch(graph=True)
+    rf.device_sync()
+    model = 1024 * reps / (time.perf_counter() - t0)
+    mb.release()
+    tune.reset()
     return {"workload": "1024 products 2x2 . 2x2, entries log-uniform in [2^10, 2^18) bits (matrix within 2x), "
                         "random signs, GMP seed 1003",
+            "autotuned_plans": nrules,
+            "cost_model_products_per_s": round(model, 1),
+            "speedup_vs_cost_model": round(batched / model, 3),
             "batched_products_per_s": round(batched, 1),
             "plain_device_products_per_s": round(plain_dev, 1),
             "plain_dropin_products_per_s": round(plain_host, 1),
             "speedup_vs_plain_device": round(batched / plain_dev, 2),
             "speedup_vs_plain_dropin": round(batched / plain_host, 2),
-            "checks": {"mod_2^61-1_mismatches": bad, "exact_sample": exact}}
+            "checks": {"mod_2^61-1_mismatches": bad, "exact_sample": exact, "identical_to_cost_model": same_model}}
 
 
 def resident_section(rf, workloads, reps):

@@ -234,6 +234,14 @@ int rf_server_destroy(rf_server* s);
    checking the server's results against other paths). */
 int rf_gen_entry_device(uint64_t* dst, uint64_t bits, uint64_t seed, uint32_t entry);
 int rf_checksum_device(const uint64_t* limbs, uint64_t n, uint32_t entry, uint64_t* sum);
+/* The same for many entries in one launch: entry e of the synthetic stream
+   of request seeds[i] with bits[i] bits into dst[i] (bits 0: nothing);
+   checksum shares of count C entries (n[c] limbs at src[c], C-entry index
+   entry[c]) summed into sums[slot[c]], nslots sums. */
+int rf_gen_entries_device(uint64_t* const* dst, const uint64_t* bits, const uint64_t* seeds,
+                          const uint32_t* entry, int n);
+int rf_checksums_device(const uint64_t* const* src, const uint64_t* n, const uint32_t* entry, const uint32_t* slot,
+                        int count, uint64_t* sums, int nslots);
 
 /* ---------------------------------------------------------------------------
  * Device and pinned memory (src/include/hwmpz.h:38-46 declares the same
@@ -260,6 +268,18 @@ int rf_device_sync(void);
 /* The plan the library would use: chunk width 64*t bits, transform length
    2^lgN, k primes.  Returns RF_OK or RF_ERR_UNSUPPORTED. */
 int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k);
+
+/* Planner tuning (measured by the autotuner, rforest_amd/tune.py): a square
+   r x r product planned with chunk width 64*t and 2^lgN points (split != 0:
+   2^14 points as two rows of 2^13) took cost_us microseconds per product.
+   For square products the planner then picks the measured valid plan of
+   least cost, falling back to its cost model when none is measured.
+   rf_tune_load reads "r lgN t split cost_us" lines (# comments) and returns
+   the number read; hw_mpz_setup loads tuned_plans.txt from the library's
+   directory when it exists. */
+int rf_tune_set(int r, int lgN, int t, int split, double cost_us);
+void rf_tune_clear(void);
+int rf_tune_load(const char* path);
 
 /* Copies up to max primes of the library's prime set (all 1 mod 2^24,
    between 2^47 and 2^51 / 9); returns how many exist. */

@@ -8,6 +8,7 @@
 //   count) -> k_carry -> k_apply
 // on the stream.  Host mpz entry points add one H2D copy of the limbs and
 // one D2H copy of the results around that.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -82,6 +83,27 @@ int host_init_locked() {
   return RF_OK;
 }
 
+// Tuned planner rules: $RF_TUNE_FILE if set ("none": no rules), otherwise
+// tuned_plans.txt beside the library when it exists.
+void load_tuned_plans() {
+  std::string path;
+  const char* env = getenv("RF_TUNE_FILE");
+  if (env) {
+    if (strcmp(env, "none") == 0) return;
+    path = env;
+  } else {
+    Dl_info info;
+    if (!dladdr((void*)&hw_mpz_setup, &info) || !info.dli_fname) return;
+    path = info.dli_fname;
+    const size_t slash = path.rfind('/');
+    path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuned_plans.txt";
+  }
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) return;
+  fclose(f);
+  (void)rf_tune_load(path.c_str());
+}
+
 int setup_locked() {
   if (g.ready) return RF_OK;
   int rc0 = host_init_locked();
@@ -103,6 +125,7 @@ int setup_locked() {
   g.tables.tw_fwd = g.d_twf;
   g.tables.tw_inv = g.d_twi;
   g.ready = true;
+  load_tuned_plans();
   return RF_OK;
 }
 
@@ -185,6 +208,42 @@ void prof_collect() {
 }
 
 
+// Tuning rules (rf_tune_*), read by every planning call; their own lock, since
+// the request server plans outside the library lock.
+std::mutex g_tune_mu;
+std::vector<RfTuneRule> g_tune;
+
+bool tuned_plan(uint64_t LA, uint64_t LB, int r, int d, RfPlan* out) {
+  if (r != d || LA == 0 || LB == 0) return false;
+  std::vector<RfTuneRule> rules;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto& x : g_tune)
+      if (x.r == r) rules.push_back(x);
+  }
+  if (rules.empty()) return false;
+  bool found = false;
+  double best = 0;
+  for (int t = 1; t <= RF_TMAX; t++) {
+    const uint64_t b = 64 * (uint64_t)t;
+    const uint64_t nA = (LA + b - 1) / b, nB = (LB + b - 1) / b;
+    int lgN = RF_LGN_MIN;
+    while (lgN < RF_LGN_MAX && ((uint64_t)1 << lgN) < nA + nB - 1) lgN++;
+    RfPlan pl;
+    if (rf_plan_fixed(&g.primes, LA, LB, r, d, t, lgN, &pl) != 0) continue;
+    for (const auto& x : rules) {
+      if (x.lgN != lgN || x.t != t || (x.split && lgN != RF_LGN_LDS)) continue;
+      if (!found || x.cost < best) {
+        best = x.cost;
+        found = true;
+        pl.split = x.split ? 1 : 0;
+        *out = pl;
+      }
+    }
+  }
+  return found;
+}
+
 int plan_product(Product& P) {
   uint64_t LA = 0, LB = 0;
   for (auto& e : P.A) LA = std::max(LA, e.sign ? e.bits : 0);
@@ -202,6 +261,7 @@ int plan_product(Product& P) {
                   (unsigned long long)LB);
     return RF_OK;
   }
+  if (LA && LB && tuned_plan(LA, LB, P.r, P.d, &P.plan)) return RF_OK;
   if (rf_plan_choose(&g.primes, LA, LB, P.r, P.d, &P.plan) != 0)
     return fail(RF_ERR_UNSUPPORTED,
                 "no transform plan for entries of %llu x %llu bits (r=%d, d=%d)",
@@ -219,7 +279,11 @@ uint64_t out_limbs_for(uint64_t la_bits, uint64_t lb_bits, int d) {
 
 // Forward launch groups: one per transform length, and per chunk width
 // where the kernels are specialised on it.
-int fwd_key(int lgN, int t) { return (lgN << 4) | ((lgN >= RF_FWD_TSPEC || lgN > RF_LGN_LDS) ? t : 0); }
+bool plan_split(const RfPlan& pl) { return pl.lgN > RF_LGN_LDS || pl.split; }
+int fwd_key(const RfPlan& pl) {
+  const bool sp = plan_split(pl);
+  return (sp ? 1 << 12 : 0) | (pl.lgN << 4) | ((pl.lgN >= RF_FWD_TSPEC || sp) ? pl.t : 0);
+}
 
 uint32_t crt_blocks(const RfPlan& pl) {
   const uint64_t ndig = (pl.rn + pl.t - 1) / pl.t;
@@ -242,8 +306,8 @@ Layout layout_of(std::vector<Product>& prods) {
     const int npoly = P.left ? dd : rd + dd;  // polys transformed here
     L.spec_n += (uint64_t)npoly * pl.k * N;
     L.coef_n += (uint64_t)rd * pl.k * N;
-    fwd_n[fwd_key(pl.lgN, pl.t)] += (size_t)npoly * pl.k;
-    if (pl.lgN > RF_LGN_LDS) {
+    fwd_n[fwd_key(pl)] += (size_t)npoly * pl.k;
+    if (plan_split(pl)) {
       const size_t rows = (size_t)1 << (pl.lgN - RF_LGN_ROW);
       row_n += (size_t)npoly * pl.k * rows;
       inv_n[RF_LGN_ROW] += (size_t)rd * pl.k * rows;
@@ -332,11 +396,11 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
         t.q = (uint16_t)q;
         t.t = (uint8_t)pl.t;
         t.scale = isB ? (uint8_t)pl.lgN : 0;
-        fwd[fwd_key(pl.lgN, pl.t)].push_back(t);
+        fwd[fwd_key(pl)].push_back(t);
       }
     }
     // Longer transforms: 2^(lgN-13) rows of 2^13 residues each.
-    const bool split = pl.lgN > RF_LGN_LDS;
+    const bool split = plan_split(pl);
     const uint32_t rows = split ? 1u << (pl.lgN - RF_LGN_ROW) : 1u;
     const uint64_t rowlen = split ? (uint64_t)1 << RF_LGN_ROW : N;
     if (split)
@@ -387,7 +451,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   // then every inverse pass (row passes before the column passes), CRT,
   // carry application.
   for (auto& kv : fwd)
-    pp.launches.push_back({(kv.first >> 4) > RF_LGN_LDS ? K_FWD_COL : K_FWD, kv.first,
+    pp.launches.push_back({(kv.first >> 12) ? K_FWD_COL : K_FWD, kv.first & 4095,
                            append_tasks(pp.image, kv.second), (int)kv.second.size()});
   if (!frow.empty()) pp.launches.push_back({K_FWD_ROW, RF_LGN_ROW, append_tasks(pp.image, frow), (int)frow.size()});
   for (auto& kv : inv)
@@ -901,7 +965,7 @@ int operand_transform(rf_operand* op, const std::vector<Entry>& A, size_t base) 
   const RfPlan& pl = op->plan;
   const int k = pl.k, lgN = pl.lgN;
   const uint64_t N = (uint64_t)1 << lgN;
-  const bool split = lgN > RF_LGN_LDS;
+  const bool split = plan_split(pl);
   const uint32_t rows = split ? 1u << (lgN - RF_LGN_ROW) : 0u;
   std::vector<RfFwdTask> ft;
   std::vector<RfRowTask> rt;
@@ -952,7 +1016,8 @@ int operand_make(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_
   }
   int rc = RF_OK;
   if (op->la_bits && b_max_bits) {
-    if (rf_plan_choose(&g.primes, op->la_bits, b_max_bits, r, d, &op->plan) != 0)
+    if (!tuned_plan(op->la_bits, b_max_bits, r, d, &op->plan) &&
+        rf_plan_choose(&g.primes, op->la_bits, b_max_bits, r, d, &op->plan) != 0)
       rc = fail(RF_ERR_UNSUPPORTED, "no transform plan for a %dx%d operand of %llu bits and right entries of %llu bits",
                 r, d, (unsigned long long)op->la_bits, (unsigned long long)b_max_bits);
     if (!rc) {
@@ -1052,11 +1117,49 @@ int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int*
   if (rc) return rc;
   if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
   RfPlan pl;
-  if (rf_plan_choose(&g.primes, la_bits, lb_bits, r, d, &pl) != 0) return fail(RF_ERR_UNSUPPORTED, "no plan");
+  if (!(la_bits && lb_bits && tuned_plan(la_bits, lb_bits, r, d, &pl)) &&
+      rf_plan_choose(&g.primes, la_bits, lb_bits, r, d, &pl) != 0)
+    return fail(RF_ERR_UNSUPPORTED, "no plan");
   *t = pl.t;
   *lgN = pl.lgN;
   *k = pl.k;
   return RF_OK;
+}
+
+int rf_tune_set(int r, int lgN, int t, int split, double cost_us) {
+  if (r <= 0 || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || t < 1 || t > RF_TMAX || !(cost_us >= 0))
+    return fail(RF_ERR_ARG, "rf_tune_set: bad measurement");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& x : g_tune)
+    if (x.r == r && x.lgN == lgN && x.t == t && x.split == (split != 0)) {
+      x.cost = cost_us;
+      return RF_OK;
+    }
+  g_tune.push_back(RfTuneRule{r, lgN, t, split != 0, cost_us});
+  return RF_OK;
+}
+
+void rf_tune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+}
+
+int rf_tune_load(const char* path) {
+  FILE* f = fopen(path, "r");
+  if (!f) return fail(RF_ERR_ARG, "cannot open %s", path);
+  char line[256];
+  int n = 0, rc = RF_OK;
+  while (fgets(line, sizeof line, f)) {
+    int r, lgN, t, sp;
+    double cost;
+    if (line[0] == '#') continue;
+    if (sscanf(line, "%d %d %d %d %lf", &r, &lgN, &t, &sp, &cost) != 5) continue;
+    rc = rf_tune_set(r, lgN, t, sp, cost);
+    if (rc) break;
+    n++;
+  }
+  fclose(f);
+  return rc ? rc : n;
 }
 
 int rf_primes(uint64_t* out, int max) {

@@ -353,11 +353,22 @@ __device__ __forceinline__ double pack_combine(const uint64_t (&x)[T], const dou
   return rf_reduce(r, p, pinv);
 }
 
+// Chunks per sub-batch: the largest power of two with at most 16 limbs
+// (at least one chunk), never more than NCH; NCH is a power of two, so
+// sub-batches tile it exactly.
+template <int NCH, int T>
+constexpr int pack_sb() {
+  int sb = 1;
+  while (sb * 2 <= NCH && sb * 2 * T <= 16) sb *= 2;
+  return sb;
+}
+
 template <int NCH, int T>
 __device__ __forceinline__ void pack_full16(const uint64_t* __restrict__ limbs, const uint32_t (&ci)[NCH],
                                             const double (*pk)[3], const double* pk2, double p, double pinv,
                                             double (&out)[NCH]) {
-  constexpr int SB = NCH * T <= 16 ? NCH : (16 / T > 0 ? 16 / T : 1);
+  constexpr int SB = pack_sb<NCH, T>();
+  static_assert(NCH % SB == 0, "sub-batches must tile the chunks");
 #pragma unroll
   for (int c0 = 0; c0 < NCH; c0 += SB) {
     uint64_t x[SB][T];
@@ -380,7 +391,8 @@ template <int NCH, int T>
 __device__ __forceinline__ void pack_any8(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
                                           const uint32_t (&ci)[NCH], const double (*pk)[3], const double* pk2,
                                           double p, double pinv, double (&out)[NCH]) {
-  constexpr int SB = NCH * T <= 16 ? NCH : (16 / T > 0 ? 16 / T : 1);
+  constexpr int SB = pack_sb<NCH, T>();
+  static_assert(NCH % SB == 0, "sub-batches must tile the chunks");
 #pragma unroll
   for (int c0 = 0; c0 < NCH; c0 += SB) {
     uint64_t x[SB][T];
@@ -933,6 +945,7 @@ hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, int t, const RfFwdT
   if (ntasks <= 0) return hipSuccess;
   const dim3 grid(ntasks * ((1 << RF_LGN_ROW) / RF_COL_THREADS)), blk(RF_COL_THREADS);
   switch (lgN - RF_LGN_ROW) {
+    case 1: return launch_fwd_col_n<1>(t, grid, blk, tb, tasks, ntasks, spec, st);
     case 2: return launch_fwd_col_n<2>(t, grid, blk, tb, tasks, ntasks, spec, st);
     case 3: return launch_fwd_col_n<3>(t, grid, blk, tb, tasks, ntasks, spec, st);
     case 4: return launch_fwd_col_n<4>(t, grid, blk, tb, tasks, ntasks, spec, st);
@@ -959,7 +972,7 @@ hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* ta
   case N1:          \
     hipLaunchKernelGGL(k_inv_col<N1>, grid, blk, 0, st, tb, tasks, ntasks, coef); \
     return hipGetLastError();
-    RF_CASE(2) RF_CASE(3) RF_CASE(4)
+    RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4)
 #undef RF_CASE
     default:
       return hipErrorInvalidValue;

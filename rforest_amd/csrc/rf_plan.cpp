@@ -118,6 +118,7 @@ int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int 
   memset(&pl, 0, sizeof pl);
   pl.t = t;
   pl.lgN = lgN;
+  pl.split = 0;
   size_t nA = (LA + b - 1) / b, nB = (LB + b - 1) / b;
   pl.nA = (int)nA;
   pl.nB = (int)nB;

@@ -41,6 +41,17 @@ struct RfPlan {
   int M;        // coefficients overlapping one output digit: ceil(s / t)
   size_t rn;    // output limbs per C entry (two's complement, sign included)
   double cost;  // planner's estimate, arbitrary units
+  int split;    // 2^14 points as two rows of 2^13 (column + row passes) instead of one workgroup
+};
+
+// A tuning measurement: a square r x r product whose plan has chunk width
+// 64 t and 2^lgN points (with the two-row split of 2^14 points if split)
+// took `cost` microseconds per product in a batch of such products.  The
+// planner picks, among the valid plans of a product that have measurements,
+// the cheapest.
+struct RfTuneRule {
+  int r, lgN, t, split;
+  double cost;
 };
 
 // Generates the RF_MAXP largest primes 2^47 < p < RF_PMAX with p = 1 mod 2^RF_S.

@@ -582,43 +582,69 @@ int rf_server_destroy(rf_server* S) {
   return RF_OK;
 }
 
-int rf_gen_entry_device(uint64_t* dst, uint64_t bits, uint64_t seed, uint32_t entry) {
-  if (!dst || bits == 0) return fail(RF_ERR_ARG, "rf_gen_entry_device: bad arguments");
+int rf_gen_entries_device(uint64_t* const* dst, const uint64_t* bits, const uint64_t* seeds,
+                          const uint32_t* entry, int n) {
+  if (n < 0 || (n > 0 && (!dst || !bits || !seeds || !entry))) return fail(RF_ERR_ARG, "rf_gen_entries_device: bad arguments");
+  std::vector<RfSpan> spans;
   std::vector<RfBlock> blocks;
-  const uint64_t n = (bits + 63) / 64;
-  for (uint64_t f = 0; f < n; f += RF_UTIL_LIMBS) blocks.push_back(RfBlock{0, 0, f});
-  RfSpan sp{dst, bits, rf_gen_key(seed, entry), 0, 0};
+  for (int e = 0; e < n; e++) {
+    if (bits[e] == 0) continue;
+    if (!dst[e]) return fail(RF_ERR_ARG, "rf_gen_entries_device: null destination");
+    spans.push_back(RfSpan{dst[e], bits[e], rf_gen_key(seeds[e], entry[e]), 0, 0});
+    const uint64_t nl = (bits[e] + 63) / 64;
+    for (uint64_t f = 0; f < nl; f += RF_UTIL_LIMBS) blocks.push_back(RfBlock{(uint32_t)(spans.size() - 1), 0, f});
+  }
+  if (blocks.empty()) return RF_OK;
+  const size_t b0 = align256(spans.size() * sizeof(RfSpan));
   char* d = nullptr;
-  const size_t b0 = align256(sizeof(RfSpan));
   HIPCHK(hipMalloc((void**)&d, b0 + blocks.size() * sizeof(RfBlock)));
-  hipError_t e = hipMemcpy(d, &sp, sizeof sp, hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(d, spans.data(), spans.size() * sizeof(RfSpan), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(d + b0, blocks.data(), blocks.size() * sizeof(RfBlock), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = rf_launch_gen((const RfSpan*)d, (const RfBlock*)(d + b0), (int)blocks.size(), nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   (void)hipFree(d);
-  if (e != hipSuccess) return fail(RF_ERR_HIP, "rf_gen_entry_device: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return fail(RF_ERR_HIP, "rf_gen_entries_device: %s", hipGetErrorString(e));
+  return RF_OK;
+}
+
+int rf_gen_entry_device(uint64_t* dst, uint64_t bits, uint64_t seed, uint32_t entry) {
+  if (!dst || bits == 0) return fail(RF_ERR_ARG, "rf_gen_entry_device: bad arguments");
+  return rf_gen_entries_device(&dst, &bits, &seed, &entry, 1);
+}
+
+int rf_checksums_device(const uint64_t* const* src, const uint64_t* n, const uint32_t* entry, const uint32_t* slot,
+                        int count, uint64_t* sums, int nslots) {
+  if (count < 0 || nslots <= 0 || !sums || (count > 0 && (!src || !n || !entry || !slot)))
+    return fail(RF_ERR_ARG, "rf_checksums_device: bad arguments");
+  std::vector<RfSpan> spans;
+  std::vector<RfBlock> blocks;
+  for (int c = 0; c < count; c++) {
+    if (slot[c] >= (uint32_t)nslots) return fail(RF_ERR_ARG, "rf_checksums_device: slot out of range");
+    if (n[c] == 0) continue;
+    spans.push_back(RfSpan{(uint64_t*)src[c], n[c], (uint64_t)entry[c] << 40, slot[c], 0});
+    for (uint64_t f = 0; f < n[c]; f += RF_UTIL_LIMBS) blocks.push_back(RfBlock{(uint32_t)(spans.size() - 1), 0, f});
+  }
+  const size_t b0 = align256(std::max<size_t>(spans.size(), 1) * sizeof(RfSpan));
+  const size_t b1 = b0 + align256(std::max<size_t>(blocks.size(), 1) * sizeof(RfBlock));
+  char* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, b1 + (size_t)nslots * 8));
+  hipError_t e = hipSuccess;
+  if (!spans.empty()) e = hipMemcpy(d, spans.data(), spans.size() * sizeof(RfSpan), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !blocks.empty())
+    e = hipMemcpy(d + b0, blocks.data(), blocks.size() * sizeof(RfBlock), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = rf_launch_fill((uint64_t*)(d + b1), (uint64_t)nslots, 0, nullptr);
+  if (e == hipSuccess)
+    e = rf_launch_checksum((const RfSpan*)d, (const RfBlock*)(d + b0), (int)blocks.size(), (uint64_t*)(d + b1), nullptr);
+  if (e == hipSuccess) e = hipMemcpy(sums, d + b1, (size_t)nslots * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(RF_ERR_HIP, "rf_checksums_device: %s", hipGetErrorString(e));
   return RF_OK;
 }
 
 int rf_checksum_device(const uint64_t* limbs, uint64_t n, uint32_t entry, uint64_t* sum) {
   if (!limbs || !sum) return fail(RF_ERR_ARG, "rf_checksum_device: bad arguments");
-  *sum = 0;
-  if (n == 0) return RF_OK;
-  std::vector<RfBlock> blocks;
-  for (uint64_t f = 0; f < n; f += RF_UTIL_LIMBS) blocks.push_back(RfBlock{0, 0, f});
-  RfSpan sp{(uint64_t*)limbs, n, (uint64_t)entry << 40, 0, 0};
-  char* d = nullptr;
-  const size_t b0 = align256(sizeof(RfSpan)), b1 = b0 + align256(blocks.size() * sizeof(RfBlock));
-  HIPCHK(hipMalloc((void**)&d, b1 + 8));
-  hipError_t e = hipMemcpy(d, &sp, sizeof sp, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d + b0, blocks.data(), blocks.size() * sizeof(RfBlock), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = rf_launch_fill((uint64_t*)(d + b1), 1, 0, nullptr);
-  if (e == hipSuccess)
-    e = rf_launch_checksum((const RfSpan*)d, (const RfBlock*)(d + b0), (int)blocks.size(), (uint64_t*)(d + b1), nullptr);
-  if (e == hipSuccess) e = hipMemcpy(sum, d + b1, 8, hipMemcpyDeviceToHost);
-  (void)hipFree(d);
-  if (e != hipSuccess) return fail(RF_ERR_HIP, "rf_checksum_device: %s", hipGetErrorString(e));
-  return RF_OK;
+  const uint32_t slot = 0;
+  return rf_checksums_device(&limbs, &n, &entry, &slot, 1, sum, 1);
 }
 
 }  // extern "C"

@@ -117,14 +117,33 @@ class DeviceBatch:
     ResidentOperand) once; run() enqueues the whole pipeline over all of
     them; results() reads back."""
 
-    def __init__(self, products):
+    def __init__(self, products, synthetic=None):
         import numpy as np
         self.products = products
         L = lib()
         chunks = []
         pos = 0
         plan_in = []
-        for (r, d, A, B) in products:
+        if synthetic is not None:
+            # entries of the synthetic stream (rf_util.h), generated in place
+            gen = []
+            for (r, d, bits, signs, seed) in synthetic:
+                ents = []
+                for e, (b, sg) in enumerate(zip(bits, signs)):
+                    n = (int(b) + 63) // 64 if sg else 0
+                    ents.append((pos, n, int(sg) if n else 0))
+                    if n:
+                        gen.append((pos, int(b), seed, e))
+                    pos += n + (n % 2)
+                la = max([n for (_, n, s) in ents[:r * d] if s] or [0])
+                lb = max([n for (_, n, s) in ents[r * d:] if s] or [0])
+                plan_in.append((r, d, ents, int(L.rf_out_limbs(la, lb, d)), None))
+            self.products = [None] * len(synthetic)
+            self.limbs = DeviceBuffer(8 * max(pos, 1))
+            from .serve import gen_entries_device
+            gen_entries_device([self.limbs.ptr + 8 * off for (off, _, _, _) in gen], [b for (_, b, _, _) in gen],
+                               [sd for (_, _, sd, _) in gen], [e for (_, _, _, e) in gen])
+        for (r, d, A, B) in ([] if synthetic is not None else products):
             left = A if isinstance(A, ResidentOperand) else None
             ents = []
             for x in ([] if left else list(A)) + list(B):
@@ -141,15 +160,16 @@ class DeviceBatch:
             la = left.la_limbs if left else max([n for (_, n, s) in ents[:na] if s] or [0])
             lb = max([n for (_, n, s) in ents[na:] if s] or [0])
             plan_in.append((r, d, ents, int(L.rf_out_limbs(la, lb, d)), left))
-        host = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.uint64)
-        self.limbs = DeviceBuffer(host.nbytes)
-        self.limbs.upload(host)
+        if synthetic is None:
+            host = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.uint64)
+            self.limbs = DeviceBuffer(host.nbytes)
+            self.limbs.upload(host)
         total_out = sum(max(rn, 1) * r * d for (r, d, _, rn, _) in plan_in)
         self.out = DeviceBuffer(8 * max(total_out, 1))
         self.total_out = total_out
         base, obase = self.limbs.ptr, self.out.ptr
         self._keep = []
-        prods = (DevProduct * len(products))()
+        prods = (DevProduct * len(plan_in))()
         opos = 0
         self.layout = []
         for pi, (r, d, ents, rn, left) in enumerate(plan_in):
@@ -178,6 +198,28 @@ class DeviceBatch:
         self.prods = prods
 
         self.handle = None
+
+    @classmethod
+    def synthetic(cls, specs):
+        """Products whose entries are the synthetic stream of rf_util.h,
+        generated in device memory: specs = [(r, d, bits, signs, seed)], bits
+        and signs per entry (A's r*d, then B's d*d)."""
+        return cls(None, synthetic=specs)
+
+    def checksums(self):
+        """Per product, the checksum of its result (rf_util.h), computed on
+        the device."""
+        from .serve import checksums_device
+        device_sync()
+        ptrs, ns, ents, slots = [], [], [], []
+        for pi, cpos in enumerate(self.layout):
+            for e, (o, rn) in enumerate(cpos):
+                if rn:
+                    ptrs.append(self.out.ptr + 8 * o)
+                    ns.append(rn)
+                    ents.append(e)
+                    slots.append(pi)
+        return checksums_device(ptrs, ns, ents, slots, len(self.layout))
 
     def run(self, sync=True):
         """One-shot: plans, builds task tables and runs (rf_batch_mult_device)."""

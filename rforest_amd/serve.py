@@ -71,6 +71,12 @@ def _bind():
     L.rf_checksum_device.restype = ctypes.c_int
     L.rf_checksum_device.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
                                      ctypes.POINTER(ctypes.c_uint64)]
+    L.rf_gen_entries_device.restype = ctypes.c_int
+    L.rf_gen_entries_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int]
+    L.rf_checksums_device.restype = ctypes.c_int
+    L.rf_checksums_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
     L._serve_bound = True
     return L
 
@@ -280,3 +286,23 @@ def checksum_device(ptr, n, entry):
     s = ctypes.c_uint64()
     check(_bind().rf_checksum_device(ctypes.c_void_p(ptr), n, entry, ctypes.byref(s)))
     return s.value
+
+
+def gen_entries_device(ptrs, bits, seeds, entries):
+    """rf_gen_entries_device over parallel lists."""
+    n = len(ptrs)
+    if n == 0:
+        return
+    a = lambda v, t: np.ascontiguousarray(np.array(v, dtype=t))
+    P, B, S, E = a(ptrs, np.uint64), a(bits, np.uint64), a([x & _M64 for x in seeds], np.uint64), a(entries, np.uint32)
+    check(_bind().rf_gen_entries_device(P.ctypes.data, B.ctypes.data, S.ctypes.data, E.ctypes.data, n))
+
+
+def checksums_device(ptrs, nlimbs, entries, slots, nslots):
+    """rf_checksums_device over parallel lists; returns nslots sums."""
+    a = lambda v, t: np.ascontiguousarray(np.array(v, dtype=t))
+    P, N, E, S = a(ptrs, np.uint64), a(nlimbs, np.uint64), a(entries, np.uint32), a(slots, np.uint32)
+    out = np.zeros(max(nslots, 1), dtype=np.uint64)
+    check(_bind().rf_checksums_device(P.ctypes.data, N.ctypes.data, E.ctypes.data, S.ctypes.data, len(ptrs),
+                                      out.ctypes.data, max(nslots, 1)))
+    return [int(x) for x in out[:nslots]]

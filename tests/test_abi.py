@@ -83,11 +83,41 @@ def test_planner_bound_is_proven_and_minimal(la, lb, r, d):
 
 def test_planner_headline_plan():
     # 2x2 of 1,048,576-bit entries: 256-bit chunks, 4096 chunks, 2^13 points
+    from rforest_amd import tune
+    tune.clear()
     assert rf.plan(1 << 20, 1 << 20, 2, 2) == (4, 13, 11)
+
+
+def test_tuning_rules_steer_the_planner(tmp_path):
+    from rforest_amd import tune
+    tune.clear()
+    base = rf.plan(3000, 3000, 2, 2)
+    tune.force(2, 8)
+    t, lgN, k = rf.plan(3000, 3000, 2, 2)
+    assert t == 8 and _bound_ok(t, lgN, k, 3000, 3000, 2)
+    assert rf.plan(3000, 3000, 3, 3) == rf.plan(3000, 3000, 3, 3)
+    tune.clear()
+    assert rf.plan(3000, 3000, 2, 2) == base
+    # the cheapest measured valid plan wins; unmeasured plans do not compete
+    tune.set_rule(2, 13, 4, 0, 5.0)   # 2^20-bit entries at 256-bit chunks
+    tune.set_rule(2, 14, 3, 0, 4.0)   # ... at 192-bit chunks
+    assert rf.plan(1 << 20, 1 << 20, 2, 2)[:2] == (3, 14)
+    tune.set_rule(2, 14, 3, 0, 6.0)
+    assert rf.plan(1 << 20, 1 << 20, 2, 2)[:2] == (4, 13)
+    tune.clear()
+    f = tmp_path / "rules.txt"
+    f.write_text("# comment\n2 12 2 0 1.5\n2 12 3 0 9.0\nbad line\n")
+    assert tune.load(str(f)) == 2
+    assert rf.plan(3000 * 64, 3000 * 64, 2, 2)[0] in (2, 3)
+    tune.clear()
+    with pytest.raises(rf.RfError):
+        tune.set_rule(2, 11, 9, 0, 1.0)
 
 
 def test_long_transform_plans():
     # entries beyond 4 Mbit need more than 2^14 points: column + row passes
+    from rforest_amd import tune
+    tune.clear()
     for la, r in [(1 << 24, 2), (1 << 24, 4), (5 << 20, 2)]:
         t, lgN, k = rf.plan(la, la, r, r)
         assert 14 < lgN <= 17 and _bound_ok(t, lgN, k, la, la, r)

@@ -194,3 +194,53 @@ def test_prepared_batch_lanes_match_one_lane(lanes):
         rf.device_sync()
         assert batch.results() == want
     batch.release()
+
+
+@pytest.mark.parametrize("t,split,bits", [(1, 0, 9000), (8, 0, 9000), (3, 0, 70000), (1, 1, 400000), (5, 1, 2500000), (2, 0, 1 << 20)])
+def test_tuned_plans_give_the_same_products(t, split, bits):
+    """Planner rules force other chunk widths (and the two-row split of
+    2^14-point transforms); the products stay exact."""
+    from rforest_amd import tune
+    rng = random.Random(t * 100 + bits)
+    A, B = _rand_product(rng, bits, 2, 2)
+    tune.clear()
+    tune.force(2, t, split)
+    try:
+        assert rf.plan(max(abs(x).bit_length() for x in A), max(abs(x).bit_length() for x in B), 2, 2)[0] == t
+        assert rf.mpz_rmatrix_mult_fft(A, 2, B, 2) == naive(A, 2, B, 2)
+        batch = rf.DeviceBatch([(2, 2, A, B)])
+        batch.prepare()
+        batch.launch(graph=True)
+        rf.device_sync()
+        assert batch.results()[0] == naive(A, 2, B, 2)
+        batch.release()
+    finally:
+        tune.reset()
+
+
+@pytest.mark.parametrize("bits,r", [(5000, 2), (300000, 3), (3000000, 2), (6 << 20, 2)])
+def test_every_chunk_width_gives_the_same_products(bits, r):
+    """Each valid chunk width (forced through a planner rule) packs, transforms
+    and recomposes to the schoolbook product."""
+    from rforest_amd import tune
+    rng = random.Random(bits)
+    A, B = _rand_product(rng, bits, r, r)
+    want = naive(A, r, B, r)
+    seen = 0
+    try:
+        for t in range(1, 9):
+            tune.clear()
+            tune.force(r, t)
+            try:
+                pt = rf.plan(max(abs(x).bit_length() for x in A), max(abs(x).bit_length() for x in B), r, r)[0]
+            except rf.RfError:
+                continue
+            if pt != t:
+                continue
+            seen += 1
+            batch = rf.DeviceBatch([(r, r, A, B)])
+            batch.run()
+            assert batch.results()[0] == want, t
+    finally:
+        tune.reset()
+    assert seen >= 3

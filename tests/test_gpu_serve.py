@@ -136,3 +136,27 @@ def test_server_unsupported_request_is_refused():
         srv.submit_synth(2, 2, [1 << 27] * 8, [1] * 8, 1)
     assert e.value.code in (rf.RF_ERR_UNSUPPORTED, rf.RF_ERR_CAPACITY)
     srv.close()
+
+
+def test_synthetic_device_batch_checksums_match_host():
+    """Device batches of synthetic entries (generated in place) and their
+    device checksums agree with GMP products of the host-generated entries."""
+    rng = random.Random(21)
+    specs = []
+    for i in range(12):
+        r = rng.choice([1, 2, 3])
+        bits = [rng.randrange(1, 30000) for _ in range(2 * r * r)]
+        signs = [rng.choice([-1, 0, 1]) for _ in range(2 * r * r)]
+        specs.append((r, r, bits, signs, rng.getrandbits(64)))
+    batch = rf.DeviceBatch.synthetic(specs)
+    batch.run()
+    sums = batch.checksums()
+    res = batch.results()
+    for (r, _, bits, signs, seed), h, C in zip(specs, sums, res):
+        vals = [serve.gen_value(b, s, seed, e) for e, (b, s) in enumerate(zip(bits, signs))]
+        want = naive(vals[:r * r], r, vals[r * r:], r)
+        assert C == want
+        la = max([(b + 63) // 64 for b, s in zip(bits[:r * r], signs[:r * r]) if s] or [0])
+        lb = max([(b + 63) // 64 for b, s in zip(bits[r * r:], signs[r * r:]) if s] or [0])
+        rn = int(rf.lib().rf_out_limbs(la, lb, r))
+        assert h == serve.checksum_values(want, rn)

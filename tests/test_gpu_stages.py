@@ -38,7 +38,7 @@ def test_fwd_stage(lgN):
     rng = random.Random(lgN)
     P = rf.primes(32)
     N = 1 << lgN
-    for t in (1, 3, 4, 8):
+    for t in ((1, 3, 4, 8) if lgN < 11 else (1, 2, 3, 4, 5, 6, 7, 8)):
         nchunk = max(1, N // 2)
         ents = []
         for sign in (1, -1, 0):
@@ -71,7 +71,8 @@ def test_fwd_stage_long(lgN):
     rng = random.Random(lgN)
     P = rf.primes(32)
     N = 1 << lgN
-    for t, nchunk, scale in ((1, N // 2, 0), (8, N // 2, lgN), (2, N, 0), (3, 1000, lgN)):
+    for t, nchunk, scale in ((1, N // 2, 0), (8, N // 2, lgN), (2, N, 0), (3, 1000, lgN), (5, N // 2, 0),
+                             (7, N // 4, lgN), (6, N // 2 - 3, 0)):
         ents = [([rng.getrandbits(64) for _ in range(nchunk * t - 1)], 1),
                 ([rng.getrandbits(64) for _ in range(nchunk * t)], -1)]
         for q in (0, 23):
