@@ -86,6 +86,47 @@ def cpu_baseline(products, gpu_results, budget_s):
             "gpu_results_identical": mismatches == 0}
 
 
+def _cpu_fn():
+    import oracle
+    if oracle.ref_available():
+        return oracle.ref_rmatrix_mult
+    def fn(A, r, B, d):
+        pl, _ = oracle.ref_plan(max((abs(x).bit_length() + 63) // 64 for x in A + B), d)
+        return oracle.rmatrix_mult(A, r, B, d, pl)
+    return fn
+
+
+def cpu_worker(seconds, batch):
+    """One CPU worker process of the all-core baseline: the same workload,
+    products in turn for `seconds`; prints its count as JSON."""
+    from rforest_amd import workloads
+    products = workloads.headline(batch, seed=42)
+    fn = _cpu_fn()
+    n, t0 = 0, time.perf_counter()
+    while n < 1 or time.perf_counter() - t0 < seconds:
+        r, d, A, B = products[n % len(products)]
+        fn(A, r, B, d)
+        n += 1
+    print(json.dumps({"n": n, "s": time.perf_counter() - t0}), flush=True)
+
+
+def cpu_all_cores(seconds, batch):
+    """All-core CPU throughput: one worker process per core (at most 16),
+    started before this process touches the GPU."""
+    import subprocess
+    workers = max(1, min(16, os.cpu_count() or 1))
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(seconds),
+                               "--batch", str(batch)], stdout=subprocess.PIPE, text=True)
+             for _ in range(workers)]
+    n, wall = 0, 0.0
+    for p in procs:
+        out, _ = p.communicate()
+        rec = json.loads(out.strip().splitlines()[-1])
+        n += rec["n"]
+        wall = max(wall, rec["s"])
+    return {"workers": workers, "products_per_s": round(n / wall, 2), "products": n, "seconds": round(wall, 2)}
+
+
 def tree_level_section(rf, workloads, reps):
     """1024 independent 2x2 . 2x2 products of uneven sizes (entries
     log-uniform in [2^10, 2^18) bits, each matrix within 2x, random signs):
@@ -287,18 +328,26 @@ def main():
     ap.add_argument("--lanes", type=int, default=1, help="overlapping pipelines per prepared batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-worker", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host mpz drop-in (PCIe-inclusive, one product per call)")
     ap.add_argument("--requests", type=int, default=20000, help="requests in the served stream")
     ap.add_argument("--no-extra", action="store_true",
                     help="headline only (skip the tree-level and resident-operand sections)")
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker, args.batch)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the product pipeline runs on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md and not measured (--gpus 1 only)",
+              "sharding over several GPUs is a design, not a measured path (--gpus 1 only)",
               file=sys.stderr)
         sys.exit(2)
+    all_cores = None
+    if not args.no_cpu:
+        log("timing the CPU baseline on all cores (before the GPU is used)")
+        all_cores = cpu_all_cores(args.cpu_seconds, args.batch)
 
     import rforest_amd as rf
     from rforest_amd import workloads
@@ -393,6 +442,7 @@ def main():
     if not args.no_cpu:
         log("timing the CPU baseline")
         cpu = cpu_baseline(products, results, args.cpu_seconds)
+        cpu["all_cores"] = all_cores
 
     out = {
         "metric": "r=2 1-Mbit matmuls/sec; mixed-stream p50/p99 latency, batched vs plain",
