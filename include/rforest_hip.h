@@ -154,6 +154,22 @@ int rf_operand_plan(const rf_operand* op, int* t, int* lgN, int* k);
 void rf_operand_destroy(rf_operand* op);
 
 /* ---------------------------------------------------------------------------
+ * Product trees kept in device memory (no reference counterpart: rforest's
+ * trees call mpz_rmatrix_mult_fft level by level through mpz_t).  M_0 M_1 ...
+ * M_{n-1} of n square d x d matrices, level by level: each level's pairs run
+ * as one batched pipeline and its results are converted in place from two's
+ * complement to sign and magnitude, which is the next level's input format,
+ * so only the entries' signs and lengths visit the host between levels.
+ * ------------------------------------------------------------------------- */
+/* mats: n*d*d device entries (matrix after matrix, row-major).  The root's
+   magnitudes go to out + e*out_cap (device), its signs and limb counts to
+   sign[e], nlimbs[e] (host); RF_ERR_CAPACITY if an entry exceeds out_cap. */
+int rf_tree_product_device(const rf_dev_entry* mats, int n, int d, uint64_t* out, uint64_t out_cap, int* sign,
+                           uint64_t* nlimbs);
+/* C (d x d) = mats[0..d*d) * mats[d*d..2*d*d) * ... (n matrices of mpz_t). */
+int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d);
+
+/* ---------------------------------------------------------------------------
  * Request server (no reference counterpart; the reference serves one
  * synchronous call at a time).  Continuous batching: a worker thread admits
  * queued requests in arrival order whenever a stream slot is free and the

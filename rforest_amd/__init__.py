@@ -227,6 +227,24 @@ def mpz_rmatrix_mult_naive(A, r, B, d):
     return rmatrix_mult_schoolbook(A, r, B, d)
 
 
+def tree_product(mats, d):
+    """M_0 M_1 ... M_{n-1} of d x d matrices (lists of d*d Python ints,
+    row-major) as a product tree kept in device memory
+    (rf_mpz_tree_product)."""
+    from .gmp import MpzArray
+    flat = [x for M in mats for x in M]
+    mA, mC = MpzArray(flat), MpzArray.zeros(d * d)
+    try:
+        L = lib()
+        L.rf_mpz_tree_product.restype = ctypes.c_int
+        L.rf_mpz_tree_product.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        check(L.rf_mpz_tree_product(mC.ptr(), mA.ptr(), len(mats), d))
+        return mC.values()
+    finally:
+        mA.close()
+        mC.close()
+
+
 def plan(la_bits, lb_bits, r, d):
     """(t, lgN, k): chunk width 64 t bits, transform 2^lgN, k primes."""
     t, lg, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

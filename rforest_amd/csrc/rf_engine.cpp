@@ -26,6 +26,7 @@
 #include "rf_internal.h"
 #include "rf_kernels.h"
 #include "rf_plan.h"
+#include "rf_util.h"
 
 extern "C" {
 int hw_disable_fft = 0;
@@ -1109,6 +1110,193 @@ void rf_operand_destroy(rf_operand* op) {
   if (g.ready) (void)hipDeviceSynchronize();
   if (op->spec) (void)hipFree(op->spec);
   delete op;
+}
+
+}  // extern "C"
+
+namespace {
+// A d x d matrix of sign/magnitude entries in device memory.
+using Mat = std::vector<Entry>;
+
+// One level of a product tree: pairs (M_2i, M_2i+1) multiplied in one
+// pipeline, outputs converted in place to sign and magnitude; an odd last
+// matrix is carried up.  The level's outputs live in *buf (allocated here).
+int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next, uint64_t** buf) {
+  const int np = (int)cur.size() / 2;
+  std::vector<Product> ps(np);
+  size_t total = 0;
+  std::vector<size_t> base(np);
+  for (int i = 0; i < np; i++) {
+    Product& P = ps[i];
+    P.r = P.d = d;
+    P.A = cur[2 * i];
+    P.B = cur[2 * i + 1];
+    int rc = plan_product(P);
+    if (rc) return rc;
+    base[i] = total;
+    total += (size_t)d * d * (P.plan.k ? P.plan.rn : 0);
+  }
+  *buf = nullptr;
+  if (total) HIPCHK(hipMalloc((void**)buf, total * 8));
+  for (int i = 0; i < np; i++)
+    for (int e = 0; e < d * d; e++)
+      ps[i].out.push_back(ps[i].plan.k ? *buf + base[i] + (size_t)e * ps[i].plan.rn : nullptr);
+  const Layout L = layout_of(ps);
+  const size_t conv_b = align256((size_t)np * d * d * sizeof(RfConvTask));
+  const size_t meta_b = align256((size_t)np * d * d * sizeof(RfEntryMeta));
+  int rc = ensure_arena(L.total() + conv_b + meta_b);
+  if (rc) return rc;
+  rc = run_pipeline(ps, L, conv_b + meta_b, g.stream);
+  if (rc) return rc;
+  // run_pipeline synchronised the stream before staging its tables, and the
+  // staging buffer is free again once the stream drains
+  std::vector<RfConvTask> conv;
+  for (int i = 0; i < np; i++)
+    for (int e = 0; e < d * d; e++) conv.push_back(RfConvTask{ps[i].out[e], ps[i].plan.k ? ps[i].plan.rn : 0});
+  HIPCHK(hipStreamSynchronize(g.stream));
+  rc = ensure_stage(conv_b + meta_b);
+  if (rc) return rc;
+  RfConvTask* d_conv = (RfConvTask*)g.d_arena;
+  RfEntryMeta* d_meta = (RfEntryMeta*)(g.d_arena + conv_b);
+  memcpy(g.h_stage, conv.data(), conv.size() * sizeof(RfConvTask));
+  HIPCHK(hipMemcpyAsync(d_conv, g.h_stage, conv.size() * sizeof(RfConvTask), hipMemcpyHostToDevice, g.stream));
+  HIPCHK(rf_launch_signmag(d_conv, (int)conv.size(), d_meta, g.stream));
+  HIPCHK(hipMemcpyAsync(g.h_stage + conv_b, d_meta, conv.size() * sizeof(RfEntryMeta), hipMemcpyDeviceToHost,
+                        g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (g.prof) prof_collect();
+  const RfEntryMeta* meta = (const RfEntryMeta*)(g.h_stage + conv_b);
+  next.clear();
+  for (int i = 0; i < np; i++) {
+    Mat M(d * d);
+    for (int e = 0; e < d * d; e++) {
+      const RfEntryMeta& m = meta[i * d * d + e];
+      M[e].dptr = ps[i].out[e];
+      M[e].nlimbs = m.nlimbs;
+      M[e].sign = m.sign;
+      M[e].bits = m.sign ? 64 * (uint64_t)m.nlimbs : 0;
+    }
+    next.push_back(M);
+  }
+  if (cur.size() % 2) next.push_back(cur.back());
+  return RF_OK;
+}
+
+// The whole tree; the root's entries (sign/magnitude, device) are copied to
+// out + e * out_cap, their signs and lengths to sign[e], nlimbs[e].
+int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, int* sign, uint64_t* nlimbs) {
+  std::vector<uint64_t*> bufs;
+  int rc = RF_OK;
+  while (cur.size() > 1 && !rc) {
+    std::vector<Mat> next;
+    uint64_t* buf = nullptr;
+    rc = tree_level(cur, d, next, &buf);
+    if (buf) bufs.push_back(buf);  // a carried matrix may point into any earlier level
+    if (!rc) cur.swap(next);
+  }
+  for (int e = 0; e < d * d && !rc; e++) {
+    const Entry& E = cur[0][e];
+    if (E.sign && E.nlimbs > out_cap) rc = fail(RF_ERR_CAPACITY, "root entry %d needs %llu limbs", e,
+                                                (unsigned long long)E.nlimbs);
+    sign[e] = E.sign;
+    nlimbs[e] = E.sign ? E.nlimbs : 0;
+    if (!rc && E.sign && E.nlimbs) {
+      hipError_t x = hipMemcpyAsync(out + (size_t)e * out_cap, E.dptr, E.nlimbs * 8, hipMemcpyDeviceToDevice, g.stream);
+      if (x != hipSuccess) rc = fail(RF_ERR_HIP, "root copy: %s", hipGetErrorString(x));
+    }
+  }
+  if (!rc) {
+    hipError_t x = hipStreamSynchronize(g.stream);
+    if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree: %s", hipGetErrorString(x));
+  }
+  for (uint64_t* b : bufs) (void)hipFree(b);
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int rf_tree_product_device(const rf_dev_entry* mats, int n, int d, uint64_t* out, uint64_t out_cap, int* sign,
+                           uint64_t* nlimbs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (n <= 0 || d <= 0 || !mats || !out || !sign || !nlimbs) return fail(RF_ERR_ARG, "rf_tree_product_device: bad arguments");
+  std::vector<Mat> cur(n, Mat(d * d));
+  for (int m = 0; m < n; m++)
+    for (int e = 0; e < d * d; e++) {
+      const rf_dev_entry& x = mats[(size_t)m * d * d + e];
+      Entry& E = cur[m][e];
+      E.dptr = x.limbs;
+      E.nlimbs = x.nlimbs;
+      E.sign = x.nlimbs ? (x.sign > 0 ? 1 : (x.sign < 0 ? -1 : 0)) : 0;
+      E.bits = E.sign ? 64 * x.nlimbs : 0;
+    }
+  return tree_product(cur, d, out, out_cap, sign, nlimbs);
+}
+
+int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  if (n <= 0 || d <= 0 || !mats || !C) return fail(RF_ERR_ARG, "rf_mpz_tree_product: bad arguments");
+  const size_t ne = (size_t)n * d * d;
+  std::vector<uint64_t> off(ne);
+  uint64_t total = 0, bound = 0;
+  std::vector<uint64_t> mx(n, 0);
+  for (size_t i = 0; i < ne; i++) {
+    off[i] = total;
+    const uint64_t nl = mpz_size(mats[i]);
+    total += (nl + 1) & ~(uint64_t)1;
+    mx[i / (d * d)] = std::max<uint64_t>(mx[i / (d * d)], nl);
+  }
+  // a node's entries have at most bits(left) + bits(right) + ceil(log2 d)
+  // bits, so the root's have at most 64 sum(leaf limbs) + (n - 1) ceil(log2 d)
+  int cl = 0;
+  while ((1 << cl) < d) cl++;
+  for (int m = 0; m < n; m++) bound += mx[m];
+  bound += ((uint64_t)(n - 1) * cl + 63) / 64 + 1;
+  uint64_t *d_in = nullptr, *d_out = nullptr;
+  HIPCHK(hipMalloc((void**)&d_in, std::max<uint64_t>(total, 1) * 8));
+  hipError_t x = hipMalloc((void**)&d_out, (size_t)d * d * bound * 8);
+  if (x != hipSuccess) {
+    (void)hipFree(d_in);
+    return fail(RF_ERR_NOMEM, "tree output: %s", hipGetErrorString(x));
+  }
+  int rc = ensure_stage(std::max<uint64_t>(total * 8, (size_t)d * d * bound * 8));
+  std::vector<Mat> cur(n, Mat(d * d));
+  if (!rc) {
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (size_t i = 0; i < ne; i++) {
+      const uint64_t nl = mpz_size(mats[i]);
+      if (nl) memcpy((uint64_t*)g.h_stage + off[i], mats[i]->_mp_d, nl * 8);
+      Entry& E = cur[i / (d * d)][i % (d * d)];
+      E.dptr = d_in + off[i];
+      E.nlimbs = nl;
+      E.sign = mpz_sgn(mats[i]);
+      E.bits = E.sign ? mpz_sizeinbase(mats[i], 2) : 0;
+    }
+    x = hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream);
+    if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree upload: %s", hipGetErrorString(x));
+  }
+  std::vector<int> sg(d * d);
+  std::vector<uint64_t> nls(d * d);
+  if (!rc) rc = tree_product(cur, d, d_out, bound, sg.data(), nls.data());
+  if (!rc) {
+    x = hipMemcpy(g.h_stage, d_out, (size_t)d * d * bound * 8, hipMemcpyDeviceToHost);
+    if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree download: %s", hipGetErrorString(x));
+  }
+  if (!rc)
+    for (int e = 0; e < d * d; e++) {
+      if (!sg[e]) {
+        mpz_set_ui(C[e], 0);
+        continue;
+      }
+      _mpz_realloc(C[e], (mp_size_t)nls[e]);
+      memcpy(C[e]->_mp_d, (uint64_t*)g.h_stage + (size_t)e * bound, nls[e] * 8);
+      C[e]->_mp_size = sg[e] < 0 ? -(int)nls[e] : (int)nls[e];
+    }
+  (void)hipFree(d_in);
+  (void)hipFree(d_out);
+  return rc;
 }
 
 int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k) {

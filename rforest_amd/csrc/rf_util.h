@@ -48,6 +48,18 @@ struct RfBlock {
   uint64_t first;  // first limb of the block
 };
 
+// Two's complement -> sign and magnitude, in place, one entry per task; the
+// entry's sign and magnitude length (leading zero limbs dropped) go to meta.
+struct RfConvTask {
+  uint64_t* limbs;
+  uint64_t n;  // limbs
+};
+struct RfEntryMeta {
+  int32_t sign;
+  uint32_t nlimbs;
+};
+hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, hipStream_t st);
+
 hipError_t rf_launch_gen(const RfSpan* spans, const RfBlock* blocks, int nblocks, hipStream_t st);
 hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nblocks, uint64_t* sums,
                               hipStream_t st);

@@ -5,8 +5,10 @@
 //   k_checksum  64-bit checksums of result limbs, summed per request
 //   k_fill      writes a constant over a u64 range (clears checksum slots
 //               without a memset node, safe under graph capture)
+//   k_signmag   two's complement results -> sign and magnitude in place (a
+//               product tree's level outputs become the next level's inputs)
 //
-// All three are flat, coalesced streams over lists of spans; a span is one
+// The first three are flat, coalesced streams over lists of spans; a span is one
 // entry (k_gen) or one C entry (k_checksum), split by the host into blocks
 // of RF_UTIL_LIMBS limbs.
 #include "rf_util.h"
@@ -61,6 +63,53 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_checksum(const RfSpan* __re
   }
 }
 
+// One workgroup per entry: sign from the top bit; a negative entry becomes
+// its magnitude ~x + 1, which is 0 below the lowest nonzero limb l0, -x[l0]
+// at l0 and ~x above, so every limb is independent once l0 is known; then the
+// highest nonzero limb gives the length.
+__global__ __launch_bounds__(RF_UTIL_THREADS) void k_signmag(const RfConvTask* __restrict__ tasks, int ntasks,
+                                                            RfEntryMeta* __restrict__ meta) {
+  __shared__ unsigned long long lo_sh;
+  __shared__ long long hi_sh;
+  if ((int)blockIdx.x >= ntasks) return;
+  const RfConvTask tk = tasks[blockIdx.x];
+  const uint64_t n = tk.n;
+  if (threadIdx.x == 0) {
+    lo_sh = n;
+    hi_sh = -1;
+  }
+  __syncthreads();
+  const bool neg = n > 0 && (tk.limbs[n - 1] >> 63) != 0;
+  if (neg) {
+    unsigned long long lo = n;
+    for (uint64_t i = threadIdx.x; i < n; i += RF_UTIL_THREADS)
+      if (tk.limbs[i] != 0) {
+        lo = i;
+        break;
+      }
+    atomicMin(&lo_sh, lo);
+    __syncthreads();
+    const uint64_t l0 = lo_sh;
+    for (uint64_t i = threadIdx.x; i < n; i += RF_UTIL_THREADS) {
+      const uint64_t x = tk.limbs[i];
+      if (i == l0)
+        tk.limbs[i] = ~x + 1;
+      else if (i > l0)
+        tk.limbs[i] = ~x;
+    }
+    __syncthreads();
+  }
+  long long hi = -1;
+  for (uint64_t i = threadIdx.x; i < n; i += RF_UTIL_THREADS)
+    if (tk.limbs[i] != 0) hi = (long long)i;
+  atomicMax(&hi_sh, hi);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t len = (uint32_t)(hi_sh + 1);
+    meta[blockIdx.x] = RfEntryMeta{len == 0 ? 0 : (neg ? -1 : 1), len};
+  }
+}
+
 __global__ void k_fill(uint64_t* __restrict__ dst, uint64_t n, uint64_t v) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = v;
@@ -85,5 +134,11 @@ hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nb
 hipError_t rf_launch_fill(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, n, v);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_signmag, dim3(ntasks), dim3(RF_UTIL_THREADS), 0, st, tasks, ntasks, meta);
   return hipGetLastError();
 }
