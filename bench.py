@@ -187,6 +187,53 @@ def tree_level_section(rf, workloads, reps):
             "checks": {"mod_2^61-1_mismatches": bad, "exact_sample": exact, "identical_to_cost_model": same_model}}
 
 
+def tree_section(rf, workloads, reps):
+    """A product tree of 1024 2x2 matrices with 4096-bit signed entries (GMP
+    seed 1006) kept in device memory (rf_mpz_tree_product: leaves up, root
+    down), against the same tree level by level through the host mpz
+    drop-in."""
+    import ctypes
+    from rforest_amd.gmp import MpzArray
+    rng = workloads.GmpRandom(1006)
+    mats = [[rng.signed(4096) for _ in range(4)] for _ in range(1024)]
+    rng.close()
+    L = rf.lib()
+    L.rf_mpz_tree_product.restype = ctypes.c_int
+    L.rf_mpz_tree_product.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    leaves = MpzArray([x for M in mats for x in M])
+    root = MpzArray.zeros(4)
+    rf.check(L.rf_mpz_tree_product(root.ptr(), leaves.ptr(), len(mats), 2))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rf.check(L.rf_mpz_tree_product(root.ptr(), leaves.ptr(), len(mats), 2))
+    t_dev = (time.perf_counter() - t0) / reps
+    got = root.values()
+    # level by level through the drop-in
+    w = MpzArray.zeros(1)
+    t0 = time.perf_counter()
+    cur = [MpzArray(M) for M in mats]
+    t_conv = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    while len(cur) > 1:
+        nxt = []
+        for i in range(0, len(cur) - 1, 2):
+            C = MpzArray.zeros(4)
+            L.mpz_rmatrix_mult_fft(C.ptr(), cur[i].ptr(), 2, cur[i + 1].ptr(), 2, w.ptr())
+            nxt.append(C)
+        if len(cur) % 2:
+            nxt.append(cur[-1])
+        cur = nxt
+    t_plain = time.perf_counter() - t0
+    same = cur[0].values() == got
+    root_bits = max(abs(x).bit_length() for x in got)
+    return {"workload": "product tree of 1024 2x2 matrices, 4096-bit signed entries, GMP seed 1006 "
+                        f"(1023 products; root entries of {root_bits} bits)",
+            "device_tree_s": round(t_dev, 5),
+            "dropin_levels_s": round(t_plain, 5),
+            "speedup": round(t_plain / t_dev, 2),
+            "checks": {"identical_to_dropin": same, "mpz_conversion_excluded_s": round(t_conv, 4)}}
+
+
 def resident_section(rf, workloads, reps):
     """One 3x3 left matrix of 4,194,304-bit entries times 16 right 3x3
     matrices of 1,048,576-bit entries: the left operand transformed once and
@@ -434,6 +481,8 @@ def main():
         extra["tree_level"] = tree_level_section(rf, workloads, max(3, args.steps // 4))
         log("resident operand")
         extra["resident_operand"] = resident_section(rf, workloads, max(3, args.steps // 4))
+        log("product tree in device memory")
+        extra["product_tree"] = tree_section(rf, workloads, 3)
         log("serving: mixed request stream")
         extra["serving"] = serving_section(rf, workloads, args.requests, min(2000, args.requests), 50,
                                            min(5000, args.requests))
