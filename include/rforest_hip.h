@@ -154,6 +154,57 @@ int rf_operand_plan(const rf_operand* op, int* t, int* lgN, int* k);
 void rf_operand_destroy(rf_operand* op);
 
 /* ---------------------------------------------------------------------------
+ * Integers held in the transform domain (general transform API).  Mirrors
+ * the reference's mpzfft interface (src/include/mpzfft.h:13-120,
+ * src/lib/mpzfft.c:10-55): a params object fixes, for an output bound `bits`
+ * and `terms` products, the chunk width, transform length and primes
+ * (mpzfft_params_init, src/lib/mpnfft.c:18-78: any sum of at most `terms`
+ * products x*y with |x1*y1| + ... < 2^bits comes back exactly); an rf_fft
+ * value (mpzfft_t) is one integer's spectra, k primes x 2^lgN residues, in
+ * device memory.  Values combine by set/add/sub/neg, pointwise products and
+ * matrix products without leaving the device, and rf_fft_inverse turns one
+ * back into an integer.  Unlike the reference, rf_fft_inverse leaves its
+ * operand intact.  Every call is synchronous; results of arithmetic whose
+ * inputs exceed the params bound are unspecified, as in the reference.
+ * Aliasing of outputs and inputs is allowed everywhere.
+ * ------------------------------------------------------------------------- */
+typedef struct rf_fft_params rf_fft_params;
+typedef struct rf_fft rf_fft;
+/* mpzfft_params_init (src/include/mpzfft.h:17-22); RF_ERR_UNSUPPORTED when no
+   plan within 2^17 points and the prime set holds the bound. */
+int rf_fft_params_create(uint64_t bits, unsigned terms, rf_fft_params** out);
+/* The plan: chunk width 64*t bits, transform 2^lgN, k primes. */
+int rf_fft_params_plan(const rf_fft_params* params, int* t, int* lgN, int* k);
+void rf_fft_params_destroy(rf_fft_params* params); /* mpzfft_params_clear */
+/* mpzfft_init (src/include/mpzfft.h:35-38): a value of zero, device memory
+   allocated; it keeps a copy of the plan, so params may be destroyed first. */
+int rf_fft_create(const rf_fft_params* params, rf_fft** out);
+void rf_fft_destroy(rf_fft* f); /* mpzfft_clear */
+/* mpzfft_fft (src/lib/mpzfft.c:10-23): rop := transform of op.  RF_ERR_ARG if
+   op has more bits than the params bound. */
+int rf_fft_forward(rf_fft* rop, const mpz_t op);
+/* The same from device limbs (read during the call only). */
+int rf_fft_forward_device(rf_fft* rop, const rf_dev_entry* op);
+/* mpzfft_ifft (src/lib/mpzfft.c:26-55): rop := the integer op holds. */
+int rf_fft_inverse(mpz_t rop, const rf_fft* op);
+/* The same into device memory: magnitude limbs at out (cap limbs), sign and
+   limb count to *sign, *nlimbs (host); RF_ERR_CAPACITY if cap is too small
+   (rf_fft_out_limbs limbs always suffice). */
+int rf_fft_inverse_device(const rf_fft* op, uint64_t* out, uint64_t cap, int* sign, uint64_t* nlimbs);
+uint64_t rf_fft_out_limbs(const rf_fft* f);
+/* mpzfft_set/add/sub/neg/mul (src/include/mpzfft.h:60-107); all operands
+   must come from the same plan (RF_ERR_ARG otherwise). */
+int rf_fft_set(rf_fft* rop, const rf_fft* op);
+int rf_fft_add(rf_fft* rop, const rf_fft* op1, const rf_fft* op2);
+int rf_fft_sub(rf_fft* rop, const rf_fft* op1, const rf_fft* op2);
+int rf_fft_neg(rf_fft* rop, const rf_fft* op);
+int rf_fft_mul(rf_fft* rop, const rf_fft* op1, const rf_fft* op2);
+/* mpzfft_matrix_mul (src/include/mpzfft.h:109-118): rop (dim1 x dim3) :=
+   op1 (dim1 x dim2) * op2 (dim2 x dim3), row-major arrays of values. */
+int rf_fft_matrix_mul(rf_fft* const* rop, rf_fft* const* op1, rf_fft* const* op2, unsigned dim1, unsigned dim2,
+                      unsigned dim3);
+
+/* ---------------------------------------------------------------------------
  * Product trees kept in device memory (no reference counterpart: rforest's
  * trees call mpz_rmatrix_mult_fft level by level through mpz_t).  M_0 M_1 ...
  * M_{n-1} of n square d x d matrices, level by level: each level's pairs run

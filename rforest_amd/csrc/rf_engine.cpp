@@ -292,6 +292,30 @@ uint32_t crt_blocks(const RfPlan& pl) {
   return (uint32_t)((ndig + D - 1) / D);
 }
 
+void crt_tasks(const RfPlan& pl, uint64_t coef_off, uint64_t* out, uint32_t& npre, uint32_t& nblk_total,
+               std::vector<RfCrtTask>& v) {
+  const uint32_t nblk = crt_blocks(pl);
+  const int halo = rf_crt_halo(pl.k, pl.t);
+  for (uint32_t b = 0; b < nblk; b++) {
+    RfCrtTask t{};
+    t.coef_off = coef_off;
+    t.out = out;
+    t.pre_off = npre;
+    t.fun_idx = nblk_total + b;
+    t.N = (uint32_t)1 << pl.lgN;
+    t.nout = (uint32_t)pl.nout;
+    t.rn = (uint32_t)pl.rn;
+    t.digit0 = b * (uint32_t)(RF_CRT_THREADS - halo);
+    t.t = (uint8_t)pl.t;
+    t.k = (uint8_t)pl.k;
+    t.s = (uint8_t)pl.s;
+    t.halo = (uint8_t)halo;
+    v.push_back(t);
+    npre += RF_CRT_THREADS;
+  }
+  nblk_total += nblk;
+}
+
 Layout layout_of(std::vector<Product>& prods) {
   Layout L;
   // task counts per launch group, as build_pipeline groups them
@@ -425,28 +449,8 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
           }
           if (split) icol[pl.lgN].push_back(RfRowTask{out_off, 0, (uint16_t)q, 0});
         }
-    const uint32_t nblk = crt_blocks(pl);
-    for (int e = 0; e < r * d; e++) {
-      for (uint32_t b = 0; b < nblk; b++) {
-        RfCrtTask t{};
-        t.coef_off = P.coef_off + (uint64_t)e * k * N;
-        t.out = P.out[e];
-        t.pre_off = npre;
-        t.fun_idx = nblk_total + b;
-        t.N = (uint32_t)N;
-        t.nout = (uint32_t)pl.nout;
-        t.rn = (uint32_t)pl.rn;
-        const int halo = rf_crt_halo(k, pl.t);
-        t.digit0 = b * (uint32_t)(RF_CRT_THREADS - halo);
-        t.t = (uint8_t)pl.t;
-        t.k = (uint8_t)k;
-        t.s = (uint8_t)pl.s;
-        t.halo = (uint8_t)halo;
-        crt[k].push_back(t);
-        npre += RF_CRT_THREADS;
-      }
-      nblk_total += nblk;
-    }
+    for (int e = 0; e < r * d; e++)
+      crt_tasks(pl, P.coef_off + (uint64_t)e * k * N, P.out[e], npre, nblk_total, crt[k]);
   }
   // Stream order: every forward pass (column passes before the row pass),
   // then every inverse pass (row passes before the column passes), CRT,
@@ -652,6 +656,21 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* le
     memcpy(C[e]->_mp_d, out[e].data(), n * 8);
     C[e]->_mp_size = neg ? -(int)n : (int)n;  // src/lib/hwmpz.c:116-122
   }
+  return RF_OK;
+}
+
+int engine_host_init() { return host_init_locked(); }
+hipStream_t engine_stream() { return g.stream; }
+int engine_arena(size_t bytes, char** base) {
+  int rc = ensure_arena(bytes);
+  if (rc) return rc;
+  *base = (char*)g.d_arena;
+  return RF_OK;
+}
+int engine_stage(size_t bytes, char** host) {
+  int rc = ensure_stage(bytes);
+  if (rc) return rc;
+  *host = (char*)g.h_stage;
   return RF_OK;
 }
 

@@ -29,6 +29,15 @@ RF_DEV double rf_reduce(double x, double p, double pinv) {
   return __builtin_fma(-q, p, x);
 }
 
+// x mod p for an integer x held exactly, |x| < 2^90 (sums of constants
+// times words whose low bits are zero can exceed 2^53 and stay exact).
+// |fl(x*pinv) - x/p| <= |x/p| 2^-52 < 2^-9 since |x/p| < 2^43, so
+// |x - q p| <= p (1/2 + 2^-9) < 2^48: an integer, which the FMA forms exactly.
+RF_DEV double rf_reduce_wide(double x, double p, double pinv) {
+  double q = __builtin_rint(x * pinv);
+  return __builtin_fma(-q, p, x);
+}
+
 // a * w mod p.  h + l == a*w exactly (FMA error-free product); with
 // q = rint(fl(a * wq)), |a w / p - q| <= 1/2 + |a| 2^-52, so r = a w - q p
 // satisfies |r| <= p (1/2 + |a| 2^-52).  fma(-q, p, h) = r - l exactly because

@@ -89,6 +89,14 @@ std::mutex& engine_mutex();
 bool engine_ready();
 const RfDevTables& engine_tables();
 const RfPrimes& engine_primes();
+// Host-side set-up only (primes; no device): lets planners run without a GPU.
+int engine_host_init();
+// The engine's stream, and its device arena and pinned staging buffer grown
+// to at least `bytes` (both reused by every call made under the lock; a
+// grown buffer replaces the old one only after the stream is idle).
+hipStream_t engine_stream();
+int engine_arena(size_t bytes, char** base);
+int engine_stage(size_t bytes, char** host);
 
 int plan_product(Product& P);
 Layout layout_of(std::vector<Product>& prods);
@@ -98,6 +106,14 @@ ScratchPtrs scratch_ptrs(char* base, const Layout& L);
 // the scratch region at `base`.  No host synchronisation, no allocation.
 int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof);
 uint64_t bitlen_limbs(const uint64_t* x, uint64_t n);
+// CRT blocks of one C entry, and its CRT tasks (coefficient residues at
+// coef_off doubles, k primes x N, into out): pre bytes and aggregate slots
+// are numbered on from npre and nblk_total.
+uint32_t crt_blocks(const RfPlan& pl);
+void crt_tasks(const RfPlan& pl, uint64_t coef_off, uint64_t* out, uint32_t& npre, uint32_t& nblk_total,
+               std::vector<RfCrtTask>& v);
+// Two-row split of a plan's transforms (column + row passes).
+bool plan_split(const RfPlan& pl);
 void tc_to_signmag(std::vector<uint64_t>& x, bool& neg);
 
 }  // namespace rfi

@@ -34,7 +34,7 @@ struct RfInvTask {
   uint32_t a_stride;  // spec distance between A[i][l] and A[i][l+1]
   uint32_t b_stride;  // spec distance between B[l][j] and B[l+1][j]
   uint16_t q;
-  uint16_t d;
+  uint16_t d;         // terms; 0: inverse of the one spectrum at a_off (b unused)
 };
 
 // Transforms longer than 2^RF_LGN_LDS points, N = 2^(13 + n1), view the
@@ -119,6 +119,19 @@ hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int 
 hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* tasks, int ntasks,
                              double* coef, hipStream_t st);
 // Blocks with the same k must be launched together (the kernel is specialised on k).
+// Element-wise arithmetic on stored spectra of k primes x 2^lgN residues
+// (one value of the general transform API, [prime][N]).  Every stored
+// spectrum holds 2^-lgN times the transform of its integer, so a product
+// multiplies by 2^lgN once more.
+enum RfEwOp { RF_EW_SET, RF_EW_ADD, RF_EW_SUB, RF_EW_NEG, RF_EW_MUL };
+hipError_t rf_launch_ew(const RfDevTables& tb, int op, int k, int lgN, const double* a, const double* b,
+                        double* out, hipStream_t st);
+// out[i][j] = sum_l a[i][l] * b[l][j] per residue, for n1 x n2 times n2 x n3
+// matrices of spectra given as device arrays of pointers (row-major).
+hipError_t rf_launch_ew_matmul(const RfDevTables& tb, int k, int lgN, const double* const* a,
+                               const double* const* b, double* const* out, int n1, int n2, int n3,
+                               hipStream_t st);
+
 hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, int ntasks,
                          const double* coef, uint8_t* pre, uint8_t* fun, hipStream_t st);
 hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre, const uint8_t* fun,

@@ -278,10 +278,11 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const d
 // Residues of NCH chunks ci[c] (t limbs = 2t 32-bit words each) of an entry
 // modulo p, times the scale folded into the pack constants:
 // sum_w word_w K_w with K_w = 2^(32 w) S mod p split in 16-bit pieces
-// K_w = k0 + k1 2^16 + k2 2^32.  Each word * piece < 2^48 and at most 16
-// terms are summed, so the three accumulators stay below 2^52: exact.  Then
-// s0 + s1 2^16 + s2 2^32 mod p by two rf_mulmod (inputs < 2^52, outputs
-// <= 1.5 p) and one reduction of a value below 2^53.  Limb l of all NCH
+// K_w = k0 + k1 2^16 + k2 2^32, the constants holding k0, k1 2^16 and k2 2^32.
+// At most 16 words are summed: s0 < 2^52; s1 is a multiple of 2^16 below
+// 2^68 and s2 a multiple of 2^32 below 2^84, so all three accumulate
+// exactly.  s1 and s2 are reduced on their own (rf_reduce_wide, results
+// <= 0.51 p), and s0 + both < 2^53 once more by rf_reduce.  Limb l of all NCH
 // chunks is loaded before any is used, so NCH loads are in flight.  When
 // every chunk of the wave lies inside the entry the loads need no bounds
 // checks; otherwise loads past the entry's end read limb 0 and are
@@ -289,7 +290,7 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const d
 template <int NCH>
 __device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
                                             const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
-                                            const double* pk2, double p, double pinv,
+                                            double p, double pinv,
                                             double (&out)[NCH]) {
   uint32_t cmax = 0;
 #pragma unroll
@@ -327,7 +328,7 @@ __device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, 
   }
 #pragma unroll
   for (int c = 0; c < NCH; c++) {
-    const double r = rf_mulmod(s1[c], pk2[0], pk2[1], p) + rf_mulmod(s2[c], pk2[2], pk2[3], p) + s0[c];
+    const double r = s0[c] + rf_reduce_wide(s1[c], p, pinv) + rf_reduce_wide(s2[c], p, pinv);
     out[c] = rf_reduce(r, p, pinv);
   }
 }
@@ -338,8 +339,8 @@ __device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, 
 // is T/2 aligned 16-byte loads.  pack_any8: 8-byte loads with the bounds
 // checks of pack_chunks.  Same arithmetic and bounds as pack_chunks.
 template <int T>
-__device__ __forceinline__ double pack_combine(const uint64_t (&x)[T], const double (*pk)[3],
-                                               const double* pk2, double p, double pinv) {
+__device__ __forceinline__ double pack_combine(const uint64_t (&x)[T], const double (*pk)[3], double p,
+                                               double pinv) {
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
   for (int l = 0; l < T; l++) {
@@ -349,7 +350,7 @@ __device__ __forceinline__ double pack_combine(const uint64_t (&x)[T], const dou
     s1 = __builtin_fma(hi, pk[2 * l + 1][1], __builtin_fma(lo, pk[2 * l][1], s1));
     s2 = __builtin_fma(hi, pk[2 * l + 1][2], __builtin_fma(lo, pk[2 * l][2], s2));
   }
-  const double r = rf_mulmod(s1, pk2[0], pk2[1], p) + rf_mulmod(s2, pk2[2], pk2[3], p) + s0;
+  const double r = s0 + rf_reduce_wide(s1, p, pinv) + rf_reduce_wide(s2, p, pinv);
   return rf_reduce(r, p, pinv);
 }
 
@@ -365,7 +366,7 @@ constexpr int pack_sb() {
 
 template <int NCH, int T>
 __device__ __forceinline__ void pack_full16(const uint64_t* __restrict__ limbs, const uint32_t (&ci)[NCH],
-                                            const double (*pk)[3], const double* pk2, double p, double pinv,
+                                            const double (*pk)[3], double p, double pinv,
                                             double (&out)[NCH]) {
   constexpr int SB = pack_sb<NCH, T>();
   static_assert(NCH % SB == 0, "sub-batches must tile the chunks");
@@ -383,13 +384,13 @@ __device__ __forceinline__ void pack_full16(const uint64_t* __restrict__ limbs, 
       }
     }
 #pragma unroll
-    for (int c = 0; c < SB; c++) out[c0 + c] = pack_combine<T>(x[c], pk, pk2, p, pinv);
+    for (int c = 0; c < SB; c++) out[c0 + c] = pack_combine<T>(x[c], pk, p, pinv);
   }
 }
 
 template <int NCH, int T>
 __device__ __forceinline__ void pack_any8(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
-                                          const uint32_t (&ci)[NCH], const double (*pk)[3], const double* pk2,
+                                          const uint32_t (&ci)[NCH], const double (*pk)[3],
                                           double p, double pinv, double (&out)[NCH]) {
   constexpr int SB = pack_sb<NCH, T>();
   static_assert(NCH % SB == 0, "sub-batches must tile the chunks");
@@ -405,7 +406,7 @@ __device__ __forceinline__ void pack_any8(const uint64_t* __restrict__ limbs, ui
         x[c][l] = idx < nlimbs ? y : 0;
       }
 #pragma unroll
-    for (int c = 0; c < SB; c++) out[c0 + c] = pack_combine<T>(x[c], pk, pk2, p, pinv);
+    for (int c = 0; c < SB; c++) out[c0 + c] = pack_combine<T>(x[c], pk, p, pinv);
   }
 }
 
@@ -413,18 +414,18 @@ __device__ __forceinline__ void pack_any8(const uint64_t* __restrict__ limbs, ui
 template <int NCH, int T>
 __device__ __forceinline__ void pack_dispatch(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
                                               const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
-                                              const double* pk2, double p, double pinv, double (&out)[NCH]) {
+                                              double p, double pinv, double (&out)[NCH]) {
   if constexpr (T == 0) {
-    pack_chunks<NCH>(limbs, nlimbs, ci, t, pk, pk2, p, pinv, out);
+    pack_chunks<NCH>(limbs, nlimbs, ci, t, pk, p, pinv, out);
   } else {
     uint32_t cmax = 0;
 #pragma unroll
     for (int c = 0; c < NCH; c++) cmax = cmax > ci[c] ? cmax : ci[c];
     const bool full = __builtin_amdgcn_ballot_w64((uint64_t)(cmax + 1) * T > nlimbs) == 0;
     if (T % 2 == 0 && full && ((uintptr_t)limbs & 15) == 0)
-      pack_full16<NCH, T>(limbs, ci, pk, pk2, p, pinv, out);
+      pack_full16<NCH, T>(limbs, ci, pk, p, pinv, out);
     else
-      pack_any8<NCH, T>(limbs, nlimbs, ci, pk, pk2, p, pinv, out);
+      pack_any8<NCH, T>(limbs, nlimbs, ci, pk, p, pinv, out);
   }
 }
 
@@ -445,7 +446,6 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
   const double(*pk)[3] = C->pk[tk.q][tk.scale];
-  const double* pk2 = C->pk2[tk.q];
   const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
   const double sg = tk.sign < 0 ? -1.0 : 1.0;
   const uint32_t nchunk = tk.sign != 0 ? tk.nchunk : 0u;
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {  // a zero entry reads no limbs at all
-          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {
-          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -505,13 +505,17 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 // tid + T e (coalesced: consecutive lanes, consecutive frequencies).  D > 0
 // fixes the number of terms so that all loads of a batch are issued before
 // any is used (|sum| <= 2.4 p needs no intermediate reduction); D == 0 takes
-// d at run time and reduces every eighth term (|term| <= 0.6 p).
+// d at run time and reduces every eighth term (|term| <= 0.6 p); D == -1
+// (a task with d = 0) copies one stored spectrum, already a product, in.
 template <int LGN, int D>
 __device__ __forceinline__ void pointwise(double* v, int tid, const double* __restrict__ pa,
                                           const double* __restrict__ pb, uint32_t as, uint32_t bs, int d,
                                           double p, double pinv) {
   using S = Shape<LGN>;
-  if constexpr (D > 0) {
+  if constexpr (D < 0) {
+#pragma unroll
+    for (int e = 0; e < S::E; e++) v[e] = pa[tid + S::T * e];
+  } else if constexpr (D > 0) {
     constexpr int EB0 = D == 1 ? 16 : (D == 2 ? 8 : 4);
     constexpr int EB = S::E < EB0 ? S::E : EB0;
 #pragma unroll
@@ -575,7 +579,17 @@ __device__ __forceinline__ void pointwise_grp(double* v, int tid, const double* 
   constexpr int GB = (EB0 >> R0) > 0 ? (EB0 >> R0) : 1;      // groups per batch
 #pragma unroll
   for (int u0 = 0; u0 < NG; u0 += GB) {
-    if constexpr (D > 0) {
+    if constexpr (D < 0) {
+#pragma unroll
+      for (int u = 0; u < GB; u++)
+#pragma unroll
+        for (int k = 0; k < PR; k++) {
+          const double2 a =
+              *reinterpret_cast<const double2*>(pa + (size_t)((tid + S::T * (u0 + u)) << R0) + 2 * k);
+          v[(u0 + u) * GS + 2 * k] = a.x;
+          v[(u0 + u) * GS + 2 * k + 1] = a.y;
+        }
+    } else if constexpr (D > 0) {
       double2 a[D][GB][PR], b[D][GB][PR];
 #pragma unroll
       for (int l = 0; l < D; l++)
@@ -663,6 +677,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
     // the first pass's groups of 2 or 4 consecutive frequencies come
     // straight from global memory as 16-byte pairs
     switch (tk.d) {
+      case 0: pointwise_grp<LGN, -1, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 0, p, pinv); break;
       case 1: pointwise_grp<LGN, 1, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 1, p, pinv); break;
       case 2: pointwise_grp<LGN, 2, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 2, p, pinv); break;
       case 3: pointwise_grp<LGN, 3, R0>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 3, p, pinv); break;
@@ -674,6 +689,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
     // frequencies) into LDS; pass 0 then reads its groups of 16 consecutive
     // frequencies from there.
     switch (tk.d) {
+      case 0: pointwise<LGN, -1>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 0, p, pinv); break;
       case 1: pointwise<LGN, 1>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 1, p, pinv); break;
       case 2: pointwise<LGN, 2>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 2, p, pinv); break;
       case 3: pointwise<LGN, 3>(v, tid, pa, pb, tk.a_stride, tk.b_stride, 3, p, pinv); break;
@@ -724,7 +740,6 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
   const double(*pk)[3] = C->pk[tk.q][tk.scale];
-  const double* pk2 = C->pk2[tk.q];
   const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
   const double sg = tk.sign < 0 ? -1.0 : 1.0;
   const uint32_t nchunk = tk.sign != 0 ? tk.nchunk : 0u;
@@ -738,7 +753,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
 #pragma unroll
     for (int c = 0; c < CB; c++) ci[c] = (uint32_t)(r0 + c) * COLS + (uint32_t)col;
     if (nchunk && !(half && r0 >= ROWS / 2)) {
-      pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, pk2, p, pinv, x);
+      pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
     } else {
 #pragma unroll
       for (int c = 0; c < CB; c++) x[c] = 0.0;

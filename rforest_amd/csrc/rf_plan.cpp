@@ -188,6 +188,48 @@ int rf_plan_choose(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, RfPla
   return 0;
 }
 
+int rf_plan_general(const RfPrimes* pr, size_t bits, unsigned terms, RfPlan* out) {
+  if (bits == 0 || terms == 0) return -1;
+  bool found = false;
+  RfPlan best;
+  memset(&best, 0, sizeof best);
+  for (int t = 1; t <= RF_TMAX; t++) {
+    const size_t b = 64 * (size_t)t;
+    // |x| |y| < 2^bits gives bits(x) + bits(y) <= bits + 1, so the chunk
+    // counts satisfy nx + ny - 1 <= ceil((bits + 1) / b) + 1 = points, and
+    // min(nx, ny) <= ceil(points / 2)
+    const size_t points = (bits + 1 + b - 1) / b + 1;
+    int lgN = ceil_log2(points);
+    if (lgN < RF_LGN_MIN) lgN = RF_LGN_MIN;
+    if (lgN > RF_LGN_MAX) continue;
+    const uint64_t f = 2 * (uint64_t)terms * ((points + 1) / 2);
+    int k = 0;
+    for (int kk = 1; kk <= pr->k; kk++)
+      if (rf_bound_holds(pr, kk, (int)b, f)) {
+        k = kk;
+        break;
+      }
+    if (k == 0) continue;
+    RfPlan pl;
+    memset(&pl, 0, sizeof pl);
+    pl.t = t;
+    pl.lgN = lgN;
+    pl.k = k;
+    pl.s = (48 * k + 1 + 63) / 64;
+    if (pl.s > RF_MAXS) continue;
+    pl.M = (pl.s + t - 1) / t;
+    pl.nA = pl.nB = (int)points;
+    pl.nout = (int)points;
+    pl.rn = (bits + 63) / 64 + 1;
+    // residues moved per transform, the dominant cost of every operation
+    pl.cost = (double)k * (double)((size_t)1 << lgN) * (lgN > RF_LGN_LDS ? 2.0 : 1.0);
+    if (!found || pl.cost < best.cost) best = pl, found = true;
+  }
+  if (!found) return -1;
+  *out = best;
+  return 0;
+}
+
 void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
   memset(c, 0, sizeof *c);
   for (int j = 0; j < pr->k; j++) {
@@ -210,15 +252,10 @@ void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
       for (int w = 0; w < RF_MAXWORDS; w++) {
         uint64_t K = rf_h_mulmod(rf_h_powmod(2, (uint64_t)32 * w, p), S, p);
         c->pk[j][v][w][0] = (double)(K & 0xffff);
-        c->pk[j][v][w][1] = (double)((K >> 16) & 0xffff);
-        c->pk[j][v][w][2] = (double)(K >> 32);
+        c->pk[j][v][w][1] = (double)(K & 0xffff0000u);          // k1 2^16, exact
+        c->pk[j][v][w][2] = (double)(K & 0xffff00000000ull);    // k2 2^32, exact
       }
     }
-    const uint64_t t32 = rf_h_powmod(2, 32, p);
-    c->pk2[j][0] = 65536.0;
-    c->pk2[j][1] = 65536.0 / (double)p;
-    c->pk2[j][2] = (double)t32;
-    c->pk2[j][3] = (double)t32 / (double)p;
   }
   for (int k = 0; k <= pr->k; k++) {
     uint64_t P[RF_MAXS + 1] = {1};

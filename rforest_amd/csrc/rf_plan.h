@@ -75,6 +75,12 @@ int rf_plan_fixed(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t,
 int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int t, int lgN, int kmin,
                     RfPlan* out);
 
+// Plan of the general transform API (values in the transform domain, the
+// reference's mpzfft_params_init, src/lib/mpnfft.c:18-78): any sum of at
+// most `terms` products x y with |x1 y1| + ... < 2^bits is recovered
+// exactly.  nA = nB = nout = the point count, rn = ceil(bits / 64) + 1.
+int rf_plan_general(const RfPrimes* pr, size_t bits, unsigned terms, RfPlan* out);
+
 // ---- constant tables shared by all kernels (one copy in device memory) ----
 struct RfConsts {
   double p[RF_MAXP];
@@ -86,11 +92,11 @@ struct RfConsts {
   uint64_t pu[RF_MAXP];                  // the primes as integers
   uint64_t P[RF_MAXP + 1][RF_MAXS];      // P_k = p_0 ... p_{k-1}
   uint64_t Ph[RF_MAXP + 1][RF_MAXS];     // floor(P_k / 2)
-  // pack: pk[q][v][w] = the three 16-bit pieces (low first) of
-  // 2^(32 w) * S_v mod p_q, where S_0 = 1 and S_v = 2^-v mod p_q for v = lgN
-  // (B operands absorb 1/N); pk2[q] = (2^16, 2^16/p) and (2^32 mod p, ./p).
+  // pack: pk[q][v][w] = the three 16-bit pieces k0, k1, k2 of
+  // K = 2^(32 w) * S_v mod p_q = k0 + k1 2^16 + k2 2^32, held as k0, k1 2^16
+  // and k2 2^32, where S_0 = 1 and S_v = 2^-v mod p_q for v = lgN (B operands
+  // absorb 1/N).
   double pk[RF_MAXP][RF_LGN_MAX + 1][RF_MAXWORDS][3];
-  double pk2[RF_MAXP][4];
 };
 
 // Twiddle tables, per prime, as (w, fl(w/p)) pairs:

@@ -2,6 +2,7 @@
 public headers declare, and its host-only parts (prime set, roots, planner,
 error returns before setup) behave as documented.  No GPU work here."""
 import ctypes
+import math
 import os
 import re
 
@@ -164,3 +165,34 @@ def test_calls_before_setup_return_not_init():
         pytest.skip("library already initialised in this process")
     assert rc == rf.RF_ERR_NOT_INIT
     assert b"hw_mpz_setup" in L.rf_last_error()
+
+
+@pytest.mark.parametrize("bits,terms", [(1, 1), (64, 1), (4096, 3), (1 << 20, 1), (1 << 21, 8), (3 << 22, 2)])
+def test_general_transform_plan_bound_is_proven(bits, terms):
+    """rf_fft_params_create: the plan's points cover any x*y with
+    |x y| < 2^bits, and terms * ceil(points/2) * (2^b - 1)^2 * 2 < P_k with
+    k minimal (exact integers; src/lib/mpnfft.c:18-78 states the same
+    conditions with a floating-point estimate)."""
+    from rforest_amd.fft import FftParams
+    P = FftParams(bits, terms)
+    t, lg, k = P.plan()
+    P.close()
+    b = 64 * t
+    points = -(-(bits + 1) // b) + 1
+    assert points <= 1 << lg
+    ps = rf.primes(32)
+    f = 2 * terms * ((points + 1) // 2) * (2 ** b - 1) ** 2
+    assert f < math.prod(ps[:k])
+    assert k == 1 or f >= math.prod(ps[:k - 1])
+    # the extreme operands: bits(x) + bits(y) = bits + 1 chunks still fit
+    for bx in (1, bits // 2 + 1, bits):
+        by = bits + 1 - bx
+        assert -(-bx // b) + -(-max(by, 1) // b) - 1 <= points
+
+
+def test_general_transform_plan_rejects_what_no_plan_holds():
+    from rforest_amd.fft import FftParams
+    with pytest.raises(rf.RfError):
+        FftParams(1 << 26, 1)
+    with pytest.raises(rf.RfError):
+        FftParams(0, 1)

@@ -100,7 +100,7 @@ def test_inv_stage_long(lgN, r, d):
 
 
 @pytest.mark.parametrize("lgN", [1, 2, 3, 6, 10, 13, 14])
-@pytest.mark.parametrize("r,d", [(1, 1), (2, 2), (3, 2), (4, 4), (1, 9)])
+@pytest.mark.parametrize("r,d", [(1, 1), (2, 2), (3, 2), (2, 3), (4, 4), (1, 9)])
 def test_inv_stage(lgN, r, d):
     rng = random.Random(lgN * 100 + r * 10 + d)
     P = rf.primes(32)
