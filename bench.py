@@ -100,7 +100,7 @@ def cpu_worker(seconds, batch):
     """One CPU worker process of the all-core baseline: the same workload,
     products in turn for `seconds`; prints its count as JSON."""
     from rforest_amd import workloads
-    products = workloads.headline(batch, seed=42)
+    products = workloads.headline(min(batch, 8), seed=42)  # the first products of the same stream
     fn = _cpu_fn()
     n, t0 = 0, time.perf_counter()
     while n < 1 or time.perf_counter() - t0 < seconds:
@@ -371,7 +371,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=64, help="products per step")
+    ap.add_argument("--batch", type=int, default=256, help="products per step")
     ap.add_argument("--lanes", type=int, default=1, help="overlapping pipelines per prepared batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
