@@ -110,18 +110,25 @@ int setup_locked() {
   int rc0 = host_init_locked();
   if (rc0) return rc0;
   RfConsts* hc = (RfConsts*)calloc(1, sizeof(RfConsts));
-  std::vector<double> twf((size_t)RF_MAXP * RF_FWD_TW * 2), twi((size_t)RF_MAXP * RF_INV_TW * 2);
   if (!hc) return fail(RF_ERR_NOMEM, "host allocation failed");
   rf_consts_init(&g.primes, hc);
-  rf_twiddles_init(&g.primes, twf.data(), twi.data());
+  double roots[2 * RF_TW_PRIMES] = {0};
+  rf_twiddle_roots(&g.primes, roots, roots + RF_TW_PRIMES);
   HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc((void**)&g.d_consts, sizeof(RfConsts)));
-  HIPCHK(hipMalloc((void**)&g.d_twf, twf.size() * sizeof(double)));
-  HIPCHK(hipMalloc((void**)&g.d_twi, twi.size() * sizeof(double)));
   HIPCHK(hipMemcpy(g.d_consts, hc, sizeof(RfConsts), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g.d_twf, twf.data(), twf.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g.d_twi, twi.data(), twi.size() * sizeof(double), hipMemcpyHostToDevice));
   free(hc);
+  // Twiddle tables (RF_TW_PRIMES x (2^20 + 2^21) pairs, 1.1 GiB) are
+  // generated on the device from each prime's root of order 2^RF_LGN_MAX.
+  const size_t twf_n = (size_t)RF_TW_PRIMES * RF_FWD_TW * 2, twi_n = (size_t)RF_TW_PRIMES * RF_INV_TW * 2;
+  HIPCHK(hipMalloc((void**)&g.d_twf, twf_n * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&g.d_twi, twi_n * sizeof(double)));
+  double* d_roots = nullptr;
+  HIPCHK(hipMalloc((void**)&d_roots, sizeof roots));
+  HIPCHK(hipMemcpy(d_roots, roots, sizeof roots, hipMemcpyHostToDevice));
+  HIPCHK(rf_launch_twiddles(g.d_consts, d_roots, d_roots + RF_TW_PRIMES, RF_TW_PRIMES, g.d_twf, g.d_twi, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  HIPCHK(hipFree(d_roots));
   g.tables.consts = g.d_consts;
   g.tables.tw_fwd = g.d_twf;
   g.tables.tw_inv = g.d_twi;
@@ -1476,7 +1483,7 @@ int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs,
   std::lock_guard<std::mutex> lk(g_mu);
   int rc = setup_locked();
   if (rc) return rc;
-  if (n <= 0 || t < 1 || t > RF_TMAX || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_MAXP ||
+  if (n <= 0 || t < 1 || t > RF_TMAX || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_TW_PRIMES ||
       (scale != 0 && scale != lgN) || nchunk > (1u << lgN))
     return fail(RF_ERR_ARG, "rf_stage_fwd: bad arguments");
   const size_t N = (size_t)1 << lgN;
@@ -1532,7 +1539,7 @@ int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int
   std::lock_guard<std::mutex> lk(g_mu);
   int rc = setup_locked();
   if (rc) return rc;
-  if (r <= 0 || d <= 0 || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_MAXP)
+  if (r <= 0 || d <= 0 || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_TW_PRIMES)
     return fail(RF_ERR_ARG, "rf_stage_inv: bad arguments");
   const size_t N = (size_t)1 << lgN;
   const size_t np = (size_t)(r * d + d * d);

@@ -720,19 +720,63 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
       out[inv_index<LGN, NPI - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
 }
 
+// ---- twiddle tables, generated on the device at set-up ----
+//
+// Forward T[m] = omega^rev(m) (rev over RF_LGN_MAX - 1 bits), inverse
+// V[len + j] = omega_{2 len}^-j = omi^(j 2^(RF_LGN_MAX - 1 - log2 len)), with
+// omega of order 2^RF_LGN_MAX and omi its inverse; each entry is (w, fl(w / p))
+// with w in [0, p), computed by square-and-multiply with exact modular
+// products of canonical operands.
+__device__ __forceinline__ double tw_pow(double b, uint32_t e, double p, double pinv) {
+  double r = 1.0;
+  while (e) {
+    if (e & 1) r = rf_canon(rf_mulmod_var(r, b, p, pinv), p);
+    b = rf_canon(rf_mulmod_var(b, b, p, pinv), p);
+    e >>= 1;
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_twiddles(const RfConsts* __restrict__ C, const double* __restrict__ om,
+                                                   const double* __restrict__ omi, double* __restrict__ fwd,
+                                                   double* __restrict__ inv) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const int q = blockIdx.y;
+  const double p = C->p[q], pinv = C->pinv[q];
+  if (i < (uint32_t)RF_FWD_TW) {
+    const uint32_t rev = __builtin_bitreverse32(i) >> (32 - (RF_LGN_MAX - 1));
+    const double w = tw_pow(om[q], rev, p, pinv);
+    double* F = fwd + ((size_t)q * RF_FWD_TW + i) * 2;
+    F[0] = w;
+    F[1] = w / p;
+  }
+  if (i < (uint32_t)RF_INV_TW) {
+    double w = 0.0;
+    if (i) {
+      const int e = 31 - __builtin_clz(i);  // len = 2^e
+      const uint32_t j = i - (1u << e);
+      w = tw_pow(omi[q], j << (RF_LGN_MAX - 1 - e), p, pinv);
+    }
+    double* V = inv + ((size_t)q * RF_INV_TW + i) * 2;
+    V[0] = w;
+    V[1] = i ? w / p : 0.0;
+  }
+}
+
 // ---- transforms longer than one workgroup: N = 2^(RF_LGN_ROW + N1) ----
 
-// Forward column pass: thread `col` packs chunks row 2^13 + col (rows
-// 0 .. 2^N1 - 1) and runs the first N1 stages, whose pairs differ in the row
-// bits only; at stage s the pair (hi, hi + 2^(N1-1-s)) uses T[hi >> (N1-s)],
-// the same for every column, so the whole pass is one 2^N1-point transform
-// per column.  Output in place, reduced.
+// Forward column pass: thread `col` packs chunks row 2^lgc + col (rows
+// 0 .. 2^N1 - 1, lgc = lgN - N1) and runs the first N1 stages, whose pairs
+// differ in the row bits only; at stage s the pair (hi, hi + 2^(N1-1-s)) uses
+// T[hi >> (N1-s)], the same for every column, so the whole pass is one
+// 2^N1-point transform per column.  Output in place, reduced.
 template <int N1, int T>
 __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
                                                             const RfFwdTask* __restrict__ tasks,
-                                                            int ntasks, double* __restrict__ spec) {
-  constexpr int ROWS = 1 << N1, COLS = 1 << RF_LGN_ROW, BPT = COLS / RF_COL_THREADS;
-  constexpr uint32_t N = (uint32_t)ROWS * COLS;
+                                                            int ntasks, double* __restrict__ spec, int lgc) {
+  constexpr int ROWS = 1 << N1;
+  const int COLS = 1 << lgc, BPT = COLS / RF_COL_THREADS;
+  const uint32_t N = (uint32_t)ROWS * (uint32_t)COLS;
   const int w = xcd_task(blockIdx.x, gridDim.x, ntasks * BPT);
   if (w < 0) return;
   const RfFwdTask tk = rf_wave_uniform_load(tasks + w / BPT);
@@ -821,33 +865,38 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
   fwd_store<LGN>(a, v, tid, p, pinv);
 }
 
-// Inverse column pass: after the row pass (k_inv on each row) the last N1
-// stages L = 13 + s pair (hi, hi + 2^s) with twiddle
-// V[2^L + (hi mod 2^s) 2^13 + col].  Output in place, natural order, reduced.
-template <int N1>
-__global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
-                                                            const RfRowTask* __restrict__ tasks,
-                                                            int ntasks, double* __restrict__ coef) {
-  constexpr int ROWS = 1 << N1, COLS = 1 << RF_LGN_ROW, BPT = COLS / RF_COL_THREADS;
+// Forward middle pass of transforms above 2^17 points (N1 = lgN - 13 > 4
+// row bits): after the column pass over the top N1 - NB row bits, the next NB
+// stages act on row bits NB-1 .. 0.  A thread holds row (hi 2^NB + j) of one
+// column for j < 2^NB, hi being the processed row bits; at stage s of the pass
+// the pair (j, j + 2^(NB-1-s)) uses T[(hi << s) + (j >> (NB - s))] (the top
+// stage-count bits of the index).  Output in place, reduced.
+template <int NB>
+__global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_mid(RfDevTables tb, const RfFwdTask* __restrict__ tasks,
+                                                            int ntasks, double* __restrict__ spec, int lgN) {
+  constexpr int J = 1 << NB;
+  const int lgt = lgN - NB;                       // threads per transform: 2^lgt
+  const int BPT = (1 << lgt) / RF_COL_THREADS;
   const int w = xcd_task(blockIdx.x, gridDim.x, ntasks * BPT);
   if (w < 0) return;
-  const RfRowTask tk = rf_wave_uniform_load(tasks + w / BPT);
-  const int col = (w % BPT) * RF_COL_THREADS + (int)threadIdx.x;
+  const RfFwdTask tk = rf_wave_uniform_load(tasks + w / BPT);
+  const int th = (w % BPT) * RF_COL_THREADS + (int)threadIdx.x;
+  const int col = th & ((1 << RF_LGN_ROW) - 1), hi = th >> RF_LGN_ROW;
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
-  double* a = coef + tk.off;
-  double v[ROWS];
+  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  double* a = spec + tk.spec_off + ((size_t)hi << (RF_LGN_ROW + NB)) + col;
+  double v[J];
 #pragma unroll
-  for (int j = 0; j < ROWS; j++) v[j] = a[(size_t)j * COLS + col];
+  for (int j = 0; j < J; j++) v[j] = a[(size_t)j << RF_LGN_ROW];
 #pragma unroll
-  for (int st = 0; st < N1; st++) {
-    const int hf = 1 << st;
+  for (int s = 0; s < NB; s++) {
+    const int hf = J >> (s + 1);
 #pragma unroll
-    for (int jj = 0; jj < hf; jj++) {
-      const double2 wt = tw[(1 << (RF_LGN_ROW + st)) + (jj << RF_LGN_ROW) + col];
+    for (int b = 0; b < (1 << s); b++) {
+      const double2 wt = tw[(hi << s) + b];
 #pragma unroll
-      for (int b = 0; b < (ROWS >> (st + 1)); b++) {
+      for (int jj = 0; jj < hf; jj++) {
         const int j0 = b * 2 * hf + jj, j1 = j0 + hf;
         const double z = rf_mulmod(v[j1], wt.x, wt.y, p);
         const double x = v[j0];
@@ -857,7 +906,56 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
     }
   }
 #pragma unroll
-  for (int j = 0; j < ROWS; j++) a[(size_t)j * COLS + col] = rf_reduce(v[j], p, pinv);
+  for (int j = 0; j < J; j++) a[(size_t)j << RF_LGN_ROW] = rf_reduce(v[j], p, pinv);
+}
+
+// Inverse column pass over row bits b0 .. b0+NB-1: after the row pass (k_inv
+// on each row) and the passes over the row bits below b0, stage L = 13 + b0 + s
+// pairs rows (R, R + 2^(b0+s)); a thread holds rows hi 2^(b0+NB) + j 2^b0 + lo
+// of one column for j < 2^NB, and the pair (j, j + 2^s) uses
+// V[2^L + ((j mod 2^s) 2^b0 + lo) 2^13 + col] (omega_{2^(L+1)} to minus the
+// index mod 2^L).  Output in place, natural order, reduced.
+template <int NB>
+__global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
+                                                            const RfRowTask* __restrict__ tasks,
+                                                            int ntasks, double* __restrict__ coef, int lgN, int b0) {
+  constexpr int J = 1 << NB;
+  const int lgt = lgN - NB;
+  const int BPT = (1 << lgt) / RF_COL_THREADS;
+  const int w = xcd_task(blockIdx.x, gridDim.x, ntasks * BPT);
+  if (w < 0) return;
+  const RfRowTask tk = rf_wave_uniform_load(tasks + w / BPT);
+  const int th = (w % BPT) * RF_COL_THREADS + (int)threadIdx.x;
+  const int col = th & ((1 << RF_LGN_ROW) - 1);
+  const int lo = (th >> RF_LGN_ROW) & ((1 << b0) - 1), hi = th >> (RF_LGN_ROW + b0);
+  const RfConsts* C = tb.consts;
+  const double p = C->p[tk.q], pinv = C->pinv[tk.q];
+  const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
+  double* a = coef + tk.off + ((((size_t)hi << (b0 + NB)) + lo) << RF_LGN_ROW) + col;
+  const size_t stride = (size_t)1 << (RF_LGN_ROW + b0);
+  const int low = (lo << RF_LGN_ROW) + col;
+  double v[J];
+#pragma unroll
+  for (int j = 0; j < J; j++) v[j] = a[(size_t)j * stride];
+#pragma unroll
+  for (int st = 0; st < NB; st++) {
+    const int hf = 1 << st;
+    const int L = RF_LGN_ROW + b0 + st;
+#pragma unroll
+    for (int jj = 0; jj < hf; jj++) {
+      const double2 wt = tw[(1 << L) + (jj << (RF_LGN_ROW + b0)) + low];
+#pragma unroll
+      for (int b = 0; b < (J >> (st + 1)); b++) {
+        const int j0 = b * 2 * hf + jj, j1 = j0 + hf;
+        const double z = rf_mulmod(v[j1], wt.x, wt.y, p);
+        const double x = v[j0];
+        v[j0] = x + z;
+        v[j1] = x - z;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < J; j++) a[(size_t)j * stride] = rf_reduce(v[j], p, pinv);
 }
 
 template <int LGN, int T>
@@ -941,12 +1039,12 @@ hipError_t rf_launch_inv(const RfDevTables& tb, int lgN, const RfInvTask* tasks,
 }
 
 template <int N1>
-hipError_t launch_fwd_col_n(int t, dim3 grid, dim3 blk, const RfDevTables& tb, const RfFwdTask* tasks, int ntasks,
-                            double* spec, hipStream_t st) {
+static hipError_t launch_fwd_col_n(int t, dim3 grid, dim3 blk, const RfDevTables& tb, const RfFwdTask* tasks, int ntasks,
+                            double* spec, int lgc, hipStream_t st) {
   switch (t) {
-#define RF_TC(TT)                                                                             \
-  case TT:                                                                                    \
-    hipLaunchKernelGGL((k_fwd_col<N1, TT>), grid, blk, 0, st, tb, tasks, ntasks, spec); \
+#define RF_TC(TT)                                                                                  \
+  case TT:                                                                                         \
+    hipLaunchKernelGGL((k_fwd_col<N1, TT>), grid, blk, 0, st, tb, tasks, ntasks, spec, lgc); \
     return hipGetLastError();
     RF_TC(1) RF_TC(2) RF_TC(3) RF_TC(4) RF_TC(5) RF_TC(6) RF_TC(7) RF_TC(8)
 #undef RF_TC
@@ -955,17 +1053,35 @@ hipError_t launch_fwd_col_n(int t, dim3 grid, dim3 blk, const RfDevTables& tb, c
   }
 }
 
+hipError_t rf_launch_twiddles(const RfConsts* consts, const double* om, const double* omi, int nprimes,
+                              double* fwd, double* inv, hipStream_t st) {
+  const dim3 grid((unsigned)((RF_INV_TW + 255) / 256), (unsigned)nprimes);
+  hipLaunchKernelGGL(k_twiddles, grid, dim3(256), 0, st, consts, om, omi, fwd, inv);
+  return hipGetLastError();
+}
+
+// n1 = lgN - 13 row bits: up to 4 in one column pass (pack fused); above
+// that the column pass takes the top n1 - 4 and a middle pass the low 4.
 hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, int t, const RfFwdTask* tasks, int ntasks,
                              double* spec, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  const dim3 grid(ntasks * ((1 << RF_LGN_ROW) / RF_COL_THREADS)), blk(RF_COL_THREADS);
-  switch (lgN - RF_LGN_ROW) {
-    case 1: return launch_fwd_col_n<1>(t, grid, blk, tb, tasks, ntasks, spec, st);
-    case 2: return launch_fwd_col_n<2>(t, grid, blk, tb, tasks, ntasks, spec, st);
-    case 3: return launch_fwd_col_n<3>(t, grid, blk, tb, tasks, ntasks, spec, st);
-    case 4: return launch_fwd_col_n<4>(t, grid, blk, tb, tasks, ntasks, spec, st);
+  const int n1 = lgN - RF_LGN_ROW;
+  if (n1 < 1 || lgN > RF_LGN_MAX) return hipErrorInvalidValue;
+  const int na = n1 > 4 ? n1 - 4 : n1;  // bits of the pack-fused column pass
+  const int lgc = lgN - na;
+  const dim3 grid(ntasks * ((1 << lgc) / RF_COL_THREADS)), blk(RF_COL_THREADS);
+  hipError_t e;
+  switch (na) {
+    case 1: e = launch_fwd_col_n<1>(t, grid, blk, tb, tasks, ntasks, spec, lgc, st); break;
+    case 2: e = launch_fwd_col_n<2>(t, grid, blk, tb, tasks, ntasks, spec, lgc, st); break;
+    case 3: e = launch_fwd_col_n<3>(t, grid, blk, tb, tasks, ntasks, spec, lgc, st); break;
+    case 4: e = launch_fwd_col_n<4>(t, grid, blk, tb, tasks, ntasks, spec, lgc, st); break;
     default: return hipErrorInvalidValue;
   }
+  if (e != hipSuccess || n1 <= 4) return e;
+  const dim3 grid2(ntasks * ((1 << (lgN - 4)) / RF_COL_THREADS));
+  hipLaunchKernelGGL(k_fwd_mid<4>, grid2, blk, 0, st, tb, tasks, ntasks, spec, lgN);
+  return hipGetLastError();
 }
 
 hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int ntasks, double* spec,
@@ -978,18 +1094,28 @@ hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int 
   return hipGetLastError();
 }
 
-hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* tasks, int ntasks,
-                             double* coef, hipStream_t st) {
-  if (ntasks <= 0) return hipSuccess;
-  const dim3 grid(ntasks * ((1 << RF_LGN_ROW) / RF_COL_THREADS)), blk(RF_COL_THREADS);
-  switch (lgN - RF_LGN_ROW) {
-#define RF_CASE(N1) \
-  case N1:          \
-    hipLaunchKernelGGL(k_inv_col<N1>, grid, blk, 0, st, tb, tasks, ntasks, coef); \
+// Row bits 0 .. min(n1, 4) - 1 first, then (n1 > 4) the rest in a second pass.
+static hipError_t launch_inv_col_pass(const RfDevTables& tb, int lgN, int b0, int nb, const RfRowTask* tasks,
+                                      int ntasks, double* coef, hipStream_t st) {
+  const dim3 grid(ntasks * ((1 << (lgN - nb)) / RF_COL_THREADS)), blk(RF_COL_THREADS);
+  switch (nb) {
+#define RF_CASE(NB) \
+  case NB:          \
+    hipLaunchKernelGGL(k_inv_col<NB>, grid, blk, 0, st, tb, tasks, ntasks, coef, lgN, b0); \
     return hipGetLastError();
     RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4)
 #undef RF_CASE
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* tasks, int ntasks,
+                             double* coef, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  const int n1 = lgN - RF_LGN_ROW;
+  if (n1 < 1 || lgN > RF_LGN_MAX) return hipErrorInvalidValue;
+  hipError_t e = launch_inv_col_pass(tb, lgN, 0, n1 < 4 ? n1 : 4, tasks, ntasks, coef, st);
+  if (e != hipSuccess || n1 <= 4) return e;
+  return launch_inv_col_pass(tb, lgN, 4, n1 - 4, tasks, ntasks, coef, st);
 }

@@ -140,6 +140,7 @@ int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int 
     }
   if (k == 0 || kmin > pr->k) return -1;
   if (k < kmin) k = kmin;  // more primes than the bound needs are always valid
+  if (k > RF_KMAX) return -1;
   pl.k = k;
   pl.nout = (int)nout;
   // Coefficients lie in (-P/2, P/2) with P < 2^(48 k): 48 k + 1 bits of two's
@@ -163,6 +164,7 @@ int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int 
   // Transforms longer than one workgroup's LDS take a column pass and a row
   // pass: one more read and write of every residue, and a few more ops.
   if (lgN > RF_LGN_LDS) bytes *= 2, ops *= 1.1;
+  if (lgN > RF_LGN_ROW + 4) bytes *= 1.5;  // a second column pass
   double c = ops > bytes * 39 / 8 ? ops : bytes * 39 / 8;
   if (lgN == RF_LGN_LDS) c *= 1.25;  // one workgroup per CU at 128 KiB of LDS
   pl.cost = c;
@@ -209,7 +211,7 @@ int rf_plan_general(const RfPrimes* pr, size_t bits, unsigned terms, RfPlan* out
         k = kk;
         break;
       }
-    if (k == 0) continue;
+    if (k == 0 || k > RF_KMAX) continue;
     RfPlan pl;
     memset(&pl, 0, sizeof pl);
     pl.t = t;
@@ -222,7 +224,7 @@ int rf_plan_general(const RfPrimes* pr, size_t bits, unsigned terms, RfPlan* out
     pl.nout = (int)points;
     pl.rn = (bits + 63) / 64 + 1;
     // residues moved per transform, the dominant cost of every operation
-    pl.cost = (double)k * (double)((size_t)1 << lgN) * (lgN > RF_LGN_LDS ? 2.0 : 1.0);
+    pl.cost = (double)k * (double)((size_t)1 << lgN) * (lgN > RF_LGN_LDS ? (lgN > RF_LGN_ROW + 4 ? 3.0 : 2.0) : 1.0);
     if (!found || pl.cost < best.cost) best = pl, found = true;
   }
   if (!found) return -1;
@@ -267,37 +269,11 @@ void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
   }
 }
 
-void rf_twiddles_init(const RfPrimes* pr, double* fwd, double* inv) {
-  const int B = RF_LGN_MAX - 1;
-  std::vector<uint64_t> pw(RF_FWD_TW);
-  for (int q = 0; q < pr->k; q++) {
-    uint64_t p = pr->p[q];
-    double dp = (double)p;
-    uint64_t om = rf_h_powmod(pr->g[q], (uint64_t)1 << (RF_S - RF_LGN_MAX), p);  // order 2^LGN_MAX
-    pw[0] = 1;
-    for (int m = 1; m < RF_FWD_TW; m++) pw[m] = rf_h_mulmod(pw[m - 1], om, p);
-    double* F = fwd + (size_t)q * RF_FWD_TW * 2;
-    for (int m = 0; m < RF_FWD_TW; m++) {
-      unsigned rev = 0;
-      for (int i = 0; i < B; i++) rev |= ((m >> i) & 1u) << (B - 1 - i);
-      uint64_t w = pw[rev];
-      F[2 * m] = (double)w;
-      F[2 * m + 1] = (double)w / dp;
-    }
-    double* V = inv + (size_t)q * RF_INV_TW * 2;
-    V[0] = 0;
-    V[1] = 0;
-    uint64_t omi = rf_h_invmod(om, p);
-    for (int e = 0; e < RF_LGN_MAX; e++) {
-      int len = 1 << e;
-      // omega_{2 len}^-1 = omi^(2^LGN_MAX / (2 len))
-      uint64_t step = rf_h_powmod(omi, (uint64_t)(RF_INV_TW / (2 * len)), p);
-      uint64_t w = 1;
-      for (int j = 0; j < len; j++) {
-        V[2 * (len + j)] = (double)w;
-        V[2 * (len + j) + 1] = (double)w / dp;
-        w = rf_h_mulmod(w, step, p);
-      }
-    }
+void rf_twiddle_roots(const RfPrimes* pr, double* om, double* omi) {
+  for (int q = 0; q < pr->k && q < RF_TW_PRIMES; q++) {
+    const uint64_t p = pr->p[q];
+    const uint64_t w = rf_h_powmod(pr->g[q], (uint64_t)1 << (RF_S - RF_LGN_MAX), p);  // order 2^LGN_MAX
+    om[q] = (double)w;
+    omi[q] = (double)rf_h_invmod(w, p);
   }
 }

@@ -20,7 +20,9 @@
 #define RF_LGN_MIN 1
 #define RF_LGN_LDS 14     // largest one-workgroup transform (2^14 doubles = 128 KiB of LDS)
 #define RF_LGN_ROW 13     // longer transforms: columns of 2^(lgN-13) in registers, rows of 2^13 in LDS
-#define RF_LGN_MAX 17     // largest transform (columns of at most 16 residues)
+#define RF_LGN_MAX 21     // largest transform: 2^8 rows of 2^13 (two column passes of <= 16 residues)
+#define RF_KMAX 24        // most primes a plan may use (the CRT kernel's instantiations)
+#define RF_TW_PRIMES RF_KMAX  // primes with twiddle tables
 #define RF_TMAX 8         // chunk width up to 8 limbs = 512 bits
 #define RF_MAXWORDS (2 * RF_TMAX)  // 32-bit words per chunk
 #define RF_MAXS 24        // CRT limbs per coefficient, >= ceil((48 RF_MAXP + 2) / 64)
@@ -107,5 +109,6 @@ struct RfConsts {
 #define RF_INV_TW (1 << RF_LGN_MAX)
 
 void rf_consts_init(const RfPrimes* pr, RfConsts* c);
-void rf_twiddles_init(const RfPrimes* pr, double* fwd /* RF_MAXP*RF_FWD_TW*2 */,
-                      double* inv /* RF_MAXP*RF_INV_TW*2 */);
+// omega of order 2^RF_LGN_MAX and its inverse, per prime, as exact doubles
+// (the device generates the tables from them).
+void rf_twiddle_roots(const RfPrimes* pr, double* om, double* omi);

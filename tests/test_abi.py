@@ -125,10 +125,17 @@ def test_long_transform_plans():
 
 
 def test_unsupported_sizes_are_reported():
-    # 2^26-bit entries would need 2^18 points at the widest chunks
+    # 2^30-bit entries would need 2^22 points at the widest chunks
     with pytest.raises(rf.RfError) as e:
-        rf.plan(1 << 26, 1 << 26, 2, 2)
+        rf.plan(1 << 30, 1 << 30, 2, 2)
     assert e.value.code == rf.RF_ERR_UNSUPPORTED
+
+
+def test_longest_transform_plans():
+    """Entries up to 2^29 bits plan within 2^21 points, k <= 24 primes."""
+    for bits in (1 << 26, 3 << 26, 1 << 28, 1 << 29):
+        t, lgN, k = rf.plan(bits, bits, 2, 2)
+        assert 17 < lgN <= 21 and k <= 24 and _bound_ok(t, lgN, k, bits, bits, 2)
 
 
 def test_gmp_schoolbook_matches_python_integers():
@@ -193,6 +200,6 @@ def test_general_transform_plan_bound_is_proven(bits, terms):
 def test_general_transform_plan_rejects_what_no_plan_holds():
     from rforest_amd.fft import FftParams
     with pytest.raises(rf.RfError):
-        FftParams(1 << 26, 1)
+        FftParams(1 << 31, 1)
     with pytest.raises(rf.RfError):
         FftParams(0, 1)

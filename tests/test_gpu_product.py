@@ -57,7 +57,8 @@ def test_plain_gpu_case_4x4_seed_1002():
     assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == naive(A, r, B, d)
 
 
-@pytest.mark.parametrize("bits,r,d", [(5 << 20, 2, 2), (1 << 24, 2, 2), (3 << 21, 3, 3)])
+@pytest.mark.parametrize("bits,r,d", [(5 << 20, 2, 2), (1 << 24, 2, 2), (3 << 21, 3, 3), (1 << 26, 1, 1),
+                                      (3 << 25, 2, 1), (1 << 27, 1, 1), (1 << 29, 1, 1)])
 def test_long_transform_products(bits, r, d):
     """Entries whose plans need more than 2^14 points (column + row passes),
     against GMP schoolbook."""
@@ -118,8 +119,9 @@ def test_dropin_mpz_abi():
 
 
 def test_unsupported_size_returns_error():
+    # one entry of 2^30 bits needs 2^22 points at the widest chunks
     rng = random.Random(1)
-    big = [rng.getrandbits(1 << 26) for _ in range(4)]
+    big = [rng.getrandbits(1 << 30) | 1 << ((1 << 30) - 1)] + [rng.getrandbits(64) for _ in range(3)]
     mA = MpzArray(big)
     mC = MpzArray.zeros(4)
     assert rf.lib().mpz_rmatrix_gpu_fft_suitable(mA.ptr(), 2, mA.ptr(), 2) == 0
