@@ -205,6 +205,28 @@ int rf_fft_matrix_mul(rf_fft* const* rop, rf_fft* const* op1, rf_fft* const* op2
                       unsigned dim3);
 
 /* ---------------------------------------------------------------------------
+ * Remainders X mod D (the reference's remainder step, mpzfft_mod_init /
+ * mpzfft_mod_mod, src/lib/mpzfft.c:59-100, src/include/mpnfft_mod.h:53-75):
+ * a modulus object prepared once for dividends of up to max_bits bits keeps
+ * D and I = floor(2^(64(m+1)) / D) (m = limbs of the largest dividend) as
+ * resident transforms; each remainder is then two products with them (the
+ * quotient estimate from the high limbs of X I, then Q D) and a device
+ * fix-up to [0, D).  Unlike the reference, I is exact (GMP division at
+ * creation) rather than approximate; the results are the same.
+ * ------------------------------------------------------------------------- */
+typedef struct rf_mod rf_mod;
+/* mpzfft_mod_init: D > 0 (RF_ERR_ARG otherwise), dividends |X| < 2^max_bits. */
+int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out);
+/* Limbs of D, which every remainder of this modulus is written in. */
+uint64_t rf_mod_limbs(const rf_mod* mod);
+/* mpzfft_mod_mod: rop := op mod D in [0, D) (any sign of op). */
+int rf_mod_mod(const rf_mod* mod, mpz_t rop, const mpz_t op);
+/* A batch (a remainder-tree level) in one call: outs[i] (device,
+   rf_mod_limbs(mods[i]) limbs) := xs[i] mod D_i for device entries xs[i]. */
+int rf_mod_mod_device(const rf_mod* const* mods, const rf_dev_entry* xs, int cnt, uint64_t* const* outs);
+void rf_mod_destroy(rf_mod* mod); /* mpzfft_mod_clear */
+
+/* ---------------------------------------------------------------------------
  * Product trees kept in device memory (no reference counterpart: rforest's
  * trees call mpz_rmatrix_mult_fft level by level through mpz_t).  M_0 M_1 ...
  * M_{n-1} of n square d x d matrices, level by level: each level's pairs run

@@ -5,7 +5,7 @@ csrc/rf_crt.hip CRT and carries, csrc/rf_util.hip data kernels,
 csrc/rf_fftops.hip transform-domain arithmetic), host orchestration and the
 C ABI (csrc/rf_engine.cpp one-shot calls, batches, operands, trees;
 csrc/rf_serve.cpp the request server; csrc/rf_fft.cpp the general transform
-API), planner and tables (csrc/rf_plan.cpp).  GMP comes from /opt/conda (gmp.h is not installed
+API; csrc/rf_mod.cpp remainders), planner and tables (csrc/rf_plan.cpp).  GMP comes from /opt/conda (gmp.h is not installed
 system-wide on this image).
 """
 import os
@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librforest_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", f)
            for f in ("rf_ntt.hip", "rf_crt.hip", "rf_util.hip", "rf_fftops.hip", "rf_engine.cpp", "rf_serve.cpp",
-                     "rf_plan.cpp", "rf_fft.cpp")]
+                     "rf_plan.cpp", "rf_fft.cpp", "rf_mod.cpp")]
 OBJDIR = os.path.join(ROOT, "_build")
 GMP_PREFIX = os.environ.get("RF_GMP_PREFIX", "/opt/conda")
 

@@ -1100,7 +1100,8 @@ int rf_operand_create(mpz_t* A, int r, int d, uint64_t b_max_bits, rf_operand** 
     total += (E[e].nlimbs + 1) & ~(uint64_t)1;  // entries start 16-byte aligned
   }
   const size_t in_b = align256(total * 8 + 8);
-  int rc = ensure_arena(in_b);
+  // the arena must not grow (and move) once the limbs are in it
+  int rc = ensure_arena(in_b + operand_task_bound(r * d));
   if (rc) return rc;
   rc = ensure_stage(in_b);
   if (rc) return rc;
@@ -1660,3 +1661,23 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
 }
 
 }  // extern "C"
+
+namespace rfi {
+// Entry points for the library's other host sources (caller holds the lock).
+size_t operand_task_bound(int nent) {
+  const size_t rows = (size_t)1 << (RF_LGN_MAX - RF_LGN_ROW);
+  return align256((size_t)nent * RF_KMAX * sizeof(RfFwdTask)) + align256((size_t)nent * RF_KMAX * rows * sizeof(RfRowTask));
+}
+int make_operand(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out) {
+  return operand_make(A, r, d, b_max_bits, base, out);
+}
+int pipeline_run(std::vector<Product>& prods, const Layout& L, size_t base) {
+  return run_pipeline(prods, L, base, g.stream);
+}
+void free_operand(rf_operand* op) {
+  if (!op) return;
+  if (g.ready) (void)hipStreamSynchronize(g.stream);
+  if (op->spec) (void)hipFree(op->spec);
+  delete op;
+}
+}  // namespace rfi

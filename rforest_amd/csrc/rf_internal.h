@@ -89,6 +89,18 @@ std::mutex& engine_mutex();
 bool engine_ready();
 const RfDevTables& engine_tables();
 const RfPrimes& engine_primes();
+// A resident operand from entries already in device memory (task tables at
+// arena byte `base`), the pipeline of planned products run on the engine's
+// stream with its scratch at arena byte `base` (the arena must already hold
+// base + L.total() bytes; returns with the stream busy), and an operand's
+// release.
+int make_operand(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out);
+// Arena bytes an operand of nent entries needs for its transforms' task
+// tables at most (reserve them with its staged inputs: the arena moves when
+// it grows).
+size_t operand_task_bound(int nent);
+int pipeline_run(std::vector<Product>& prods, const Layout& L, size_t base);
+void free_operand(rf_operand* op);
 // Host-side set-up only (primes; no device): lets planners run without a GPU.
 int engine_host_init();
 // The engine's stream, and its device arena and pinned staging buffer grown

@@ -60,6 +60,26 @@ struct RfEntryMeta {
 };
 hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, hipStream_t st);
 
+// Remainder fix-up, one task per remainder X mod D (D of n limbs, top limb
+// nonzero): given QD = Q D for a quotient estimate with X - Q D in [0, 3D),
+// R = (X - QD) mod B^(n+1), then R -= D while R >= D (at most twice), then
+// R = D - R if X is negative and R != 0; out = R (n limbs, in [0, D)).
+// r and t are scratch of n + 1 limbs each.
+struct RfModTask {
+  const uint64_t* x;   // |X|
+  uint64_t xn;
+  const uint64_t* qd;  // Q D (non-negative), qdn limbs (0: Q = 0)
+  uint64_t qdn;
+  const uint64_t* d;   // D
+  uint64_t n;
+  uint64_t* r;
+  uint64_t* t;
+  uint64_t* out;
+  int32_t sign;        // sign of X: -1, 0, +1
+  int32_t pad;
+};
+hipError_t rf_launch_mod_fix(const RfModTask* tasks, int ntasks, hipStream_t st);
+
 hipError_t rf_launch_gen(const RfSpan* spans, const RfBlock* blocks, int nblocks, hipStream_t st);
 hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nblocks, uint64_t* sums,
                               hipStream_t st);

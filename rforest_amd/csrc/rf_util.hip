@@ -7,6 +7,8 @@
 //               without a memset node, safe under graph capture)
 //   k_signmag   two's complement results -> sign and magnitude in place (a
 //               product tree's level outputs become the next level's inputs)
+//   k_mod_fix   the last step of a remainder X mod D: X - Q D and its
+//               corrections by multi-limb subtraction
 //
 // The first three are flat, coalesced streams over lists of spans; a span is one
 // entry (k_gen) or one C entry (k_checksum), split by the host into blocks
@@ -110,6 +112,97 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_signmag(const RfConvTask* _
   }
 }
 
+// ---- multi-limb subtraction inside one workgroup ----
+//
+// dst = a - b over L limbs (limbs past an operand's length read 0), tiles of
+// RF_UTIL_THREADS limbs in order with the borrow carried between tiles.
+// Within a wave, limb i generates a borrow (g: a_i < b_i) or propagates one
+// (p: a_i == b_i); with the wave's masks G and P (disjoint) the borrows into
+// its limbs are the carries of the addition x + y + c for x = G | P, y = G:
+// carry into bit i = bit i of (x + y + c) ^ x ^ y, carry out of bit 63 =
+// ((x & y) | ((x | y) & ~(x + y + c))) >> 63.  The four waves' carries are
+// chained by one thread.  Returns the borrow out of limb L-1.  Thread j
+// handles limbs j, j + RF_UTIL_THREADS, ... in every call, so a thread reads
+// back only what it wrote itself.
+struct LimbSpan {
+  const uint64_t* p;
+  uint64_t n;
+  __device__ uint64_t at(uint64_t i) const { return i < n ? p[i] : 0; }
+};
+
+__device__ int sub_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uint64_t L, int* sh) {
+  // sh: [0..3] carry out with c=0, [4..7] with c=1, [8..11] wave carry-in, [12] tile borrow
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();  // every thread has read the previous call's result
+  if (threadIdx.x == 0) sh[12] = 0;
+  __syncthreads();
+  for (uint64_t base = 0; base < L; base += RF_UTIL_THREADS) {
+    const uint64_t i = base + threadIdx.x;
+    const bool in = i < L;
+    const uint64_t x = in ? a.at(i) : 0, y = in ? b.at(i) : 0;
+    const uint64_t G = __builtin_amdgcn_ballot_w64(in && x < y);
+    const uint64_t P = __builtin_amdgcn_ballot_w64(!in || x == y);
+    const uint64_t X = G | P, Y = G;
+    const uint64_t S0 = X + Y, S1 = X + Y + 1;
+    if (lane == 0) {
+      sh[wave] = (int)(((X & Y) | ((X | Y) & ~S0)) >> 63);
+      sh[4 + wave] = (int)(((X & Y) | ((X | Y) & ~S1)) >> 63);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int c = sh[12];
+      for (int w = 0; w < RF_UTIL_THREADS / 64; w++) {
+        sh[8 + w] = c;
+        c = c ? sh[4 + w] : sh[w];
+      }
+      sh[12] = c;
+    }
+    __syncthreads();
+    const uint64_t S = sh[8 + wave] ? S1 : S0;
+    const uint64_t bin = ((S ^ X ^ Y) >> lane) & 1;
+    if (in) dst[i] = x - y - bin;
+    __syncthreads();  // sh[] is rewritten by the next tile
+  }
+  return sh[12];
+}
+
+__global__ __launch_bounds__(RF_UTIL_THREADS) void k_mod_fix(const RfModTask* __restrict__ tasks, int ntasks) {
+  __shared__ int sh[16];
+  if ((int)blockIdx.x >= ntasks) return;
+  const RfModTask tk = tasks[blockIdx.x];
+  const uint64_t L = tk.n + 1;
+  uint64_t* R = tk.r;
+  uint64_t* T = tk.t;
+  const LimbSpan D{tk.d, tk.n};
+  if (tk.sign == 0) {
+    for (uint64_t i = threadIdx.x; i < tk.n; i += RF_UTIL_THREADS) tk.out[i] = 0;
+    return;
+  }
+  sub_limbs(LimbSpan{tk.x, tk.xn}, LimbSpan{tk.qd, tk.qdn}, R, L, sh);  // X - QD mod B^(n+1)
+  for (int k = 0; k < 2; k++) {
+    const int borrow = sub_limbs(LimbSpan{R, L}, D, T, L, sh);  // R >= D iff no borrow
+    if (!borrow) {
+      uint64_t* u = R;
+      R = T;
+      T = u;
+    }
+  }
+  if (tk.sign < 0) {
+    if (threadIdx.x == 0) sh[13] = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < L; base += RF_UTIL_THREADS) {
+      const uint64_t i = base + threadIdx.x;
+      if (__builtin_amdgcn_ballot_w64(i < L && R[i] != 0) && (threadIdx.x & 63) == 0) atomicOr(&sh[13], 1);
+    }
+    __syncthreads();
+    if (sh[13]) {
+      sub_limbs(D, LimbSpan{R, L}, T, L, sh);  // D - R
+      R = T;
+    }
+  }
+  for (uint64_t i = threadIdx.x; i < tk.n; i += RF_UTIL_THREADS) tk.out[i] = R[i];
+}
+
 __global__ void k_fill(uint64_t* __restrict__ dst, uint64_t n, uint64_t v) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = v;
@@ -134,6 +227,12 @@ hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nb
 hipError_t rf_launch_fill(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, n, v);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_mod_fix(const RfModTask* tasks, int ntasks, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mod_fix, dim3(ntasks), dim3(RF_UTIL_THREADS), 0, st, tasks, ntasks);
   return hipGetLastError();
 }
 

@@ -1,0 +1,71 @@
+"""Remainders X mod D (rf_mod_*, the reference's mpzfft_mod interface,
+src/lib/mpzfft.c:59-100) against Python's x % d, which is in [0, d) for
+d > 0 like the reference's output."""
+import random
+
+import pytest
+
+import rforest_amd as rf
+from rforest_amd.mod import Modulus, mod_batch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def setup():
+    rf.hw_mpz_setup()
+    yield
+
+
+@pytest.mark.parametrize("dbits,xbits", [(1, 64), (64, 64), (64, 200), (65, 130), (1000, 2000), (4096, 4096),
+                                         (50000, 100001), (1 << 20, 1 << 21), (777, 1 << 18), (3 << 20, 5 << 20)])
+def test_remainders(dbits, xbits):
+    rng = random.Random(dbits * 3 + xbits)
+    d = rng.getrandbits(dbits) | (1 << (dbits - 1))
+    M = Modulus(d, xbits)
+    xs = [0, 1, -1, d, -d, d - 1, d + 1, 2 * d, 3 * d - 1, -(3 * d) + 1,
+          (1 << xbits) - 1, -((1 << xbits) - 1)] + [rng.getrandbits(xbits) * rng.choice([-1, 1]) for _ in range(4)]
+    for x in xs:
+        if abs(x).bit_length() <= max(xbits, dbits):
+            assert M.mod(x) == x % d, (dbits, xbits, x.bit_length())
+
+
+def test_quotient_estimate_edges():
+    """Dividends right at the quotient estimate's error bound: D a power of
+    two, D = 2^k - 1 (all-ones limbs), and X = q D + r with r near 0 and D."""
+    rng = random.Random(3)
+    for d in ((1 << 640), (1 << 640) - 1, (1 << 64) + 1, 3):
+        M = Modulus(d, 2048)
+        for _ in range(20):
+            q = rng.getrandbits(2048 - d.bit_length())
+            for r in (0, 1, d - 1, d // 2):
+                x = q * d + r
+                if x.bit_length() <= 2048:
+                    assert M.mod(x) == r
+                    assert M.mod(-x) == (-x) % d
+
+
+def test_batch_matches_one_by_one():
+    """A remainder-tree level: many moduli of uneven sizes in one call."""
+    rng = random.Random(5)
+    pairs = []
+    for i in range(40):
+        db = rng.choice([64, 300, 5000, 40000, 200000])
+        d = rng.getrandbits(db) | (1 << (db - 1))
+        xb = rng.choice([db, 2 * db, 2 * db + 5])
+        M = Modulus(d, xb)
+        x = rng.getrandbits(xb) * rng.choice([-1, 1])
+        pairs.append((M, x))
+    got = mod_batch(pairs)
+    assert got == [x % M.d for (M, x) in pairs]
+    assert [M.mod(x) for (M, x) in pairs[:6]] == got[:6]
+
+
+def test_errors():
+    with pytest.raises(rf.RfError):
+        Modulus(0, 100)
+    with pytest.raises(rf.RfError):
+        Modulus(-5, 100)
+    M = Modulus(12345, 128)
+    with pytest.raises(rf.RfError):
+        M.mod(1 << 300)
