@@ -277,6 +277,48 @@ def resident_section(rf, workloads, reps):
             "checks": {"identical_to_recompute": got == want, "exact_product0": exact}}
 
 
+def remainder_section(rf, workloads, reps):
+    """A remainder-tree level: 256 remainders X mod D of 2,097,152-bit signed
+    dividends by 1,048,576-bit moduli, each modulus prepared once (D and its
+    inverse kept as resident transforms), all remainders of the level in one
+    call, against one remainder per call through mpz_t and GMP's mpz_mod on
+    one host core."""
+    from rforest_amd.gmp import mod_values
+    from rforest_amd.mod import ModBatch, Modulus
+    pairs = workloads.remainder_level(256, seed=1007)
+    t0 = time.perf_counter()
+    mods = [Modulus(d, 1 << 21) for (d, _) in pairs]
+    t_create = (time.perf_counter() - t0) / len(mods)
+    batch = ModBatch([(M, x) for M, (_, x) in zip(mods, pairs)])
+    batch.run()
+    got = batch.results()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run()
+    t_batch = (time.perf_counter() - t0) / reps
+    want = mod_values([x for (_, x) in pairs], [d for (d, _) in pairs])
+    ns = 16
+    t0 = time.perf_counter()
+    one = [mods[i].mod(pairs[i][1]) for i in range(ns)]
+    t_one = (time.perf_counter() - t0) / ns
+    t0 = time.perf_counter()
+    nc = 0
+    while nc < 8 or time.perf_counter() - t0 < 2.0:
+        mod_values([pairs[nc % len(pairs)][1]], [pairs[nc % len(pairs)][0]])
+        nc += 1
+    t_cpu = (time.perf_counter() - t0) / nc
+    batch.release()
+    for M in mods:
+        M.close()
+    return {"workload": "256 remainders X mod D, X 2,097,152-bit signed, D 1,048,576-bit (top bit set), GMP seed 1007",
+            "batched_remainders_per_s": round(len(pairs) / t_batch, 1),
+            "one_per_call_remainders_per_s": round(1 / t_one, 1),
+            "gmp_cpu_remainders_per_s": round(1 / t_cpu, 1),
+            "gmp_cpu_sample": f"{nc} calls of mpz_mod on one host core",
+            "modulus_create_ms": round(t_create * 1e3, 3),
+            "checks": {"identical_to_gmp": got == want, "one_per_call_identical": one == want[:ns]}}
+
+
 def _pct(xs, q):
     xs = sorted(xs)
     return xs[min(len(xs) - 1, int(math.ceil(q / 100.0 * len(xs))) - 1)] if xs else None
@@ -483,6 +525,8 @@ def main():
         extra["resident_operand"] = resident_section(rf, workloads, max(3, args.steps // 4))
         log("product tree in device memory")
         extra["product_tree"] = tree_section(rf, workloads, 3)
+        log("remainder-tree level")
+        extra["remainder_level"] = remainder_section(rf, workloads, 3)
         log("serving: mixed request stream")
         extra["serving"] = serving_section(rf, workloads, args.requests, min(2000, args.requests), 50,
                                            min(5000, args.requests))

@@ -106,9 +106,7 @@ int mod_batch_plan(const rf_mod* const* mods, const rf_dev_entry* xs, int cnt, M
 int mod_batch_exec(ModBatch& mb, const rf_mod* const* mods, const rf_dev_entry* xs, int cnt, uint64_t* const* outs,
                    size_t base) {
   char* d_base = nullptr;
-  char* h_base = nullptr;
   int rc = engine_arena(base + mb.bytes(), &d_base);
-  if (!rc) rc = engine_stage(mb.task_b, &h_base);
   if (rc) return rc;
   uint64_t* T = (uint64_t*)(d_base + base);
   uint64_t* QD = (uint64_t*)(d_base + base + mb.t_b);
@@ -133,6 +131,10 @@ int mod_batch_exec(ModBatch& mb, const rf_mod* const* mods, const rf_dev_entry* 
   if (rc) return rc;
   hipStream_t st = engine_stream();
   HIPCHK(hipStreamSynchronize(st));
+  // the pipelines may have grown (replaced) the staging buffer: fetch it now
+  char* h_base = nullptr;
+  rc = engine_stage(mb.task_b, &h_base);
+  if (rc) return rc;
   RfModTask* ht = (RfModTask*)h_base;
   for (int i = 0; i < cnt; i++) {
     const rf_mod* md = mods[i];
@@ -273,6 +275,8 @@ int rf_mod_mod(const rf_mod* mod, mpz_t rop, const mpz_t op) {
   x.limbs = (const uint64_t*)d_base;
   uint64_t* outs[1] = {(uint64_t*)(d_base + x_b)};
   rc = mod_batch_exec(mb, mods, &x, 1, outs, x_b + o_b);
+  if (rc) return rc;
+  rc = engine_stage(o_b, &h_base);  // the staging buffer may have been replaced
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(h_base, d_base + x_b, n * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

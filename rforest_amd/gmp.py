@@ -26,13 +26,14 @@ def libgmp():
             "neg": getattr(g, "__gmpz_neg"),
             "mul": getattr(g, "__gmpz_mul"),
             "addmul": getattr(g, "__gmpz_addmul"),
+            "mod": getattr(g, "__gmpz_mod"),
         }
         _g["init"].argtypes = [ctypes.POINTER(MPZ)]
         _g["clear"].argtypes = [ctypes.POINTER(MPZ)]
         _g["import"].argtypes = [ctypes.POINTER(MPZ), ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t,
                                  ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p]
         _g["neg"].argtypes = [ctypes.POINTER(MPZ), ctypes.POINTER(MPZ)]
-        for f in ("mul", "addmul"):
+        for f in ("mul", "addmul", "mod"):
             _g[f].argtypes = [ctypes.POINTER(MPZ), ctypes.POINTER(MPZ), ctypes.POINTER(MPZ)]
     return _g
 
@@ -112,3 +113,17 @@ def rmatrix_mult_schoolbook(A, r, B, d):
         mA.close()
         mB.close()
         mC.close()
+
+
+def mod_values(xs, ds):
+    """[x mod d] (in [0, d)) through GMP's mpz_mod."""
+    g = libgmp()
+    mx, md, mr = MpzArray(xs), MpzArray(ds), MpzArray.zeros(len(xs))
+    try:
+        for i in range(len(xs)):
+            g["mod"](ctypes.byref(mr.arr[i]), ctypes.byref(mx.arr[i]), ctypes.byref(md.arr[i]))
+        return mr.values()
+    finally:
+        mx.close()
+        md.close()
+        mr.close()

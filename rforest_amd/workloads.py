@@ -148,3 +148,17 @@ def served_stream(n=20000, seed=1005, lo=12, hi=24):
         out.append((r, [L] * ents, signs, seed * (1 << 20) + i, gap))
     rng.close()
     return out
+
+
+def remainder_level(n=256, seed=1007, dbits=1 << 20, xbits=1 << 21):
+    """n remainders of a remainder-tree level: D with dbits bits (top bit
+    set), X uniform of xbits bits with a random sign, one generator seeded
+    `seed` (D then X per pair)."""
+    rng = GmpRandom(seed)
+    out = []
+    for _ in range(n):
+        d = rng.top(dbits)
+        x = rng.urandomb(xbits) * (-1 if rng.urandomb(1) else 1)
+        out.append((d, x))
+    rng.close()
+    return out

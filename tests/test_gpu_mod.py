@@ -69,3 +69,19 @@ def test_errors():
     M = Modulus(12345, 128)
     with pytest.raises(rf.RfError):
         M.mod(1 << 300)
+
+
+def test_large_batch_of_small_remainders():
+    """Thousands of small remainders in one call: the pipelines' task images
+    outgrow the staging buffer mid-call (a regression test for reading the
+    buffer through a pointer taken before it was replaced)."""
+    rng = random.Random(8)
+    pairs = []
+    mods = {}
+    for i in range(3000):
+        db = rng.choice([64, 128, 500])
+        if db not in mods:
+            mods[db] = [Modulus(rng.getrandbits(db) | (1 << (db - 1)), 2 * db) for _ in range(4)]
+        M = rng.choice(mods[db])
+        pairs.append((M, rng.getrandbits(2 * db) * rng.choice([-1, 1])))
+    assert mod_batch(pairs) == [x % M.d for (M, x) in pairs]
