@@ -54,8 +54,13 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
     v[j] = rf_canon(r, p);
   }
   // Mixed radix to binary: X = v_{K-1}; X = X p_j + v_j for j = K-2 .. 0, in
-  // 32-bit limbs so that every limb step is one v_mad_u64_u32 (32 x 32 + 64).
-  // p_j = phi 2^32 + plo with phi < 2^16.  Before step j, X < 2^(48 (K-1-j)).
+  // 32-bit limbs.  p_j = phi 2^32 + plo with phi < 2^16 and, as p_j = 1 mod
+  // 2^24, plo <= 2^32 - 2^24 + 1.  One pass per step, column by column:
+  //   t = x_{l-1} phi + c;  t = x_l plo + t;  x_l = t mod 2^32;  c = t >> 32,
+  // starting from c = v_j (< 2^48): t < (2^32 - 1)(2^32 - 2^24 + 1) + 2^49
+  // < 2^64, so each column is two v_mad_u64_u32 whose 64-bit addend is the
+  // previous one's result.  Before step j, X < 2^(48 (K-1-j)) has nl limbs;
+  // after it, nl + 2 limbs hold X p_j + v_j.
   constexpr int W = 2 * S;
   uint32_t x[W];
 #pragma unroll
@@ -70,41 +75,21 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
     const int nl = (48 * (K - 1 - j) + 31) / 32;  // limbs of X before the step
     const uint64_t pj = C->pu[j];
     const uint32_t plo = (uint32_t)pj, phi = (uint32_t)(pj >> 32);
-    const uint64_t vj = (uint64_t)v[j];
-    // y = X plo + v_j: the low 32 bits of v_j enter at limb 0, the high 16 at
-    // limb 1, so every partial sum stays below 2^64
-    uint32_t y[W];
-    uint64_t c = (uint32_t)vj;
+    uint64_t c = (uint64_t)v[j];
+    uint32_t prev = 0;
 #pragma unroll
     for (int l = 0; l < W; l++) {
-      if (l < nl) {
-        const uint64_t t = (uint64_t)x[l] * plo + c + (l == 1 ? (vj >> 32) : 0);
-        y[l] = (uint32_t)t;
+      if (l <= nl) {
+        const uint32_t xl = l < nl ? x[l] : 0u;
+        uint64_t t = (uint64_t)prev * phi + c;
+        t = (uint64_t)xl * plo + t;
+        prev = xl;
+        x[l] = (uint32_t)t;
         c = t >> 32;
-      } else if (l == nl) {
-        y[l] = (uint32_t)c;  // nl >= 2, so v_j's high part went in at limb 1
-      } else {
-        y[l] = 0;
+      } else if (l == nl + 1) {
+        x[l] = (uint32_t)c;
       }
     }
-    // X' = y + X phi 2^32, into z (x is still read while z is written)
-    uint32_t z[W];
-    uint64_t d = 0;
-    z[0] = y[0];
-#pragma unroll
-    for (int l = 0; l + 1 < W; l++) {
-      if (l < nl) {
-        const uint64_t t = (uint64_t)x[l] * phi + y[l + 1] + d;
-        z[l + 1] = (uint32_t)t;
-        d = t >> 32;
-      } else if (l == nl) {
-        z[l + 1] = (uint32_t)d;
-      } else {
-        z[l + 1] = 0;
-      }
-    }
-#pragma unroll
-    for (int l = 0; l < W; l++) x[l] = z[l];
   }
   uint64_t X64[S];
 #pragma unroll
