@@ -183,31 +183,62 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
   // tid = H - 1); its inputs X_{i-M+1..i} and the sign term X_{i - S/t}
   // all lie at or above j0 + 1 because H = M + 2.
   const int i = digit0 + tid - H;
-  // pass 1: carry out E_i of each digit on its own
+  // pass 1: each digit on its own: its sum V_i (limbs 1.. written out now,
+  // limb 0 kept) and carry out E_i
+  const uint32_t ndig = (tk.rn + t - 1) / t;
+  const bool own = tid >= H && (uint32_t)i < ndig;
+  uint64_t* dst = tk.out + (size_t)(own ? i : 0) * t;
+  const uint32_t lim = own ? tk.rn - (uint32_t)i * t : 0u;  // limbs of the digit inside the output
+  uint64_t v0 = 0;
+  bool hz = true, ho = true;  // limbs 1 .. t-1 all zero / all ones
   {
-    auto none = [](int, uint64_t) {};
-    E[tid] = (tid >= H - 1 && i >= 0) ? digit_sum<S>(X, j0, i, t, M, 0, none) : 0;
+    auto emit = [&](int l, uint64_t v) {
+      if (l == 0) {
+        v0 = v;
+      } else {
+        hz &= (v == 0);
+        ho &= (v == ~(uint64_t)0);
+        if ((uint32_t)l < lim) dst[l] = v;
+      }
+    };
+    E[tid] = (tid >= H - 1 && i >= 0) ? digit_sum<S>(X, j0, i, t, M, 0, emit) : 0;
   }
   __syncthreads();
-  // pass 2: U_i = (V_i + E_{i-1}) mod 2^(64 t), F_i = carry - E_i in {-1, 0, 1}
-  const uint32_t ndig = (tk.rn + t - 1) / t;
+  // pass 2: U_i = (V_i + E_{i-1}) mod 2^(64 t) and F_i = its carry in
+  // {-1, 0, 1} (the carry out is E_i + F_i); the signed carry-in almost
+  // always stops at limb 0
   uint32_t f = FN_ID;
-  if (tid >= H && (uint32_t)i < ndig) {
-    bool all0 = true, all1 = true;
-    uint64_t* dst = tk.out + (size_t)i * t;
-    const uint32_t lim = tk.rn - (uint32_t)i * t;
-    auto emit = [&](int l, uint64_t v) {
-      all0 &= (v == 0);
-      all1 &= (v == ~(uint64_t)0);
-      if ((uint32_t)l < lim) dst[l] = v;
-    };
-    const int64_t co = digit_sum<S>(X, j0, i, t, M, E[tid - 1], emit);
-    const int F = (int)(co - E[tid]);
+  if (own) {
+    int64_t c = E[tid - 1];
+    const uint64_t r0 = v0 + (uint64_t)c;
+    c = c >= 0 ? (int64_t)(r0 < v0) : -(int64_t)(r0 > v0);
+    if (lim > 0) dst[0] = r0;
+    bool all0 = r0 == 0 && hz, all1 = r0 == ~(uint64_t)0 && ho;
+    if (c != 0) {
+      // propagate through limbs 1 .. (this thread wrote them in pass 1); a
+      // digit cut short by the output's end has no later digits, so its
+      // carry function is never used
+      all0 = r0 == 0;
+      all1 = r0 == ~(uint64_t)0;
+      for (int l = 1; l < t; l++) {
+        if ((uint32_t)l >= lim) break;
+        const uint64_t a = dst[l];
+        uint64_t r = a;
+        if (c != 0) {
+          r = a + (uint64_t)c;
+          c = c > 0 ? (int64_t)(r < a) : -(int64_t)(r > a);
+          dst[l] = r;
+        }
+        all0 &= (r == 0);
+        all1 &= (r == ~(uint64_t)0);
+      }
+    }
+    const int F = (int)c;
     f = 0;
 #pragma unroll
-    for (int c = -1; c <= 1; c++) {
-      int o = F + ((c == -1 && all0) ? -1 : 0) + ((c == 1 && all1) ? 1 : 0);
-      f |= (uint32_t)(o + 1) << (2 * (c + 1));
+    for (int cc = -1; cc <= 1; cc++) {
+      int o = F + ((cc == -1 && all0) ? -1 : 0) + ((cc == 1 && all1) ? 1 : 0);
+      f |= (uint32_t)(o + 1) << (2 * (cc + 1));
     }
   }
   code[tid] = f;
