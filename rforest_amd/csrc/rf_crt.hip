@@ -160,7 +160,7 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
   constexpr int S = (48 * K + 1 + 63) / 64;
   __shared__ uint64_t X[RF_CRT_THREADS * S];
   __shared__ int64_t E[RF_CRT_THREADS];
-  __shared__ uint32_t code[RF_CRT_THREADS];
+  __shared__ uint32_t code[RF_CRT_THREADS / 64];
   const int tid = threadIdx.x;
   const RfCrtTask tk = rf_wave_uniform_load(tasks + blockIdx.x);
   const int t = tk.t, M = (S + t - 1) / t, H = tk.halo;
@@ -241,19 +241,22 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
       f |= (uint32_t)(o + 1) << (2 * (cc + 1));
     }
   }
-  code[tid] = f;
-  __syncthreads();
-  // inclusive scan by composition: code[tid] = f_tid o ... o f_H
-#pragma unroll 1
-  for (int off = 1; off < RF_CRT_THREADS; off <<= 1) {
-    uint32_t g = code[tid];
-    if (tid >= off) g = fn_compose(g, code[tid - off]);
-    __syncthreads();
-    code[tid] = g;
-    __syncthreads();
+  // Scan by composition: inclusive f_tid o ... o f_0 within each wave by
+  // shuffles, then each wave composes the earlier waves' aggregates.
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t g = __shfl_up(f, off);
+    if (lane >= off) f = fn_compose(f, g);
   }
-  pre[tk.pre_off + tid] = (uint8_t)(tid == 0 ? FN_ID : code[tid - 1]);
-  if (tid == RF_CRT_THREADS - 1) fun[tk.fun_idx] = (uint8_t)code[tid];
+  if (lane == 63) code[wave] = f;
+  __syncthreads();
+  uint32_t before = FN_ID;  // composition of the earlier waves
+  for (int w = 0; w < wave; w++) before = fn_compose(code[w], before);
+  const uint32_t incl = fn_compose(f, before);
+  const uint32_t prev = __shfl_up(incl, 1);
+  pre[tk.pre_off + tid] = (uint8_t)(lane == 0 ? before : prev);
+  if (tid == RF_CRT_THREADS - 1) fun[tk.fun_idx] = (uint8_t)incl;
 }
 
 // Adds each digit's resolved carry into the output limbs.  The carry into
