@@ -85,3 +85,13 @@ def test_large_batch_of_small_remainders():
         M = rng.choice(mods[db])
         pairs.append((M, rng.getrandbits(2 * db) * rng.choice([-1, 1])))
     assert mod_batch(pairs) == [x % M.d for (M, x) in pairs]
+
+
+def test_matrix_entries_modulo_one_modulus():
+    """A 3x3 matrix of 2-Mbit signed entries reduced modulo one 1-Mbit m in
+    one call: every entry of the batch names the same prepared modulus."""
+    rng = random.Random(13)
+    m = rng.getrandbits(1 << 20) | (1 << ((1 << 20) - 1))
+    M = Modulus(m, 1 << 21)
+    X = [rng.getrandbits(1 << 21) * rng.choice([-1, 1]) for _ in range(9)]
+    assert mod_batch([(M, x) for x in X]) == [x % m for x in X]
