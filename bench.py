@@ -70,18 +70,21 @@ def cpu_baseline(products, gpu_results, budget_s):
             return oracle.rmatrix_mult(A, r, B, d, pl)
         kind = "port"
         desc = "C restatement of the reference path (oracle/rf_oracle.c)"
-    n, el, mismatches = 0, 0.0, 0
-    while n < 2 or el < budget_s:
+    n, el, mismatches, lat = 0, 0.0, 0, []
+    while n < 5 or el < budget_s:
         i = n % len(products)
         r, d, A, B = products[i]
         t0 = time.perf_counter()
         C = fn(A, r, B, d)
-        el += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        el += dt
+        lat.append(dt)
         if n < len(products):
             mismatches += int(C != gpu_results[i])
         n += 1
     distinct = min(n, len(products))
     return {"value": n / el, "unit": "products/s", "cores": 1, "kind": kind,
+            "median_latency_ms": round(sorted(lat)[len(lat) // 2] * 1e3, 3),
             "sample": f"{n} products ({distinct} distinct) of the same workload, one thread, {el:.1f} s; {desc}",
             "gpu_results_identical": mismatches == 0}
 
@@ -385,6 +388,18 @@ def serving_section(rf, workloads, n_req, n_plain, n_dropin, n_open):
     for a in arrs:
         for m in a[1:]:
             m.close()
+    # the reference CPU path on a prefix of the stream (at most 200 requests
+    # or 15 s of one host core), extrapolated to the whole stream
+    fn = _cpu_fn()
+    n_cpu, t_cpu = 0, 0.0
+    while n_cpu < min(200, len(reqs)) and t_cpu < 15.0:
+        r, bits, signs, seed, _ = reqs[n_cpu]
+        vals = [serve.gen_value(b, sg, seed, e) for e, (b, sg) in enumerate(zip(bits, signs))]
+        t0 = time.perf_counter()
+        fn(vals[:r * r], r, vals[r * r:], r)
+        t_cpu += time.perf_counter() - t0
+        n_cpu += 1
+    cpu_rate = n_cpu / t_cpu
     # open loop at 0.7 x the plain capacity: latency of served vs plain
     rate = 0.7 * plain_rate
     op, _, _ = _run_stream(rf, serve, reqs[:n_open], served_cfg, rate)
@@ -398,6 +413,9 @@ def serving_section(rf, workloads, n_req, n_plain, n_dropin, n_open):
             "plain_sample": f"first {n_plain} requests, same server with one slot and one product per batch",
             "dropin_products_per_s": round(dropin_rate, 1),
             "dropin_sample": f"first {n_dropin} requests, host mpz_rmatrix_mult_fft (PCIe included)",
+            "cpu_reference_products_per_s": round(cpu_rate, 2),
+            "cpu_reference_sample": f"first {n_cpu} requests on one host core ({t_cpu:.1f} s), the reference CPU path; "
+                                    f"extrapolated to the whole stream: {len(reqs) / cpu_rate:.0f} s",
             "speedup_vs_plain": round(served_rate / plain_rate, 2),
             "speedup_vs_dropin": round(served_rate / dropin_rate, 2),
             "batches": st["batches"], "mean_batch": round(st["products_in_batches"] / max(st["batches"], 1), 1),
@@ -505,6 +523,8 @@ def main():
     ntt_ms = prof["fwd"][0] + prof["inv"][0]
     ntt_launch_s = ntt_ms / psteps / 1e3
     ntt_ach = (fwd_bytes + inv_bytes) / ntt_launch_s / 1e9
+    # secondary figure: radix-2 butterflies, (N/2) lgN per transform and prime
+    bfly = (4 + 4 + 4) * k * (N // 2) * lgN
     stage_ms = {s: round(prof[s][0] / psteps, 4) for s in prof}
 
     dropin = None
@@ -558,7 +578,11 @@ def main():
                      "traffic": traffic, "alg_bytes_per_launch": alg[dom],
                      "avg_launch_ms": round(avg_s * 1e3, 5)},
         "roofline_ntt_passes": {"achieved": round(ntt_ach, 2), "frac": round(ntt_ach / HBM_PEAK_GBS, 4),
-                                "alg_bytes_per_product": (fwd_bytes + inv_bytes) // args.batch},
+                                "alg_bytes_per_product": (fwd_bytes + inv_bytes) // args.batch,
+                                "alg_bytes_per_product_reference_params": 25165824,
+                                "frac_reference_params": round(25165824 * args.batch / ntt_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                                "butterflies_per_product": bfly,
+                                "gbutterflies_per_s": round(bfly * args.batch / ntt_launch_s / 1e9, 2)},
         "stage_ms_per_step": stage_ms,
         "dropin_pcie_products_per_s": dropin,
         "cpu_baseline": cpu,
