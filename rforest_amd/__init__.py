@@ -5,7 +5,15 @@ Python mirror of the reference's matrix-product interface
 (C = A B with A r x d and B d x d, row-major lists of Python ints) and
 ``mpz_rmatrix_gpu_fft_suitable``, all calling the C ABI of
 librforest_hip.so through ctypes.  The library is required: nothing here
-computes a product on the CPU.
+computes a product on the CPU (``mpz_rmatrix_mult_naive`` is the
+reference's schoolbook helper, src/include/hwmpz.h:117-124, kept for
+checking).
+
+Beside it: ``device`` (device-resident batches, resident operands),
+``tree_product`` (product trees in device memory), ``fft`` (the mpzfft
+transform-domain interface), ``mod`` (remainders), ``serve`` (the request
+server), ``tune`` (the plan autotuner), ``workloads`` (the seeded inputs of
+the benchmarks) and ``stages`` (per-stage entry points for parity tests).
 """
 import ctypes
 import os

@@ -865,7 +865,7 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
   fwd_store<LGN>(a, v, tid, p, pinv);
 }
 
-// Forward middle pass of transforms above 2^17 points (N1 = lgN - 13 > 4
+// Forward middle pass of transforms above 2^(13+4) points (N1 = lgN - 13 > 4
 // row bits): after the column pass over the top N1 - NB row bits, the next NB
 // stages act on row bits NB-1 .. 0.  A thread holds row (hi 2^NB + j) of one
 // column for j < 2^NB, hi being the processed row bits; at stage s of the pass
