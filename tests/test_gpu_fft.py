@@ -120,3 +120,19 @@ def test_errors():
         P1.value(1 << 1000)          # more bits than the bound
     with pytest.raises(rf.RfError):
         P1.value().add(P1.value(1), P2.value(1))   # different plans
+
+
+def test_longest_transforms_round_trip_and_product():
+    """Values whose plans need more than 2^17 points (three-pass forward,
+    row plus two column passes back) through the transform domain."""
+    rng = random.Random(21)
+    bits = 1 << 27
+    P = FftParams(2 * bits + 1, 1)
+    t, lg, k = P.plan()
+    assert lg > 17
+    x, y = _r(rng, bits), _r(rng, bits)
+    X, Y = P.value(x), P.value(y)
+    assert X.inverse() == x
+    Z = P.value().mul(X, Y)
+    from rforest_amd.gmp import rmatrix_mult_schoolbook
+    assert Z.inverse() == rmatrix_mult_schoolbook([x], 1, [y], 1)[0]

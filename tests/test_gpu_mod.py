@@ -95,3 +95,14 @@ def test_matrix_entries_modulo_one_modulus():
     M = Modulus(m, 1 << 21)
     X = [rng.getrandbits(1 << 21) * rng.choice([-1, 1]) for _ in range(9)]
     assert mod_batch([(M, x) for x in X]) == [x % m for x in X]
+
+
+def test_remainder_with_long_transforms():
+    """A 2^26-bit modulus and 2^27-bit dividends: both products of the
+    remainder need more than 2^17 points."""
+    from rforest_amd.gmp import mod_values
+    rng = random.Random(17)
+    d = rng.getrandbits(1 << 26) | (1 << ((1 << 26) - 1))
+    M = Modulus(d, 1 << 27)
+    xs = [rng.getrandbits(1 << 27) * s for s in (1, -1)]
+    assert [M.mod(x) for x in xs] == mod_values(xs, [d, d])
