@@ -171,7 +171,7 @@ void rf_operand_destroy(rf_operand* op);
 typedef struct rf_fft_params rf_fft_params;
 typedef struct rf_fft rf_fft;
 /* mpzfft_params_init (src/include/mpzfft.h:17-22); RF_ERR_UNSUPPORTED when no
-   plan within 2^21 points and 24 primes holds the bound. */
+   plan within 2^24 points and 24 primes holds the bound. */
 int rf_fft_params_create(uint64_t bits, unsigned terms, rf_fft_params** out);
 /* The plan: chunk width 64*t bits, transform 2^lgN, k primes. */
 int rf_fft_params_plan(const rf_fft_params* params, int* t, int* lgN, int* k);

@@ -223,6 +223,36 @@ def mpz_rmatrix_mult_fft(A, r, B, d):
     return out
 
 
+def rmatrix_mult_limb_arrays(A, r, B, d):
+    """mpz_rmatrix_mult_fft on entries given as (sign, uint64 limb array)
+    pairs (little-endian magnitudes), without Python integers: for entries
+    of billions of bits.  Returns r*d (sign, limb array) pairs, the arrays
+    views into one output buffer."""
+    import numpy as np
+    if len(A) != r * d or len(B) != d * d:
+        raise ValueError("A must have r*d entries and B d*d entries")
+
+    def pack(ents):
+        arrs = [np.asarray(v, dtype=np.uint64) if s else np.zeros(0, np.uint64) for s, v in ents]
+        ns = np.array([len(a) for a in arrs], dtype=np.uintp)
+        offs = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.uintp)
+        limbs = np.ascontiguousarray(np.concatenate(arrs + [np.zeros(1, np.uint64)]))
+        return limbs, offs, ns, np.array([s for s, _ in ents], dtype=np.intc)
+
+    a, ao, an, asg = pack(A)
+    b, bo, bn, bsg = pack(B)
+    cap = int(an.max()) + int(bn.max()) + 2
+    coff = np.arange(r * d, dtype=np.uintp) * cap
+    ccap = np.full(r * d, cap, dtype=np.uintp)
+    c = np.zeros(r * d * cap + 1, dtype=np.uint64)
+    cs = np.zeros(r * d, dtype=np.int_)
+    check(lib().rf_rmatrix_mult_limbs(_ptr(c, _u64p), _ptr(coff, _szp), _ptr(ccap, _szp), _ptr(cs, _longp),
+                                      _ptr(a, _u64p), _ptr(ao, _szp), _ptr(an, _szp), _ptr(asg, _intp),
+                                      _ptr(b, _u64p), _ptr(bo, _szp), _ptr(bn, _szp), _ptr(bsg, _intp),
+                                      r, d))
+    return [(int(np.sign(cs[e])), c[int(coff[e]):int(coff[e]) + abs(int(cs[e]))]) for e in range(r * d)]
+
+
 def mpz_matrix_mult_fft(A, B, d):
     """src/include/hwmpz.h:130 -- square case r = d."""
     return mpz_rmatrix_mult_fft(A, d, B, d)

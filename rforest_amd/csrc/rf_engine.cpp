@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -63,6 +64,17 @@ struct Ctx {
 Ctx g;
 std::mutex g_mu;
 thread_local std::string g_err;
+
+// Twiddle tables of order 2^RF_LGN_MAX for transforms above 2^RF_LGN_TW
+// points (RF_TW_PRIMES x (2^23 + 2^24) pairs, 9.7 GB): made on the first
+// plan that needs them and kept until hw_mpz_clear.  Server workers launch
+// without the engine mutex, so the pointers are atomics and g_tw_mu alone
+// serialises their creation.  Their entries agree with the resident tables'
+// on every index both hold (both come from powers of the same root of order
+// 2^RF_S), so a transform may mix row passes over one with column passes
+// over the other.
+std::mutex g_tw_mu;
+std::atomic<double*> g_twf_long{nullptr}, g_twi_long{nullptr};
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -113,20 +125,21 @@ int setup_locked() {
   if (!hc) return fail(RF_ERR_NOMEM, "host allocation failed");
   rf_consts_init(&g.primes, hc);
   double roots[2 * RF_TW_PRIMES] = {0};
-  rf_twiddle_roots(&g.primes, roots, roots + RF_TW_PRIMES);
+  rf_twiddle_roots(&g.primes, RF_LGN_TW, roots, roots + RF_TW_PRIMES);
   HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc((void**)&g.d_consts, sizeof(RfConsts)));
   HIPCHK(hipMemcpy(g.d_consts, hc, sizeof(RfConsts), hipMemcpyHostToDevice));
   free(hc);
   // Twiddle tables (RF_TW_PRIMES x (2^20 + 2^21) pairs, 1.1 GiB) are
-  // generated on the device from each prime's root of order 2^RF_LGN_MAX.
+  // generated on the device from each prime's root of order 2^RF_LGN_TW.
   const size_t twf_n = (size_t)RF_TW_PRIMES * RF_FWD_TW * 2, twi_n = (size_t)RF_TW_PRIMES * RF_INV_TW * 2;
   HIPCHK(hipMalloc((void**)&g.d_twf, twf_n * sizeof(double)));
   HIPCHK(hipMalloc((void**)&g.d_twi, twi_n * sizeof(double)));
   double* d_roots = nullptr;
   HIPCHK(hipMalloc((void**)&d_roots, sizeof roots));
   HIPCHK(hipMemcpy(d_roots, roots, sizeof roots, hipMemcpyHostToDevice));
-  HIPCHK(rf_launch_twiddles(g.d_consts, d_roots, d_roots + RF_TW_PRIMES, RF_TW_PRIMES, g.d_twf, g.d_twi, g.stream));
+  HIPCHK(rf_launch_twiddles(g.d_consts, d_roots, d_roots + RF_TW_PRIMES, RF_TW_PRIMES, RF_LGN_TW, g.d_twf, g.d_twi,
+                            g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipFree(d_roots));
   g.tables.consts = g.d_consts;
@@ -148,10 +161,54 @@ void clear_locked() {
   hipFree(g.d_consts);
   hipFree(g.d_twf);
   hipFree(g.d_twi);
+  {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    hipFree(g_twf_long.exchange(nullptr));
+    hipFree(g_twi_long.exchange(nullptr));
+  }
   if (g.d_arena) hipFree(g.d_arena);
   if (g.h_stage) hipHostFree(g.h_stage);
   hipStreamDestroy(g.stream);
   g = Ctx();
+}
+
+// The tables a launch passes to its kernels (long-table pointers null until
+// made).
+RfDevTables tables() {
+  RfDevTables t = g.tables;
+  t.tw_fwd_long = g_twf_long.load(std::memory_order_acquire);
+  t.tw_inv_long = g_twi_long.load(std::memory_order_acquire);
+  return t;
+}
+
+int ensure_long_tables(int lgN) {
+  if (lgN <= RF_LGN_TW || g_twi_long.load(std::memory_order_acquire)) return RF_OK;
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  if (g_twi_long.load(std::memory_order_acquire)) return RF_OK;
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  double roots[2 * RF_TW_PRIMES] = {0};
+  rf_twiddle_roots(&g.primes, RF_LGN_MAX, roots, roots + RF_TW_PRIMES);
+  const size_t twf_n = (size_t)RF_TW_PRIMES * RF_FWD_TW_LONG * 2, twi_n = (size_t)RF_TW_PRIMES * RF_INV_TW_LONG * 2;
+  double *f = nullptr, *i = nullptr, *d_roots = nullptr;
+  hipStream_t st = nullptr;
+  hipError_t e = hipMalloc((void**)&f, twf_n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&i, twi_n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_roots, sizeof roots);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_roots, roots, sizeof roots, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = rf_launch_twiddles(g.d_consts, d_roots, d_roots + RF_TW_PRIMES, RF_TW_PRIMES, RF_LGN_MAX, f, i, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (st) hipStreamDestroy(st);
+  hipFree(d_roots);
+  if (e != hipSuccess) {
+    hipFree(f);
+    hipFree(i);
+    return fail(e == hipErrorOutOfMemory ? RF_ERR_NOMEM : RF_ERR_HIP,
+                "twiddle tables for transforms of 2^%d points: %s", lgN, hipGetErrorString(e));
+  }
+  g_twf_long.store(f, std::memory_order_release);
+  g_twi_long.store(i, std::memory_order_release);
+  return RF_OK;
 }
 
 int ensure_arena(size_t bytes) {
@@ -269,12 +326,12 @@ int plan_product(Product& P) {
                   (unsigned long long)LB);
     return RF_OK;
   }
-  if (LA && LB && tuned_plan(LA, LB, P.r, P.d, &P.plan)) return RF_OK;
+  if (LA && LB && tuned_plan(LA, LB, P.r, P.d, &P.plan)) return ensure_long_tables(P.plan.lgN);
   if (rf_plan_choose(&g.primes, LA, LB, P.r, P.d, &P.plan) != 0)
     return fail(RF_ERR_UNSUPPORTED,
                 "no transform plan for entries of %llu x %llu bits (r=%d, d=%d)",
                 (unsigned long long)LA, (unsigned long long)LB, P.r, P.d);
-  return RF_OK;
+  return ensure_long_tables(P.plan.lgN);
 }
 
 uint64_t out_limbs_for(uint64_t la_bits, uint64_t lb_bits, int d) {
@@ -438,7 +495,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
     if (split)
       for (int e = P.left ? rd : 0; e < r * d + d * d; e++)
         for (int q = 0; q < k; q++)
-          for (uint32_t h = 0; h < rows; h++) frow.push_back(RfRowTask{spec_of(e, q) + h * rowlen, h, (uint16_t)q, 0});
+          for (uint32_t h = 0; h < rows; h++) frow.push_back(RfRowTask{spec_of(e, q) + h * rowlen, h, (uint16_t)q, (uint16_t)pl.lgN});
     for (int q = 0; q < k; q++)
       for (int i = 0; i < r; i++)
         for (int j = 0; j < d; j++) {
@@ -454,7 +511,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
             t.d = (uint16_t)d;
             inv[{split ? RF_LGN_ROW : pl.lgN, abase}].push_back(t);
           }
-          if (split) icol[pl.lgN].push_back(RfRowTask{out_off, 0, (uint16_t)q, 0});
+          if (split) icol[pl.lgN].push_back(RfRowTask{out_off, 0, (uint16_t)q, (uint16_t)pl.lgN});
         }
     for (int e = 0; e < r * d; e++)
       crt_tasks(pl, P.coef_off + (uint64_t)e * k * N, P.out[e], npre, nblk_total, crt[k]);
@@ -500,23 +557,23 @@ int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
     const char* t = S.tasks + l.off;
     switch (l.kind) {
       case K_FWD:  // param: fwd_key
-        HIPCHK(rf_launch_fwd(g.tables, l.param >> 4, l.param & 15, (const RfFwdTask*)t, l.n, S.spec, st));
+        HIPCHK(rf_launch_fwd(tables(), l.param >> 4, l.param & 15, (const RfFwdTask*)t, l.n, S.spec, st));
         break;
       case K_FWD_COL:
-        HIPCHK(rf_launch_fwd_col(g.tables, l.param >> 4, l.param & 15, (const RfFwdTask*)t, l.n, S.spec, st));
+        HIPCHK(rf_launch_fwd_col(tables(), l.param >> 4, l.param & 15, (const RfFwdTask*)t, l.n, S.spec, st));
         break;
       case K_FWD_ROW:
-        HIPCHK(rf_launch_fwd_row(g.tables, (const RfRowTask*)t, l.n, S.spec, st));
+        HIPCHK(rf_launch_fwd_row(tables(), (const RfRowTask*)t, l.n, S.spec, st));
         break;
       case K_INV:
-        HIPCHK(rf_launch_inv(g.tables, l.param, (const RfInvTask*)t, l.n, l.abase ? l.abase : S.spec, S.spec,
+        HIPCHK(rf_launch_inv(tables(), l.param, (const RfInvTask*)t, l.n, l.abase ? l.abase : S.spec, S.spec,
                              S.coef, st));
         break;
       case K_INV_COL:
-        HIPCHK(rf_launch_inv_col(g.tables, l.param, (const RfRowTask*)t, l.n, S.coef, st));
+        HIPCHK(rf_launch_inv_col(tables(), l.param, (const RfRowTask*)t, l.n, S.coef, st));
         break;
       case K_CRT:
-        HIPCHK(rf_launch_crt(g.tables, l.param, (const RfCrtTask*)t, l.n, S.coef, S.pre, S.fun, st));
+        HIPCHK(rf_launch_crt(tables(), l.param, (const RfCrtTask*)t, l.n, S.coef, S.pre, S.fun, st));
         break;
       case K_APPLY:
         HIPCHK(rf_launch_apply((const RfCrtTask*)t, l.n, S.pre, S.fun, st));
@@ -683,7 +740,8 @@ int engine_stage(size_t bytes, char** host) {
 
 std::mutex& engine_mutex() { return g_mu; }
 bool engine_ready() { return g.ready; }
-const RfDevTables& engine_tables() { return g.tables; }
+RfDevTables engine_tables() { return tables(); }
+int engine_long_tables(int lgN) { return ensure_long_tables(lgN); }
 const RfPrimes& engine_primes() { return g.primes; }
 
 }  // namespace rfi
@@ -1008,7 +1066,7 @@ int operand_transform(rf_operand* op, const std::vector<Entry>& A, size_t base) 
       t.t = (uint8_t)pl.t;
       t.scale = 0;
       ft.push_back(t);
-      for (uint32_t h = 0; h < rows; h++) rt.push_back(RfRowTask{t.spec_off + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, 0});
+      for (uint32_t h = 0; h < rows; h++) rt.push_back(RfRowTask{t.spec_off + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, (uint16_t)lgN});
     }
   const size_t fb = align256(ft.size() * sizeof(RfFwdTask)), rb = align256(rt.size() * sizeof(RfRowTask));
   int rc = ensure_arena(base + fb + rb);
@@ -1022,10 +1080,10 @@ int operand_transform(rf_operand* op, const std::vector<Entry>& A, size_t base) 
   RfRowTask* d_rt = (RfRowTask*)(g.d_arena + base + fb);
   HIPCHK(hipMemcpyAsync(d_ft, g.h_stage, fb + rb, hipMemcpyHostToDevice, g.stream));
   if (split) {
-    HIPCHK(rf_launch_fwd_col(g.tables, lgN, pl.t, d_ft, (int)ft.size(), op->spec, g.stream));
-    HIPCHK(rf_launch_fwd_row(g.tables, d_rt, (int)rt.size(), op->spec, g.stream));
+    HIPCHK(rf_launch_fwd_col(tables(), lgN, pl.t, d_ft, (int)ft.size(), op->spec, g.stream));
+    HIPCHK(rf_launch_fwd_row(tables(), d_rt, (int)rt.size(), op->spec, g.stream));
   } else {
-    HIPCHK(rf_launch_fwd(g.tables, lgN, pl.t, d_ft, (int)ft.size(), op->spec, g.stream));
+    HIPCHK(rf_launch_fwd(tables(), lgN, pl.t, d_ft, (int)ft.size(), op->spec, g.stream));
   }
   HIPCHK(hipStreamSynchronize(g.stream));
   return RF_OK;
@@ -1047,6 +1105,7 @@ int operand_make(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_
         rf_plan_choose(&g.primes, op->la_bits, b_max_bits, r, d, &op->plan) != 0)
       rc = fail(RF_ERR_UNSUPPORTED, "no transform plan for a %dx%d operand of %llu bits and right entries of %llu bits",
                 r, d, (unsigned long long)op->la_bits, (unsigned long long)b_max_bits);
+    if (!rc) rc = ensure_long_tables(op->plan.lgN);
     if (!rc) {
       const size_t bytes = (size_t)r * d * op->plan.k * ((size_t)8 << op->plan.lgN);
       hipError_t e = hipMalloc((void**)&op->spec, bytes);
@@ -1487,6 +1546,8 @@ int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs,
   if (n <= 0 || t < 1 || t > RF_TMAX || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_TW_PRIMES ||
       (scale != 0 && scale != lgN) || nchunk > (1u << lgN))
     return fail(RF_ERR_ARG, "rf_stage_fwd: bad arguments");
+  rc = ensure_long_tables(lgN);
+  if (rc) return rc;
   const size_t N = (size_t)1 << lgN;
   size_t total = 0;
   for (int e = 0; e < n; e++) total = std::max(total, off[e] + nlimbs[e]);
@@ -1520,14 +1581,14 @@ int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs,
   RfRowTask* d_row = (RfRowTask*)((char*)d_task + align256(n * sizeof(RfFwdTask)));
   for (int e = 0; e < n; e++)
     for (uint32_t h = 0; h < rows; h++)
-      hr[(size_t)e * rows + h] = RfRowTask{(uint64_t)e * N + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, 0};
+      hr[(size_t)e * rows + h] = RfRowTask{(uint64_t)e * N + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, (uint16_t)lgN};
   HIPCHK(hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream));
   HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
   if (split) {
-    HIPCHK(rf_launch_fwd_col(g.tables, lgN, t, d_task, n, d_spec, g.stream));
-    HIPCHK(rf_launch_fwd_row(g.tables, d_row, n * (int)rows, d_spec, g.stream));
+    HIPCHK(rf_launch_fwd_col(tables(), lgN, t, d_task, n, d_spec, g.stream));
+    HIPCHK(rf_launch_fwd_row(tables(), d_row, n * (int)rows, d_spec, g.stream));
   } else {
-    HIPCHK(rf_launch_fwd(g.tables, lgN, t, d_task, n, d_spec, g.stream));
+    HIPCHK(rf_launch_fwd(tables(), lgN, t, d_task, n, d_spec, g.stream));
   }
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(g.h_stage, d_spec, (size_t)n * N * 8, hipMemcpyDeviceToHost));
@@ -1542,6 +1603,8 @@ int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int
   if (rc) return rc;
   if (r <= 0 || d <= 0 || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || q < 0 || q >= RF_TW_PRIMES)
     return fail(RF_ERR_ARG, "rf_stage_inv: bad arguments");
+  rc = ensure_long_tables(lgN);
+  if (rc) return rc;
   const size_t N = (size_t)1 << lgN;
   const size_t np = (size_t)(r * d + d * d);
   const bool split = lgN > RF_LGN_LDS;
@@ -1585,15 +1648,15 @@ int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int
         tk.d = (uint16_t)d;
         ht[(size_t)(i * d + j) * rows + h] = tk;
       }
-      hc[i * d + j] = RfRowTask{(uint64_t)(i * d + j) * N, 0, (uint16_t)q, 0};
+      hc[i * d + j] = RfRowTask{(uint64_t)(i * d + j) * N, 0, (uint16_t)q, (uint16_t)lgN};
     }
   HIPCHK(hipMemcpyAsync(d_spec, hs, np * N * 8, hipMemcpyHostToDevice, g.stream));
   HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
   if (split) {
-    HIPCHK(rf_launch_inv(g.tables, RF_LGN_ROW, d_task, (int)ninv, d_spec, d_spec, d_coef, g.stream));
-    HIPCHK(rf_launch_inv_col(g.tables, lgN, d_col, r * d, d_coef, g.stream));
+    HIPCHK(rf_launch_inv(tables(), RF_LGN_ROW, d_task, (int)ninv, d_spec, d_spec, d_coef, g.stream));
+    HIPCHK(rf_launch_inv_col(tables(), lgN, d_col, r * d, d_coef, g.stream));
   } else {
-    HIPCHK(rf_launch_inv(g.tables, lgN, d_task, r * d, d_spec, d_spec, d_coef, g.stream));
+    HIPCHK(rf_launch_inv(tables(), lgN, d_task, r * d, d_spec, d_spec, d_coef, g.stream));
   }
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(g.h_stage, d_coef, (size_t)r * d * N * 8, hipMemcpyDeviceToHost));
@@ -1653,7 +1716,7 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
   HIPCHK(hipMemcpyAsync(d_coef, hs, (size_t)k * nout * 8, hipMemcpyHostToDevice, g.stream));
   HIPCHK(hipMemcpyAsync(d_task, ht, task_b, hipMemcpyHostToDevice, g.stream));
   const RfCrtTask* tasks = (const RfCrtTask*)d_task;
-  HIPCHK(rf_launch_crt(g.tables, k, tasks, (int)nblk, d_coef, d_pre, d_fun, g.stream));
+  HIPCHK(rf_launch_crt(tables(), k, tasks, (int)nblk, d_coef, d_pre, d_fun, g.stream));
   HIPCHK(rf_launch_apply(tasks, (int)nblk, d_pre, d_fun, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
   HIPCHK(hipMemcpy(out, d_out, rn * 8, hipMemcpyDeviceToHost));

@@ -87,7 +87,7 @@ int forward_locked(rf_fft* f, const uint64_t* dptr, uint64_t nlimbs, int sign, s
     t.scale = (uint8_t)lgN;  // the value holds 2^-lgN times its transform
     ft.push_back(t);
     for (uint32_t h = 0; h < rows; h++)
-      rt.push_back(RfRowTask{t.spec_off + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, 0});
+      rt.push_back(RfRowTask{t.spec_off + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, (uint16_t)pl.lgN});
   }
   const size_t fb = align256(ft.size() * sizeof(RfFwdTask)), rb = align256(rt.size() * sizeof(RfRowTask));
   char* d_base = nullptr;
@@ -100,7 +100,7 @@ int forward_locked(rf_fft* f, const uint64_t* dptr, uint64_t nlimbs, int sign, s
   RfFwdTask* d_ft = (RfFwdTask*)(d_base + base);
   RfRowTask* d_rt = (RfRowTask*)(d_base + base + fb);
   HIPCHK(hipMemcpyAsync(d_ft, h_base + hbase, fb + rb, hipMemcpyHostToDevice, st));
-  const RfDevTables& tb = engine_tables();
+  const RfDevTables tb = engine_tables();
   if (split) {
     HIPCHK(rf_launch_fwd_col(tb, lgN, pl.t, d_ft, k, f->spec, st));
     HIPCHK(rf_launch_fwd_row(tb, d_rt, (int)rt.size(), f->spec, st));
@@ -131,7 +131,7 @@ int inverse_locked(const rf_fft* op, uint64_t* out, size_t base) {
       t.d = 0;  // the one stored spectrum, no product
       it.push_back(t);
     }
-    if (split) ct.push_back(RfRowTask{(uint64_t)q * N, 0, (uint16_t)q, 0});
+    if (split) ct.push_back(RfRowTask{(uint64_t)q * N, 0, (uint16_t)q, (uint16_t)pl.lgN});
   }
   std::vector<RfCrtTask> crt;
   uint32_t npre = 0, nblk = 0;
@@ -156,7 +156,7 @@ int inverse_locked(const rf_fft* op, uint64_t* out, size_t base) {
   uint8_t* pre = (uint8_t*)(d_tasks + tasks_b);
   uint8_t* fun = pre + pre_b;
   HIPCHK(hipMemcpyAsync(d_tasks, h_base, tasks_b, hipMemcpyHostToDevice, st));
-  const RfDevTables& tb = engine_tables();
+  const RfDevTables tb = engine_tables();
   HIPCHK(rf_launch_inv(tb, split ? RF_LGN_ROW : lgN, (const RfInvTask*)d_tasks, (int)it.size(), op->spec, op->spec,
                        coef, st));
   if (split) HIPCHK(rf_launch_inv_col(tb, lgN, (const RfRowTask*)(d_tasks + ib), (int)ct.size(), coef, st));
@@ -213,6 +213,8 @@ int rf_fft_create(const rf_fft_params* params, rf_fft** out) {
   RF_FFT_ENTER();
   if (!params || !out) return fail(RF_ERR_ARG, "rf_fft_create: null argument");
   *out = nullptr;
+  const int rc = engine_long_tables(params->plan.lgN);
+  if (rc) return rc;
   rf_fft* f = new rf_fft();
   f->plan = params->plan;
   f->bits = params->bits;

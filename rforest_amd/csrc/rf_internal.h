@@ -87,7 +87,11 @@ struct ScratchPtrs {
 // The library's global lock, its readiness and its device tables.
 std::mutex& engine_mutex();
 bool engine_ready();
-const RfDevTables& engine_tables();
+// Device tables for a launch (a copy: long-table pointers may appear later).
+RfDevTables engine_tables();
+// Makes the twiddle tables transforms above 2^RF_LGN_TW points need (no-op
+// below); callable without the engine mutex.
+int engine_long_tables(int lgN);
 const RfPrimes& engine_primes();
 // A resident operand from entries already in device memory (task tables at
 // arena byte `base`), the pipeline of planned products run on the engine's

@@ -39,16 +39,16 @@ struct RfInvTask {
 
 // Transforms longer than 2^RF_LGN_LDS points, N = 2^(13 + n1), view the
 // array as 2^n1 rows of 2^13 (i = row 2^13 + col).  Forward: a column pass
-// (pack + the first n1 stages, one column per thread; for n1 > 4 the first
-// n1 - 4, then a middle pass for the next 4) then a row pass (the last 13
-// stages of each row in LDS).  Inverse: a row pass (pointwise product + the
+// (pack + the first stages, one column per thread: all n1 for n1 <= 4,
+// otherwise the top n1 mod 4 (or 4), then middle passes of 4 row bits each)
+// then a row pass (the last 13 stages of each row in LDS).  Inverse: a row pass (pointwise product + the
 // first 13 stages per row, an RfInvTask per row) then column passes over the
 // row bits (4 at a time).  All passes work in place on the same array.
 struct RfRowTask {
   uint64_t off;       // first residue of the row (forward rows) or of the array (inverse columns)
   uint32_t row;       // row index (forward rows; its bits precede the row's own in the twiddle index)
   uint16_t q;
-  uint16_t pad;
+  uint16_t lg;        // log2 of the whole transform's length (forward rows: picks the twiddle table)
 };
 #define RF_COL_THREADS 256
 
@@ -101,14 +101,17 @@ struct RfDevTables {
   const RfConsts* consts;
   const double* tw_fwd;  // RF_TW_PRIMES * RF_FWD_TW * 2
   const double* tw_inv;  // RF_TW_PRIMES * RF_INV_TW * 2
+  const double* tw_fwd_long;  // RF_TW_PRIMES * RF_FWD_TW_LONG * 2 (null until a transform above 2^RF_LGN_TW)
+  const double* tw_inv_long;  // RF_TW_PRIMES * RF_INV_TW_LONG * 2
 };
 
 // Every task of one forward launch has the same chunk width t (the kernels
 // for lengths from 2^RF_FWD_TSPEC on are specialised on it).
 #define RF_FWD_TSPEC 11
-// Fills the twiddle tables of the first nprimes primes (RF_TW_PRIMES at most)
-// on the device: om[q], omi[q] = omega of order 2^RF_LGN_MAX and its inverse.
-hipError_t rf_launch_twiddles(const RfConsts* consts, const double* om, const double* omi, int nprimes,
+// Fills the twiddle tables (maximum length 2^lgmax) of the first nprimes
+// primes (RF_TW_PRIMES at most) on the device: om[q], omi[q] = omega of order
+// 2^lgmax and its inverse.
+hipError_t rf_launch_twiddles(const RfConsts* consts, const double* om, const double* omi, int nprimes, int lgmax,
                               double* fwd, double* inv, hipStream_t st);
 hipError_t rf_launch_fwd(const RfDevTables& tb, int lgN, int t, const RfFwdTask* tasks, int ntasks,
                          double* spec, hipStream_t st);

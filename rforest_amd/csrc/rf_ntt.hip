@@ -739,28 +739,41 @@ __device__ __forceinline__ double tw_pow(double b, uint32_t e, double p, double 
 
 __global__ __launch_bounds__(256) void k_twiddles(const RfConsts* __restrict__ C, const double* __restrict__ om,
                                                    const double* __restrict__ omi, double* __restrict__ fwd,
-                                                   double* __restrict__ inv) {
+                                                   double* __restrict__ inv, int lgmax) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const int q = blockIdx.y;
   const double p = C->p[q], pinv = C->pinv[q];
-  if (i < (uint32_t)RF_FWD_TW) {
-    const uint32_t rev = __builtin_bitreverse32(i) >> (32 - (RF_LGN_MAX - 1));
+  const uint32_t nf = 1u << (lgmax - 1), ni = 1u << lgmax;
+  if (i < nf) {
+    const uint32_t rev = __builtin_bitreverse32(i) >> (32 - (lgmax - 1));
     const double w = tw_pow(om[q], rev, p, pinv);
-    double* F = fwd + ((size_t)q * RF_FWD_TW + i) * 2;
+    double* F = fwd + ((size_t)q * nf + i) * 2;
     F[0] = w;
     F[1] = w / p;
   }
-  if (i < (uint32_t)RF_INV_TW) {
+  if (i < ni) {
     double w = 0.0;
     if (i) {
       const int e = 31 - __builtin_clz(i);  // len = 2^e
       const uint32_t j = i - (1u << e);
-      w = tw_pow(omi[q], j << (RF_LGN_MAX - 1 - e), p, pinv);
+      w = tw_pow(omi[q], j << (lgmax - 1 - e), p, pinv);
     }
-    double* V = inv + ((size_t)q * RF_INV_TW + i) * 2;
+    double* V = inv + ((size_t)q * ni + i) * 2;
     V[0] = w;
     V[1] = i ? w / p : 0.0;
   }
+}
+
+// The twiddle table of prime q for a transform of 2^lgN points: the resident
+// table up to 2^RF_LGN_TW, the long one above (they agree on every index
+// both hold).
+__device__ __forceinline__ const double2* fwd_table(const RfDevTables& tb, int q, int lgN) {
+  return lgN > RF_LGN_TW ? (const double2*)(tb.tw_fwd_long + (size_t)q * RF_FWD_TW_LONG * 2)
+                         : (const double2*)(tb.tw_fwd + (size_t)q * RF_FWD_TW * 2);
+}
+__device__ __forceinline__ const double2* inv_table(const RfDevTables& tb, int q, int lgN) {
+  return lgN > RF_LGN_TW ? (const double2*)(tb.tw_inv_long + (size_t)q * RF_INV_TW_LONG * 2)
+                         : (const double2*)(tb.tw_inv + (size_t)q * RF_INV_TW * 2);
 }
 
 // ---- transforms longer than one workgroup: N = 2^(RF_LGN_ROW + N1) ----
@@ -853,7 +866,7 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
   const RfRowTask tk = rf_wave_uniform_load(tasks + wg);
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  const double2* tw = fwd_table(tb, tk.q, tk.lg);
   double* a = spec + tk.off;
   double v[S::E];
 #pragma unroll
@@ -865,15 +878,16 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
   fwd_store<LGN>(a, v, tid, p, pinv);
 }
 
-// Forward middle pass of transforms above 2^(13+4) points (N1 = lgN - 13 > 4
-// row bits): after the column pass over the top N1 - NB row bits, the next NB
-// stages act on row bits NB-1 .. 0.  A thread holds row (hi 2^NB + j) of one
-// column for j < 2^NB, hi being the processed row bits; at stage s of the pass
-// the pair (j, j + 2^(NB-1-s)) uses T[(hi << s) + (j >> (NB - s))] (the top
-// stage-count bits of the index).  Output in place, reduced.
+// Forward middle pass of transforms above 2^(13+4) points: after the passes
+// over the row bits above b0 + NB, the next NB stages act on row bits
+// b0 + NB - 1 .. b0.  A thread holds rows hi 2^(b0+NB) + j 2^b0 + lo of one
+// column for j < 2^NB, hi being the processed row bits and lo the ones still
+// to come; at stage s of the pass the pair (j, j + 2^(NB-1-s)) uses
+// T[(hi << s) + (j >> (NB - s))] (the top stage-count bits of the index; lo
+// does not enter).  Output in place, reduced.
 template <int NB>
 __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_mid(RfDevTables tb, const RfFwdTask* __restrict__ tasks,
-                                                            int ntasks, double* __restrict__ spec, int lgN) {
+                                                            int ntasks, double* __restrict__ spec, int lgN, int b0) {
   constexpr int J = 1 << NB;
   const int lgt = lgN - NB;                       // threads per transform: 2^lgt
   const int BPT = (1 << lgt) / RF_COL_THREADS;
@@ -881,14 +895,16 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_mid(RfDevTables tb, c
   if (w < 0) return;
   const RfFwdTask tk = rf_wave_uniform_load(tasks + w / BPT);
   const int th = (w % BPT) * RF_COL_THREADS + (int)threadIdx.x;
-  const int col = th & ((1 << RF_LGN_ROW) - 1), hi = th >> RF_LGN_ROW;
+  const int col = th & ((1 << RF_LGN_ROW) - 1);
+  const int lo = (th >> RF_LGN_ROW) & ((1 << b0) - 1), hi = th >> (RF_LGN_ROW + b0);
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
-  double* a = spec + tk.spec_off + ((size_t)hi << (RF_LGN_ROW + NB)) + col;
+  const double2* tw = fwd_table(tb, tk.q, lgN);
+  const size_t stride = (size_t)1 << (RF_LGN_ROW + b0);
+  double* a = spec + tk.spec_off + ((((size_t)hi << (b0 + NB)) + lo) << RF_LGN_ROW) + col;
   double v[J];
 #pragma unroll
-  for (int j = 0; j < J; j++) v[j] = a[(size_t)j << RF_LGN_ROW];
+  for (int j = 0; j < J; j++) v[j] = a[(size_t)j * stride];
 #pragma unroll
   for (int s = 0; s < NB; s++) {
     const int hf = J >> (s + 1);
@@ -906,7 +922,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_mid(RfDevTables tb, c
     }
   }
 #pragma unroll
-  for (int j = 0; j < J; j++) a[(size_t)j << RF_LGN_ROW] = rf_reduce(v[j], p, pinv);
+  for (int j = 0; j < J; j++) a[(size_t)j * stride] = rf_reduce(v[j], p, pinv);
 }
 
 // Inverse column pass over row bits b0 .. b0+NB-1: after the row pass (k_inv
@@ -930,7 +946,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
   const int lo = (th >> RF_LGN_ROW) & ((1 << b0) - 1), hi = th >> (RF_LGN_ROW + b0);
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
+  const double2* tw = inv_table(tb, tk.q, lgN);
   double* a = coef + tk.off + ((((size_t)hi << (b0 + NB)) + lo) << RF_LGN_ROW) + col;
   const size_t stride = (size_t)1 << (RF_LGN_ROW + b0);
   const int low = (lo << RF_LGN_ROW) + col;
@@ -1053,21 +1069,24 @@ static hipError_t launch_fwd_col_n(int t, dim3 grid, dim3 blk, const RfDevTables
   }
 }
 
-hipError_t rf_launch_twiddles(const RfConsts* consts, const double* om, const double* omi, int nprimes,
+hipError_t rf_launch_twiddles(const RfConsts* consts, const double* om, const double* omi, int nprimes, int lgmax,
                               double* fwd, double* inv, hipStream_t st) {
-  const dim3 grid((unsigned)((RF_INV_TW + 255) / 256), (unsigned)nprimes);
-  hipLaunchKernelGGL(k_twiddles, grid, dim3(256), 0, st, consts, om, omi, fwd, inv);
+  if (lgmax < 2 || lgmax > RF_LGN_MAX) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)(((1u << lgmax) + 255) / 256), (unsigned)nprimes);
+  hipLaunchKernelGGL(k_twiddles, grid, dim3(256), 0, st, consts, om, omi, fwd, inv, lgmax);
   return hipGetLastError();
 }
 
-// n1 = lgN - 13 row bits: up to 4 in one column pass (pack fused); above
-// that the column pass takes the top n1 - 4 and a middle pass the low 4.
+// n1 = lgN - 13 row bits: the pack-fused column pass takes the top
+// na = n1 - 4 floor((n1 - 1) / 4) (1 .. 4) of them, then middle passes take
+// 4 at a time down to bit 0.
 hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, int t, const RfFwdTask* tasks, int ntasks,
                              double* spec, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   const int n1 = lgN - RF_LGN_ROW;
   if (n1 < 1 || lgN > RF_LGN_MAX) return hipErrorInvalidValue;
-  const int na = n1 > 4 ? n1 - 4 : n1;  // bits of the pack-fused column pass
+  if (lgN > RF_LGN_TW && (!tb.tw_fwd_long || !tb.tw_inv_long)) return hipErrorInvalidValue;  // tables not made
+  const int na = n1 - 4 * ((n1 - 1) / 4);  // bits of the pack-fused column pass
   const int lgc = lgN - na;
   const dim3 grid(ntasks * ((1 << lgc) / RF_COL_THREADS)), blk(RF_COL_THREADS);
   hipError_t e;
@@ -1078,10 +1097,12 @@ hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, int t, const RfFwdT
     case 4: e = launch_fwd_col_n<4>(t, grid, blk, tb, tasks, ntasks, spec, lgc, st); break;
     default: return hipErrorInvalidValue;
   }
-  if (e != hipSuccess || n1 <= 4) return e;
   const dim3 grid2(ntasks * ((1 << (lgN - 4)) / RF_COL_THREADS));
-  hipLaunchKernelGGL(k_fwd_mid<4>, grid2, blk, 0, st, tb, tasks, ntasks, spec, lgN);
-  return hipGetLastError();
+  for (int b0 = n1 - na - 4; e == hipSuccess && b0 >= 0; b0 -= 4) {
+    hipLaunchKernelGGL(k_fwd_mid<4>, grid2, blk, 0, st, tb, tasks, ntasks, spec, lgN, b0);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int ntasks, double* spec,
@@ -1094,7 +1115,7 @@ hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int 
   return hipGetLastError();
 }
 
-// Row bits 0 .. min(n1, 4) - 1 first, then (n1 > 4) the rest in a second pass.
+// One inverse column pass over row bits b0 .. b0 + nb - 1.
 static hipError_t launch_inv_col_pass(const RfDevTables& tb, int lgN, int b0, int nb, const RfRowTask* tasks,
                                       int ntasks, double* coef, hipStream_t st) {
   const dim3 grid(ntasks * ((1 << (lgN - nb)) / RF_COL_THREADS)), blk(RF_COL_THREADS);
@@ -1115,7 +1136,9 @@ hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* ta
   if (ntasks <= 0) return hipSuccess;
   const int n1 = lgN - RF_LGN_ROW;
   if (n1 < 1 || lgN > RF_LGN_MAX) return hipErrorInvalidValue;
-  hipError_t e = launch_inv_col_pass(tb, lgN, 0, n1 < 4 ? n1 : 4, tasks, ntasks, coef, st);
-  if (e != hipSuccess || n1 <= 4) return e;
-  return launch_inv_col_pass(tb, lgN, 4, n1 - 4, tasks, ntasks, coef, st);
+  if (lgN > RF_LGN_TW && !tb.tw_inv_long) return hipErrorInvalidValue;  // tables not made
+  hipError_t e = hipSuccess;
+  for (int b0 = 0; e == hipSuccess && b0 < n1; b0 += 4)
+    e = launch_inv_col_pass(tb, lgN, b0, n1 - b0 < 4 ? n1 - b0 : 4, tasks, ntasks, coef, st);
+  return e;
 }

@@ -141,6 +141,7 @@ int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int 
   if (k == 0 || kmin > pr->k) return -1;
   if (k < kmin) k = kmin;  // more primes than the bound needs are always valid
   if (k > RF_KMAX) return -1;
+  if (((size_t)d * k << lgN) > 0xffffffffu) return -1;  // spectrum strides are 32-bit
   pl.k = k;
   pl.nout = (int)nout;
   // Coefficients lie in (-P/2, P/2) with P < 2^(48 k): 48 k + 1 bits of two's
@@ -165,6 +166,7 @@ int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int 
   // pass: one more read and write of every residue, and a few more ops.
   if (lgN > RF_LGN_LDS) bytes *= 2, ops *= 1.1;
   if (lgN > RF_LGN_ROW + 4) bytes *= 1.5;  // a second column pass
+  if (lgN > RF_LGN_ROW + 8) bytes *= 1.33;  // a third
   double c = ops > bytes * 39 / 8 ? ops : bytes * 39 / 8;
   if (lgN == RF_LGN_LDS) c *= 1.25;  // one workgroup per CU at 128 KiB of LDS
   pl.cost = c;
@@ -224,7 +226,8 @@ int rf_plan_general(const RfPrimes* pr, size_t bits, unsigned terms, RfPlan* out
     pl.nout = (int)points;
     pl.rn = (bits + 63) / 64 + 1;
     // residues moved per transform, the dominant cost of every operation
-    pl.cost = (double)k * (double)((size_t)1 << lgN) * (lgN > RF_LGN_LDS ? (lgN > RF_LGN_ROW + 4 ? 3.0 : 2.0) : 1.0);
+    const int passes = lgN <= RF_LGN_LDS ? 1 : 2 + (lgN - RF_LGN_ROW - 1) / 4;  // row pass + column passes
+    pl.cost = (double)k * (double)((size_t)1 << lgN) * passes;
     if (!found || pl.cost < best.cost) best = pl, found = true;
   }
   if (!found) return -1;
@@ -269,10 +272,10 @@ void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
   }
 }
 
-void rf_twiddle_roots(const RfPrimes* pr, double* om, double* omi) {
+void rf_twiddle_roots(const RfPrimes* pr, int lgmax, double* om, double* omi) {
   for (int q = 0; q < pr->k && q < RF_TW_PRIMES; q++) {
     const uint64_t p = pr->p[q];
-    const uint64_t w = rf_h_powmod(pr->g[q], (uint64_t)1 << (RF_S - RF_LGN_MAX), p);  // order 2^LGN_MAX
+    const uint64_t w = rf_h_powmod(pr->g[q], (uint64_t)1 << (RF_S - lgmax), p);  // order 2^lgmax
     om[q] = (double)w;
     omi[q] = (double)rf_h_invmod(w, p);
   }

@@ -20,7 +20,9 @@
 #define RF_LGN_MIN 1
 #define RF_LGN_LDS 14     // largest one-workgroup transform (2^14 doubles = 128 KiB of LDS)
 #define RF_LGN_ROW 13     // longer transforms: columns of 2^(lgN-13) in registers, rows of 2^13 in LDS
-#define RF_LGN_MAX 21     // largest transform: 2^8 rows of 2^13 (two column passes of <= 16 residues)
+#define RF_LGN_MAX 24     // largest transform: 2^11 rows of 2^13 (column passes of <= 16 residues)
+#define RF_LGN_TW 21      // twiddle tables kept from set-up on; longer transforms use tables of 2^RF_LGN_MAX
+                          // generated on first use
 #define RF_KMAX 24        // most primes a plan may use (the CRT kernel's instantiations)
 #define RF_TW_PRIMES RF_KMAX  // primes with twiddle tables
 #define RF_TMAX 8         // chunk width up to 8 limbs = 512 bits
@@ -101,14 +103,18 @@ struct RfConsts {
   double pk[RF_MAXP][RF_LGN_MAX + 1][RF_MAXWORDS][3];
 };
 
-// Twiddle tables, per prime, as (w, fl(w/p)) pairs:
-//   fwd[q][m] = omega^(rev_{RF_LGN_MAX-1}(m)), m < 2^(RF_LGN_MAX-1),
-//     omega of order 2^RF_LGN_MAX; the prefix of length N/2 serves length N.
-//   inv[q][len + j] = omega_{2 len}^(-j), len = 2^e < 2^RF_LGN_MAX, j < len.
-#define RF_FWD_TW (1 << (RF_LGN_MAX - 1))
-#define RF_INV_TW (1 << RF_LGN_MAX)
+// Twiddle tables, per prime, as (w, fl(w/p)) pairs, for a maximum length
+// 2^L (L = RF_LGN_TW for the resident tables, RF_LGN_MAX for the long ones):
+//   fwd[q][m] = omega^(rev_{L-1}(m)), m < 2^(L-1), omega of order 2^L; the
+//     prefix of length N/2 serves length N, so both tables agree where both
+//     exist.
+//   inv[q][len + j] = omega_{2 len}^(-j), len = 2^e < 2^L, j < len.
+#define RF_FWD_TW (1 << (RF_LGN_TW - 1))
+#define RF_INV_TW (1 << RF_LGN_TW)
+#define RF_FWD_TW_LONG (1 << (RF_LGN_MAX - 1))
+#define RF_INV_TW_LONG (1 << RF_LGN_MAX)
 
 void rf_consts_init(const RfPrimes* pr, RfConsts* c);
-// omega of order 2^RF_LGN_MAX and its inverse, per prime, as exact doubles
-// (the device generates the tables from them).
-void rf_twiddle_roots(const RfPrimes* pr, double* om, double* omi);
+// omega of order 2^lgmax and its inverse, per prime, as exact doubles (the
+// device generates the tables from them).
+void rf_twiddle_roots(const RfPrimes* pr, int lgmax, double* om, double* omi);

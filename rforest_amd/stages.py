@@ -43,6 +43,31 @@ def inv(specA, specB, r, d, lgN, q):
     return [[int(v) for v in out[e * N:(e + 1) * N]] for e in range(r * d)]
 
 
+def fwd_array(limbs, sign, t, nchunk, lgN, q, scale=0):
+    """One entry given as a numpy uint64 limb array; returns its 2^lgN
+    canonical spectrum values as a numpy uint64 array (for transforms too
+    long for Python lists)."""
+    import numpy as np
+    la = np.ascontiguousarray(np.concatenate([np.asarray(limbs, dtype=np.uint64), np.zeros(1, np.uint64)]))
+    oa, na, sa = _a([0], np.uintp), _a([len(limbs)], np.uintp), _a([sign], np.intc)
+    out = np.zeros(1 << lgN, dtype=np.uint64)
+    check(lib().rf_stage_fwd(la.ctypes.data_as(_u64p), oa.ctypes.data_as(_szp), na.ctypes.data_as(_szp),
+                             sa.ctypes.data_as(_intp), 1, t, nchunk, lgN, q, scale, out.ctypes.data_as(_u64p)))
+    return out
+
+
+def inv_array(specA, specB, r, d, lgN, q):
+    """inv() on numpy uint64 arrays: specA holds r*d spectra, specB d*d, each
+    of 2^lgN values, back to back; returns the r*d outputs the same way."""
+    import numpy as np
+    a = np.ascontiguousarray(specA, dtype=np.uint64)
+    b = np.ascontiguousarray(specB, dtype=np.uint64)
+    out = np.zeros(r * d << lgN, dtype=np.uint64)
+    check(lib().rf_stage_inv(a.ctypes.data_as(_u64p), b.ctypes.data_as(_u64p), r, d, lgN, q,
+                             out.ctypes.data_as(_u64p)))
+    return out
+
+
 def crt(res, nout, k, t, rn):
     """res: k*nout residues (prime-major).  Returns rn limbs (two's complement)."""
     a = _a(res)

@@ -125,17 +125,21 @@ def test_long_transform_plans():
 
 
 def test_unsupported_sizes_are_reported():
-    # 2^30-bit entries would need 2^22 points at the widest chunks
+    # 2^33-bit entries would need 2^25 points at the widest (512-bit) chunks
     with pytest.raises(rf.RfError) as e:
-        rf.plan(1 << 30, 1 << 30, 2, 2)
+        rf.plan(1 << 33, 1 << 33, 2, 2)
     assert e.value.code == rf.RF_ERR_UNSUPPORTED
 
 
 def test_longest_transform_plans():
-    """Entries up to 2^29 bits plan within 2^21 points, k <= 24 primes."""
-    for bits in (1 << 26, 3 << 26, 1 << 28, 1 << 29):
+    """Entries up to 2^32 bits plan within 2^24 points, k <= 24 primes; the
+    spectrum strides d k 2^lgN stay 32-bit."""
+    for bits in (1 << 26, 3 << 26, 1 << 28, 1 << 29, 1 << 30, 1 << 31, 1 << 32):
         t, lgN, k = rf.plan(bits, bits, 2, 2)
-        assert 17 < lgN <= 21 and k <= 24 and _bound_ok(t, lgN, k, bits, bits, 2)
+        assert 17 < lgN <= 24 and k <= 24 and _bound_ok(t, lgN, k, bits, bits, 2)
+        assert (2 * k) << lgN < 1 << 32
+    assert rf.plan(1 << 30, 1 << 30, 1, 1)[1] >= 22
+    assert rf.plan(1 << 32, 1 << 32, 1, 1)[1] == 24
 
 
 def test_gmp_schoolbook_matches_python_integers():
@@ -200,6 +204,7 @@ def test_general_transform_plan_bound_is_proven(bits, terms):
 def test_general_transform_plan_rejects_what_no_plan_holds():
     from rforest_amd.fft import FftParams
     with pytest.raises(rf.RfError):
-        FftParams(1 << 31, 1)
+        FftParams(1 << 34, 1)
+    assert FftParams(1 << 31, 1).plan()[1] >= 22
     with pytest.raises(rf.RfError):
         FftParams(0, 1)

@@ -136,3 +136,19 @@ def test_longest_transforms_round_trip_and_product():
     Z = P.value().mul(X, Y)
     from rforest_amd.gmp import rmatrix_mult_schoolbook
     assert Z.inverse() == rmatrix_mult_schoolbook([x], 1, [y], 1)[0]
+
+
+def test_transforms_beyond_2_21_points():
+    """Values whose plans need 2^22 points or more (twiddle tables of order
+    2^24): round trip and a product against GMP."""
+    rng = random.Random(22)
+    bits = 1 << 29
+    P = FftParams(2 * bits + 1, 1)
+    t, lg, k = P.plan()
+    assert lg >= 22
+    x, y = _r(rng, bits), _r(rng, bits)
+    X, Y = P.value(x), P.value(y)
+    assert X.inverse() == x
+    Z = P.value().mul(X, Y)
+    from rforest_amd.gmp import rmatrix_mult_schoolbook
+    assert Z.inverse() == rmatrix_mult_schoolbook([x], 1, [y], 1)[0]

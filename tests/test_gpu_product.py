@@ -69,6 +69,53 @@ def test_long_transform_products(bits, r, d):
     assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == naive(A, r, B, d)
 
 
+def test_product_at_2_22_points_vs_gmp():
+    """1x1 product of 2^30-bit entries: a plan of at least 2^22 points (three
+    column passes each way, the twiddle tables of order 2^24)."""
+    rng = random.Random(22)
+    bits = 1 << 30
+    assert rf.plan(bits, bits, 1, 1)[1] >= 22
+    A, B = _rand_product(rng, bits, 1, 1)
+    assert rf.mpz_rmatrix_mult_fft(A, 1, B, 1) == naive(A, 1, B, 1)
+
+
+def _limb_mods(v):
+    """x mod 2^64 - 1 and x mod 2^64 + 1 for x given by its uint64 limbs
+    (digit sums in base 2^64; 32-bit halves keep numpy's sums exact)."""
+    import numpy as np
+    lo, hi = v & np.uint64(0xffffffff), v >> np.uint64(32)
+    s = lambda a: int(a.sum(dtype=np.uint64))
+    tot = s(lo) + (s(hi) << 32)
+    alt = s(lo[0::2]) + (s(hi[0::2]) << 32) - s(lo[1::2]) - (s(hi[1::2]) << 32)
+    return tot % ((1 << 64) - 1), alt % ((1 << 64) + 1)
+
+
+def test_product_at_2_24_points_digit_sums():
+    """2x1 product of 2^32-bit entries (2^24 points, the longest plan), on
+    limb arrays: each result modulo 2^64 - 1 and 2^64 + 1 (so modulo
+    2^128 - 1) against the inputs' residues, its low 4096 limbs exactly, and
+    its sign and length."""
+    import numpy as np
+    bits = 1 << 32
+    r, d = 2, 1
+    assert rf.plan(bits, bits, r, d)[1] == 24
+    gen = np.random.default_rng(24)
+    n = bits // 64
+    a0 = gen.integers(0, 1 << 64, n, dtype=np.uint64, endpoint=False)
+    a1 = gen.integers(0, 1 << 64, n, dtype=np.uint64, endpoint=False)
+    a1[-1] |= np.uint64(1 << 63)  # top bit set
+    b0 = gen.integers(0, 1 << 64, n, dtype=np.uint64, endpoint=False)
+    C = rf.rmatrix_mult_limb_arrays([(1, a0), (-1, a1)], r, [(1, b0)], d)
+    mods = [(1 << 64) - 1, (1 << 64) + 1]
+    lowb = 64 * 4096
+    for (sa, av), (sc, cv) in zip([(1, a0), (-1, a1)], C):
+        assert sc == sa and len(cv) in (2 * n - 1, 2 * n) and int(cv[-1]) != 0
+        for ma, mb, mc, m in zip(_limb_mods(av), _limb_mods(b0), _limb_mods(cv), mods):
+            assert mc == ma * mb % m
+        low = lambda v: int.from_bytes(v[:4096].tobytes(), "little")
+        assert low(cv) == low(av) * low(b0) % (1 << lowb)
+
+
 def test_zero_and_ragged_entries():
     rng = random.Random(11)
     A = [0, rng.getrandbits(9000), -rng.getrandbits(10), 0]
@@ -119,9 +166,10 @@ def test_dropin_mpz_abi():
 
 
 def test_unsupported_size_returns_error():
-    # one entry of 2^30 bits needs 2^22 points at the widest chunks
+    # one entry of 2^33 bits needs 2^25 points at the widest chunks (the
+    # longest plan has 2^24); a sparse entry keeps the host side cheap
     rng = random.Random(1)
-    big = [rng.getrandbits(1 << 30) | 1 << ((1 << 30) - 1)] + [rng.getrandbits(64) for _ in range(3)]
+    big = [1 << ((1 << 33) - 1) | 12345] + [rng.getrandbits(64) for _ in range(3)]
     mA = MpzArray(big)
     mC = MpzArray.zeros(4)
     assert rf.lib().mpz_rmatrix_gpu_fft_suitable(mA.ptr(), 2, mA.ptr(), 2) == 0

@@ -94,6 +94,37 @@ def test_fwd_inv_stage_longest(lgN):
             assert back == [x * N % p for x in red + [0] * (N - nchunk)]
 
 
+@pytest.mark.parametrize("lgN,q", [(22, 23), (23, 5), (24, 0)])
+def test_fwd_inv_stage_beyond_2_21(lgN, q):
+    """Transforms of 2^22 .. 2^24 points (three column passes each way and the
+    twiddle tables of order 2^24), on numpy arrays: delta_0 -> all ones; the
+    chunk-1 delta -> omega^rev(m) at 4096 sampled positions and the pairs
+    y[2m + 1] = -y[2m] everywhere; the round trip inv(fwd(x) . 1) = N x for a
+    random negative x filling every point."""
+    import numpy as np
+    p = rf.primes(24)[q]
+    N = 1 << lgN
+    ones = stages.fwd_array(np.array([1], np.uint64), 1, 1, 1, lgN, q)
+    assert bool(np.all(ones == 1))
+    e1 = stages.fwd_array(np.array([0, 1], np.uint64), 1, 1, 2, lgN, q)
+    w = oracle.root(p, 24, lgN)
+    rng = random.Random(lgN)
+    for m in [0, 1, N // 2, N - 1] + [rng.randrange(N) for _ in range(4096)]:
+        rev = int(format(m, "0%db" % lgN)[::-1], 2)
+        assert int(e1[m]) == pow(w, rev, p), m
+    assert bool(np.all((e1[0::2] + e1[1::2]) % np.uint64(p) == 0))
+    gen = np.random.default_rng(lgN)
+    x = gen.integers(0, 1 << 63, size=N, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    spec = stages.fwd_array(x, -1, 1, N, lgN, q)
+    back = stages.inv_array(spec, ones, 1, 1, lgN, q)
+    pp = np.uint64(p)
+    want = x % pp
+    for _ in range(lgN):  # N x mod p by doubling (values stay below 2^52)
+        want = (want * np.uint64(2)) % pp
+    want = (pp - want) % pp  # the entry is negative
+    assert bool(np.array_equal(back, want))
+
+
 @pytest.mark.parametrize("lgN", [15, 16, 17])
 def test_fwd_stage_long(lgN):
     """Column pass + row pass transforms (beyond one workgroup's LDS): half
