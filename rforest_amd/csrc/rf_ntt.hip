@@ -114,6 +114,48 @@ __device__ __forceinline__ int fwd_index(int g, int j) {
   return (H << (LGN - Q::L0)) | (j << NLO) | lo;
 }
 
+// Group of forward pass P held by thread tid as its u-th group.  Passes of
+// four stages hold one group per thread: g = tid.  A last pass of fewer
+// stages (several groups per thread) in a workgroup of several waves takes
+// g = wave 2^(10-R) + u 64 + lane: the residues i = g 2^R + j of a wave are
+// then the block [1024 wave, 1024 wave + 1024), the same block the wave held
+// in the pass before (see fwd_local), and stores stay coalesced (consecutive
+// lanes, consecutive groups).
+template <int LGN, int P>
+__device__ __forceinline__ int fwd_group(int tid, int u) {
+  using Q = Pass<LGN, P>;
+  if constexpr (P > 0 && P == Shape<LGN>::NP - 1 && Q::NG > 1 && Shape<LGN>::T > 64)
+    return ((tid >> 6) << (10 - Q::R)) | (u << 6) | (tid & 63);
+  else
+    return tid + Shape<LGN>::T * u;
+}
+
+// Whether the exchange from forward pass P-1 into pass P stays inside each
+// wave.  A transform of at most one wave trivially does.  Otherwise pass
+// P-1 has four stages and one group per thread, i = H 2^(NLO+4) + j 2^NLO + lo
+// with H = tid >> NLO, lo = tid mod 2^NLO; for NLO <= 6, i >> 10 = tid >> 6,
+// so a wave holds exactly the block of 1024 residues numbered by its wave
+// index, and so does pass P (NLO 4 lower, or the last-pass grouping above).
+// The stages in between act on bits below 10 only, within those blocks.
+template <int LGN, int P>
+constexpr bool fwd_local() {
+  return Shape<LGN>::T <= 64 || (LGN - Pass<LGN, P - 1>::L0 - 4) <= 6;
+}
+
+// Synchronisation around an exchange through LDS: the whole workgroup, or
+// (LOCAL) only the wave, whose LDS accesses execute in order; the fences keep
+// the compiler from moving this lane's accesses across the point.
+template <bool LOCAL>
+__device__ __forceinline__ void xsync() {
+  if constexpr (LOCAL) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Inverse group indexing: stages L0 .. L0+R-1 act on index bits L0 .. L0+R-1.
 template <int LGN, int P, int R0 = inv_r0<LGN>()>
 __device__ __forceinline__ int inv_index(int g, int j) {
@@ -137,7 +179,7 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __re
     const int half = 1 << (Q::R - 1 - s);
 #pragma unroll
     for (int u = 0; u < Q::NG; u++) {
-      const int g = tid + Shape<LGN>::T * u;
+      const int g = fwd_group<LGN, P>(tid, u);
       const int H = (ROW ? (hb << Q::L0) : 0) + (g >> NLO);
 #pragma unroll
       for (int b = 0; b < (1 << s); b++) {
@@ -210,12 +252,12 @@ __device__ __forceinline__ void fwd_store(double* __restrict__ out, const double
     if constexpr (NLO == 0 && QL::GS >= 2) {
 #pragma unroll
       for (int j = 0; j < QL::GS; j += 2)
-        *reinterpret_cast<double2*>(out + fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)) =
+        *reinterpret_cast<double2*>(out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j)) =
             make_double2(rf_reduce(v[u * QL::GS + j], p, pinv), rf_reduce(v[u * QL::GS + j + 1], p, pinv));
     } else {
 #pragma unroll
       for (int j = 0; j < QL::GS; j++)
-        out[fwd_index<LGN, S::NP - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+        out[fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
     }
   }
 }
@@ -236,14 +278,16 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const d
     for (int u = 0; u < Qp::NG; u++)
 #pragma unroll
       for (int j = 0; j < Qp::GS; j++)
-        sh[lds_slot<SW>(fwd_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
-    __syncthreads();
+        sh[lds_slot<SW>(fwd_index<LGN, P - 1>(fwd_group<LGN, P - 1>(tid, u), j))] = v[u * Qp::GS + j];
+    // the exchanges after a wave-local one are wave-local too, so a wave
+    // barrier after the reads suffices there as well
+    xsync<fwd_local<LGN, P>()>();
 #pragma unroll
     for (int u = 0; u < Q::NG; u++)
 #pragma unroll
       for (int j = 0; j < Q::GS; j++)
-        v[u * Q::GS + j] = sh[lds_slot<SW>(fwd_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
-    __syncthreads();
+        v[u * Q::GS + j] = sh[lds_slot<SW>(fwd_index<LGN, P>(fwd_group<LGN, P>(tid, u), j))];
+    xsync<fwd_local<LGN, P>()>();
     fwd_pass<LGN, P, 0, ROW>(v, tid, tw, p, hb);
     fwd_rest<LGN, P + 1, ROW>(v, sh, tid, tw, p, pinv, hb);
   }
