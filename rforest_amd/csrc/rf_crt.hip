@@ -51,7 +51,10 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
       for (int l = j - 2; l >= 0; l--) acc = rf_mulmod(acc, C->pm[j][l][0], C->pm[j][l][1], p) + v[l];
       r = rf_mulmod(x - acc, C->ci[j][0], C->ci[j][1], p);
     }
-    v[j] = rf_canon(r, p);
+    // |r| < p: x is reduced (|x| <= p/2 + 8); acc, a product plus a
+    // canonical digit, stays below 1.5 p + 8, so |x - acc| < 2^50 and the
+    // product r satisfies |r| <= p (1/2 + 2^50 2^-52) = 0.75 p
+    v[j] = rf_canon1(r, p, C->pinv[j]);
   }
   // Mixed radix to binary: X = v_{K-1}; X = X p_j + v_j for j = K-2 .. 0, in
   // 32-bit limbs.  p_j = phi 2^32 + plo with phi < 2^16 and, as p_j = 1 mod

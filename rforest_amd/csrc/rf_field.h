@@ -63,6 +63,17 @@ RF_DEV double rf_mulmod_var(double a, double b, double p, double pinv) {
   return r + l;
 }
 
+// Canonical representative in [0, p) of an integer |x| < p, without compares:
+// y = fl(x pinv + 2^-50) has floor -1 for x < 0 and 0 for x >= 0.  With
+// pinv = (1/p)(1 + d), |d| <= 2^-53, and 2^47 < p < 2^48: for 0 <= x <= p - 1,
+// 0 < x pinv + 2^-50 <= 1 - 1/p + 2^-53 + 2^-50 < 1 - 2^-49, which rounds
+// below 1; for -(p - 1) <= x <= -1, -1 < x pinv + 2^-50 <= -2^-48 + 2^-50 < 0.
+// fma(-q, p, x) is then x + p or x, an integer below 2^48: exact.
+RF_DEV double rf_canon1(double x, double p, double pinv) {
+  const double q = __builtin_floor(__builtin_fma(x, pinv, 0x1p-50));
+  return __builtin_fma(-q, p, x);
+}
+
 // Canonical representative in [0, p) of |x| <= 2p.
 RF_DEV double rf_canon(double x, double p) {
   x = (x < 0.0) ? x + p : x;
