@@ -99,14 +99,20 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
   for (int l = 0; l < S; l++) X64[l] = (uint64_t)x[2 * l] | ((uint64_t)x[2 * l + 1] << 32);
 #pragma unroll
   for (int l = 0; l < S; l++) X[l] = X64[l];
-  // symmetric lift: X > floor(P / 2)  =>  X - P (two's complement in S limbs)
-  bool gt = false, decided = false;
+  // symmetric lift: X > floor(P / 2)  =>  X - P (two's complement in S limbs).
+  // floor(P / 2) = (P - 1) / 2 has the mixed-radix digits (p_j - 1) / 2 (the
+  // sum of (p_j - 1) / 2 p_0 .. p_{j-1} telescopes), and the mixed-radix
+  // order is lexicographic from the top digit, which nearly always decides.
+  const double htop = __builtin_fma(C->p[K - 1], 0.5, -0.5);
+  bool gt = v[K - 1] > htop;
+  if (v[K - 1] == htop) {
 #pragma unroll
-  for (int l = S - 1; l >= 0; l--) {
-    const uint64_t h = C->Ph[K][l];
-    if (!decided && X[l] != h) {
-      gt = X[l] > h;
-      decided = true;
+    for (int l = K - 2; l >= 0; l--) {
+      const double h = __builtin_fma(C->p[l], 0.5, -0.5);
+      if (v[l] != h) {
+        gt = v[l] > h;
+        break;
+      }
     }
   }
   if (gt) {

@@ -265,10 +265,7 @@ void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
   for (int k = 0; k <= pr->k; k++) {
     uint64_t P[RF_MAXS + 1] = {1};
     for (int j = 0; j < k; j++) mw_mul_1(P, RF_MAXS + 1, pr->p[j]);
-    for (int l = 0; l < RF_MAXS; l++) {
-      c->P[k][l] = P[l];
-      c->Ph[k][l] = (P[l] >> 1) | (P[l + 1] << 63);
-    }
+    for (int l = 0; l < RF_MAXS; l++) c->P[k][l] = P[l];
   }
 }
 

@@ -95,7 +95,6 @@ struct RfConsts {
   double ci[RF_MAXP][2];
   uint64_t pu[RF_MAXP];                  // the primes as integers
   uint64_t P[RF_MAXP + 1][RF_MAXS];      // P_k = p_0 ... p_{k-1}
-  uint64_t Ph[RF_MAXP + 1][RF_MAXS];     // floor(P_k / 2)
   // pack: pk[q][v][w] = the three 16-bit pieces k0, k1, k2 of
   // K = 2^(32 w) * S_v mod p_q = k0 + k1 2^16 + k2 2^32, held as k0, k1 2^16
   // and k2 2^32, where S_0 = 1 and S_v = 2^-v mod p_q for v = lgN (B operands

@@ -191,7 +191,15 @@ def test_crt_stage(k):
             cs = []
             for _ in range(nout):
                 u = rng.random()
-                if u < 0.1:
+                if u < 0.05:
+                    # within one mixed-radix digit of the lift threshold
+                    # (P - 1) / 2: the digits above position j tie with it
+                    j = rng.randrange(k)
+                    Mj = 1
+                    for p in P[:j]:
+                        Mj *= p
+                    c = rng.choice([1, -1]) * ((Pk - 1) // 2 - rng.choice([0, 1]) * Mj)
+                elif u < 0.1:
                     c = (Pk - 1) // 2
                 elif u < 0.2:
                     c = -((Pk - 1) // 2)
