@@ -125,6 +125,22 @@ def test_fwd_inv_stage_beyond_2_21(lgN, q):
     assert bool(np.array_equal(back, want))
 
 
+def test_long_tables_made_again_after_clear():
+    """The order-2^24 twiddle tables are freed by hw_mpz_clear and made again
+    by the next plan that needs them after hw_mpz_setup."""
+    import numpy as np
+    lgN, q = 22, 7
+    p = rf.primes(24)[q]
+    w = oracle.root(p, 24, lgN)
+    for _ in range(2):
+        e1 = stages.fwd_array(np.array([0, 1], np.uint64), 1, 1, 2, lgN, q)
+        for m in (1, 3, 1 << 20, (1 << 22) - 1):
+            rev = int(format(m, "0%db" % lgN)[::-1], 2)
+            assert int(e1[m]) == pow(w, rev, p)
+        rf.hw_mpz_clear()
+        rf.hw_mpz_setup()
+
+
 @pytest.mark.parametrize("lgN", [15, 16, 17])
 def test_fwd_stage_long(lgN):
     """Column pass + row pass transforms (beyond one workgroup's LDS): half
