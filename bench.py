@@ -431,7 +431,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=256, help="products per step")
+    ap.add_argument("--batch", type=int, default=1024, help="products per step")
     ap.add_argument("--lanes", type=int, default=1, help="overlapping pipelines per prepared batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
