@@ -272,45 +272,47 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
 // the block is the composition, in order, of the aggregates of the entry's
 // earlier blocks applied to 0: the first wave's lanes each compose a
 // contiguous run of them, then an ordered shuffle tree combines the runs.
-__global__ __launch_bounds__(RF_CRT_THREADS) void k_apply(const RfCrtTask* __restrict__ tasks,
-                                                         const uint8_t* __restrict__ pre,
-                                                         const uint8_t* __restrict__ fun) {
-  __shared__ int cin_sh;
-  const int tid = threadIdx.x;
+__global__ __launch_bounds__(64) void k_apply(const RfCrtTask* __restrict__ tasks,
+                                           const uint8_t* __restrict__ pre,
+                                           const uint8_t* __restrict__ fun) {
+  // one wave per block: the composition of the earlier blocks' aggregates
+  // (a shuffle tree leaves it in lane 0), then RF_CRT_THREADS / 64 digits per
+  // lane (carries are rare, so most lanes only read their codes)
+  const int lane = threadIdx.x;
   const RfCrtTask tk = rf_wave_uniform_load(tasks + blockIdx.x);
   const int t = tk.t, H = tk.halo;
-  if (tid < 64) {
-    const uint32_t nprev = tk.digit0 / (uint32_t)(RF_CRT_THREADS - H);  // earlier blocks of the entry
-    const uint32_t first = tk.fun_idx - nprev;
-    const uint32_t cnt = (nprev + 63) / 64;
-    uint32_t f = FN_ID;
-    const uint32_t b1 = min(nprev, (uint32_t)(tid + 1) * cnt);
-    for (uint32_t b = (uint32_t)tid * cnt; b < b1; b++) f = fn_compose(fun[first + b], f);
+  const uint32_t nprev = tk.digit0 / (uint32_t)(RF_CRT_THREADS - H);  // earlier blocks of the entry
+  const uint32_t first = tk.fun_idx - nprev;
+  const uint32_t cnt = (nprev + 63) / 64;
+  uint32_t f = FN_ID;
+  const uint32_t b1 = min(nprev, (uint32_t)(lane + 1) * cnt);
+  for (uint32_t b = (uint32_t)lane * cnt; b < b1; b++) f = fn_compose(fun[first + b], f);
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t g = __shfl_down(f, off);
-      if ((tid & (2 * off - 1)) == 0) f = fn_compose(g, f);
-    }
-    if (tid == 0) cin_sh = fn_apply(f, 0);
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t g = __shfl_down(f, off);
+    if ((lane & (2 * off - 1)) == 0) f = fn_compose(g, f);
   }
-  __syncthreads();
-  const int cin = cin_sh;
-  if (tid < H) return;
-  const uint32_t i = tk.digit0 + (uint32_t)(tid - H);
+  const int cin = __shfl(fn_apply(f, 0), 0);
   const uint32_t ndig = (tk.rn + t - 1) / t;
-  if (i >= ndig) return;
-  const int c = fn_apply(pre[tk.pre_off + tid], cin);
-  if (c == 0) return;
-  uint64_t* dst = tk.out + (size_t)i * t;
-  const uint32_t lim = min((uint32_t)t, tk.rn - i * t);
-  for (uint32_t l = 0; l < lim; l++) {
-    const uint64_t x = dst[l];
-    if (c > 0) {
-      dst[l] = x + 1;
-      if (x != ~(uint64_t)0) break;
-    } else {
-      dst[l] = x - 1;
-      if (x != 0) break;
+#pragma unroll
+  for (int k = 0; k < RF_CRT_THREADS / 64; k++) {
+    const int tid = lane + 64 * k;
+    if (tid < H) continue;
+    const uint32_t i = tk.digit0 + (uint32_t)(tid - H);
+    if (i >= ndig) continue;
+    const int c = fn_apply(pre[tk.pre_off + tid], cin);
+    if (c == 0) continue;
+    uint64_t* dst = tk.out + (size_t)i * t;
+    const uint32_t lim = min((uint32_t)t, tk.rn - i * t);
+    for (uint32_t l = 0; l < lim; l++) {
+      const uint64_t x = dst[l];
+      if (c > 0) {
+        dst[l] = x + 1;
+        if (x != ~(uint64_t)0) break;
+      } else {
+        dst[l] = x - 1;
+        if (x != 0) break;
+      }
     }
   }
 }
@@ -345,6 +347,6 @@ hipError_t rf_launch_crt(const RfDevTables& tb, int k, const RfCrtTask* tasks, i
 hipError_t rf_launch_apply(const RfCrtTask* tasks, int ntasks, const uint8_t* pre, const uint8_t* fun,
                            hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_apply, dim3(ntasks), dim3(RF_CRT_THREADS), 0, st, tasks, pre, fun);
+  hipLaunchKernelGGL(k_apply, dim3(ntasks), dim3(64), 0, st, tasks, pre, fun);
   return hipGetLastError();
 }
