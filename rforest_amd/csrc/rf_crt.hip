@@ -131,15 +131,20 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
 // (limbs [t i, t i + t)), plus the incoming carry cin; writes limb l through
 // emit(l, value) and returns the signed carry out.  Coefficient j occupies
 // limbs [t j, t j + S) unsigned, minus 2^64 at limb t j + S if negative.
-template <int S, typename Emit>
-__device__ __forceinline__ int64_t digit_sum(const uint64_t* __restrict__ X, int j0, int i, int t,
-                                             int M, int64_t cin, Emit emit) {
+// TT > 0 fixes t = TT (and M) at compile time so the loops unroll.
+template <int S, int TT, typename Emit>
+__device__ __forceinline__ int64_t digit_sum(const uint64_t* __restrict__ X, int j0, int i, int t_rt,
+                                             int M_rt, int64_t cin, Emit emit) {
+  const int t = TT > 0 ? TT : t_rt;
+  const int M = TT > 0 ? (S + TT - 1) / TT : M_rt;
   const int jneg = i - S / t, lneg = S % t;
   bool neg = false;
   if (jneg >= j0 && jneg >= 0) neg = (X[(size_t)(jneg - j0) * S + (S - 1)] >> 63) != 0;
   uint64_t lo = (uint64_t)cin;
   int64_t hi = cin >> 63;
+#pragma unroll
   for (int l = 0; l < t; l++) {
+#pragma unroll
     for (int m = 0; m < M; m++) {
       const int idx = m * t + l;
       const int j = i - m;
@@ -210,7 +215,10 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
         if ((uint32_t)l < lim) dst[l] = v;
       }
     };
-    E[tid] = (tid >= H - 1 && i >= 0) ? digit_sum<S>(X, j0, i, t, M, 0, emit) : 0;
+    int64_t e = 0;
+    if (tid >= H - 1 && i >= 0)
+      e = t == 4 ? digit_sum<S, 4>(X, j0, i, t, M, 0, emit) : digit_sum<S, 0>(X, j0, i, t, M, 0, emit);
+    E[tid] = e;
   }
   __syncthreads();
   // pass 2: U_i = (V_i + E_{i-1}) mod 2^(64 t) and F_i = its carry in
