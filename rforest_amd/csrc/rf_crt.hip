@@ -216,8 +216,19 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
       }
     };
     int64_t e = 0;
-    if (tid >= H - 1 && i >= 0)
-      e = t == 4 ? digit_sum<S, 4>(X, j0, i, t, M, 0, emit) : digit_sum<S, 0>(X, j0, i, t, M, 0, emit);
+    if (tid >= H - 1 && i >= 0) {
+      switch (t) {  // wave-uniform: one unrolled instance per chunk width
+        case 1: e = digit_sum<S, 1>(X, j0, i, t, M, 0, emit); break;
+        case 2: e = digit_sum<S, 2>(X, j0, i, t, M, 0, emit); break;
+        case 3: e = digit_sum<S, 3>(X, j0, i, t, M, 0, emit); break;
+        case 4: e = digit_sum<S, 4>(X, j0, i, t, M, 0, emit); break;
+        case 5: e = digit_sum<S, 5>(X, j0, i, t, M, 0, emit); break;
+        case 6: e = digit_sum<S, 6>(X, j0, i, t, M, 0, emit); break;
+        case 7: e = digit_sum<S, 7>(X, j0, i, t, M, 0, emit); break;
+        case 8: e = digit_sum<S, 8>(X, j0, i, t, M, 0, emit); break;
+        default: e = digit_sum<S, 0>(X, j0, i, t, M, 0, emit); break;
+      }
+    }
     E[tid] = e;
   }
   __syncthreads();
