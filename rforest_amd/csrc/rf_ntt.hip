@@ -17,9 +17,11 @@
 // four passes (4 + 4 + 4 + 1 stages) and three LDS exchanges.  Transforms
 // smaller than a wave share a workgroup.
 //
-// Both directions use Cooley-Tukey butterflies (x, y) -> (x + w y, x - w y):
-// a residue grows by at most p per stage, and every eighth stage reduces, so
-// all rf_mulmod inputs stay below 8.5 p + 8 < 2^51 (primes < 2^51 / 9).
+// Both directions use Cooley-Tukey butterflies (x, y) -> (x + w y, x - w y).
+// A stage adds at most p (1/2 + |y| 2^-52) to a residue's bound, or doubles it
+// where w = 1 is not multiplied (the first pass's block 0), and every eighth
+// stage reduces, so all values stay below 12.2 p + 160 < 2^51.4 (primes below
+// 2^51 / 9; the derivation is at RF_PMAX).
 #include "rf_field.h"
 #include "rf_kernels.h"
 
