@@ -411,8 +411,6 @@ Layout layout_of(std::vector<Product>& prods) {
   L.coef_b = align256(L.coef_n * 8);
   size_t tb = 0;
   for (auto& kv : fwd_n) tb += align256(kv.second * sizeof(RfFwdTask)), L.nfwd += kv.second;
-  // inverse launches split further by A base (one per resident operand)
-  tb += 256 * (prods.size() + 1);
   for (auto& kv : inv_n) tb += align256(kv.second * sizeof(RfInvTask)), L.ninv += kv.second;
   for (auto& kv : col_n) tb += align256(kv.second * sizeof(RfRowTask)), L.ncol += kv.second;
   tb += align256(row_n * sizeof(RfRowTask));
@@ -451,7 +449,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
   pp.image.clear();
   pp.launches.clear();
   std::map<int, std::vector<RfFwdTask>> fwd;
-  std::map<std::pair<int, const double*>, std::vector<RfInvTask>> inv;
+  std::map<int, std::vector<RfInvTask>> inv;
   std::map<int, std::vector<RfRowTask>> icol;
   std::vector<RfRowTask> frow;
   std::map<int, std::vector<RfCrtTask>> crt;
@@ -509,7 +507,8 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
             t.out_off = out_off + h * rowlen;
             t.q = (uint16_t)q;
             t.d = (uint16_t)d;
-            inv[{split ? RF_LGN_ROW : pl.lgN, abase}].push_back(t);
+            t.a_base = abase;  // a resident operand's spectra, in the same launch as the rest
+            inv[split ? RF_LGN_ROW : pl.lgN].push_back(t);
           }
           if (split) icol[pl.lgN].push_back(RfRowTask{out_off, 0, (uint16_t)q, (uint16_t)pl.lgN});
         }
@@ -524,8 +523,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
                            append_tasks(pp.image, kv.second), (int)kv.second.size()});
   if (!frow.empty()) pp.launches.push_back({K_FWD_ROW, RF_LGN_ROW, append_tasks(pp.image, frow), (int)frow.size()});
   for (auto& kv : inv)
-    pp.launches.push_back({K_INV, kv.first.first, append_tasks(pp.image, kv.second), (int)kv.second.size(),
-                           kv.first.second});
+    pp.launches.push_back({K_INV, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
   for (auto& kv : icol)
     pp.launches.push_back({K_INV_COL, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
   std::vector<LaunchRec> crt_l;
@@ -566,7 +564,7 @@ int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
         HIPCHK(rf_launch_fwd_row(tables(), (const RfRowTask*)t, l.n, S.spec, st));
         break;
       case K_INV:
-        HIPCHK(rf_launch_inv(tables(), l.param, (const RfInvTask*)t, l.n, l.abase ? l.abase : S.spec, S.spec,
+        HIPCHK(rf_launch_inv(tables(), l.param, (const RfInvTask*)t, l.n, S.spec, S.spec,
                              S.coef, st));
         break;
       case K_INV_COL:

@@ -68,7 +68,6 @@ struct LaunchRec {
   int param;   // lgN for transforms, k for CRT
   size_t off;  // byte offset of its task table in the image
   int n;       // tasks
-  const double* abase = nullptr;  // K_INV: A spectra base (nullptr: the batch's spectra)
 };
 struct Pipeline {
   Layout L;

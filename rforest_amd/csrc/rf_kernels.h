@@ -35,6 +35,7 @@ struct RfInvTask {
   uint32_t b_stride;  // spec distance between B[l][j] and B[l+1][j]
   uint16_t q;
   uint16_t d;         // terms; 0: inverse of the one spectrum at a_off (b unused)
+  const double* a_base;  // A spectra (a resident operand's), or nullptr: the launch's A base
 };
 
 // Transforms longer than 2^RF_LGN_LDS points, N = 2^(13 + n1), view the

@@ -716,7 +716,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
   const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
   double v[S::E];
-  const double* pa = aspec + tk.a_off;
+  const double* pa = (tk.a_base ? tk.a_base : aspec) + tk.a_off;
   const double* pb = bspec + tk.b_off;
   constexpr int R0 = inv_r0<LGN>();
   if constexpr (R0 <= 2) {
