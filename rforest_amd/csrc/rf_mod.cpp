@@ -84,6 +84,10 @@ int mod_batch_plan(const rf_mod* const* mods, const rf_dev_entry* xs, int cnt, M
     Q.B.push_back(F);
     rc = plan_product(Q);
     if (rc) return rc;
+    // only (X - QD) mod B^(n+1) is formed: the recomposition stops after the
+    // low n + 1 limbs of QD (carries only move upward, so they are exact),
+    // which halves that product's CRT blocks
+    Q.plan.rn = std::min<uint64_t>(Q.plan.rn, md->n + 1);
     mb.p1.push_back(std::move(P));
     if (Q.plan.k == 0) continue;
     I.p2 = (int)mb.p2.size();
