@@ -28,7 +28,7 @@ struct RfFwdTask {
 // One inverse task: C[i][j] at prime q = sum_l A[i][l] B[l][j], inverse
 // transform, coefficient residues out.
 struct RfInvTask {
-  uint64_t a_off;     // spec offset of A[i][0] at prime q
+  uint64_t a_off;     // offset of A[i][0] at prime q from a_base (or from the launch's A base)
   uint64_t b_off;     // spec offset of B[0][j] at prime q
   uint64_t out_off;   // coef offset of C[i][j] at prime q
   uint32_t a_stride;  // spec distance between A[i][l] and A[i][l+1]
