@@ -34,6 +34,12 @@ HIPCC_FLAGS = [
 ]
 
 
+# Per-source additions: the CRT kernel for 22 and 24 primes unrolls its
+# mixed-radix conversion only above LLVM's default pragma-unroll size limit;
+# otherwise its Garner digits are indexed at run time and live in scratch.
+SOURCE_FLAGS = {"rf_crt.hip": ["-mllvm", "-pragma-unroll-threshold=200000"]}
+
+
 def build(verbose=False):
     """Compiles every source for gfx950 in parallel, then links the library."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -43,7 +49,7 @@ def build(verbose=False):
     for src in SOURCES:
         obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [hipcc] + HIPCC_FLAGS + inc + ["-c", src, "-o", obj]
+        cmd = [hipcc] + HIPCC_FLAGS + SOURCE_FLAGS.get(os.path.basename(src), []) + inc + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
