@@ -24,8 +24,9 @@ extern "C" {
 /* Pack + forward transform of n entries at prime index q: entry e has
    magnitude limbs + off[e], nlimbs[e] limbs, sign signs[e]; chunks of 64*t
    bits, the first nchunk of them (<= 2^lgN) are packed, scaled by 2^-scale
-   (scale = 0 or lgN), prime index q < 24 (the primes with twiddle tables).
-   Writes n x 2^lgN spectrum values; position m holds
+   (scale = 0 or lgN), prime index q < 24 (the primes with twiddle tables),
+   1 <= lgN <= 24 (above 2^21 points the first call makes the long twiddle
+   tables).  Writes n x 2^lgN spectrum values; position m holds
    sum_i a_i omega^(i rev(m)), omega = rf_root(q, lgN). */
 int rf_stage_fwd(const uint64_t* limbs, const size_t* off, const size_t* nlimbs, const int* signs,
                  int n, int t, uint32_t nchunk, int lgN, int q, int scale, uint64_t* out);
