@@ -219,6 +219,11 @@ typedef struct rf_mod rf_mod;
 int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out);
 /* Limbs of D, which every remainder of this modulus is written in. */
 uint64_t rf_mod_limbs(const rf_mod* mod);
+/* No counterpart in the reference (a query of this implementation's plan):
+   W when the product Q D is formed modulo 2^(64 W) - 1 by a cyclic
+   convolution (then folded with X; W >= limbs of D + 1), 0 when it is formed
+   whole. */
+uint64_t rf_mod_wrap_limbs(const rf_mod* mod);
 /* mpzfft_mod_mod: rop := op mod D in [0, D) (any sign of op). */
 int rf_mod_mod(const rf_mod* mod, mpz_t rop, const mpz_t op);
 /* A batch (a remainder-tree level) in one call: outs[i] (device,

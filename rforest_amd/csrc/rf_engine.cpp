@@ -321,6 +321,14 @@ int plan_product(Product& P) {
       return RF_OK;
     }
     const RfPlan& f = op.plan;
+    if (f.cyclic) {  // the operand's wrap-around plan, if the right side's chunks fit it
+      const uint64_t nB = (LB + 64 * (uint64_t)f.t - 1) / (64 * (uint64_t)f.t);
+      if (nB > (uint64_t)f.nB)
+        return fail(RF_ERR_ARG, "right operand of %llu bits does not fit the resident operand's plan",
+                    (unsigned long long)LB);
+      P.plan = f;
+      return RF_OK;
+    }
     if (rf_plan_fixed_k(&g.primes, op.la_bits, LB, P.r, P.d, f.t, f.lgN, f.k, &P.plan) != 0 || P.plan.k != f.k)
       return fail(RF_ERR_ARG, "right operand of %llu bits does not fit the resident operand's plan",
                   (unsigned long long)LB);
@@ -1089,7 +1097,8 @@ int operand_transform(rf_operand* op, const std::vector<Entry>& A, size_t base) 
 
 // Plans and allocates an operand for entries A (bits/sign filled), then
 // transforms them.
-int operand_make(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out) {
+int operand_make(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out,
+                 const RfPlan* fixed = nullptr) {
   rf_operand* op = new rf_operand();
   op->r = r;
   op->d = d;
@@ -1099,8 +1108,10 @@ int operand_make(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_
   }
   int rc = RF_OK;
   if (op->la_bits && b_max_bits) {
-    if (!tuned_plan(op->la_bits, b_max_bits, r, d, &op->plan) &&
-        rf_plan_choose(&g.primes, op->la_bits, b_max_bits, r, d, &op->plan) != 0)
+    if (fixed)
+      op->plan = *fixed;
+    else if (!tuned_plan(op->la_bits, b_max_bits, r, d, &op->plan) &&
+             rf_plan_choose(&g.primes, op->la_bits, b_max_bits, r, d, &op->plan) != 0)
       rc = fail(RF_ERR_UNSUPPORTED, "no transform plan for a %dx%d operand of %llu bits and right entries of %llu bits",
                 r, d, (unsigned long long)op->la_bits, (unsigned long long)b_max_bits);
     if (!rc) rc = ensure_long_tables(op->plan.lgN);
@@ -1729,8 +1740,9 @@ size_t operand_task_bound(int nent) {
   const size_t rows = (size_t)1 << (RF_LGN_MAX - RF_LGN_ROW);
   return align256((size_t)nent * RF_KMAX * sizeof(RfFwdTask)) + align256((size_t)nent * RF_KMAX * rows * sizeof(RfRowTask));
 }
-int make_operand(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out) {
-  return operand_make(A, r, d, b_max_bits, base, out);
+int make_operand(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out,
+                 const RfPlan* fixed) {
+  return operand_make(A, r, d, b_max_bits, base, out, fixed);
 }
 int pipeline_run(std::vector<Product>& prods, const Layout& L, size_t base) {
   return run_pipeline(prods, L, base, g.stream);

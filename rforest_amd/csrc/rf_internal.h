@@ -97,7 +97,9 @@ const RfPrimes& engine_primes();
 // stream with its scratch at arena byte `base` (the arena must already hold
 // base + L.total() bytes; returns with the stream busy), and an operand's
 // release.
-int make_operand(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out);
+// fixed: the plan to use (a cyclic one, say) instead of the planner's.
+int make_operand(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_t base, rf_operand** out,
+                 const RfPlan* fixed = nullptr);
 // Arena bytes an operand of nent entries needs for its transforms' task
 // tables at most (reserve them with its staged inputs: the arena moves when
 // it grows).

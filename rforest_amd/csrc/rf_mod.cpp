@@ -24,6 +24,7 @@
 
 struct rf_mod {
   uint64_t m = 0, n = 0;  // most limbs of |X|, limbs of D
+  uint64_t wrap = 0;      // W when Q D is formed modulo B^W - 1 (div has a cyclic plan), else 0
   rf_operand* inv = nullptr;
   rf_operand* div = nullptr;
   uint64_t* d_dev = nullptr;  // D, n limbs
@@ -57,7 +58,7 @@ int mod_batch_plan(const rf_mod* const* mods, const rf_dev_entry* xs, int cnt, M
                                 (unsigned long long)xn, (unsigned long long)md->m);
     Item& I = mb.it[i];
     I.rt_off = rt_limbs;
-    rt_limbs += 2 * (md->n + 1);
+    rt_limbs += md->wrap ? 3 * md->wrap : 2 * (md->n + 1);
     if (xs[i].sign == 0 || xn < md->n) continue;  // X_hi = 0: Q = 0
     Product P;
     P.r = P.d = 1;
@@ -86,8 +87,9 @@ int mod_batch_plan(const rf_mod* const* mods, const rf_dev_entry* xs, int cnt, M
     if (rc) return rc;
     // only (X - QD) mod B^(n+1) is formed: the recomposition stops after the
     // low n + 1 limbs of QD (carries only move upward, so they are exact),
-    // which halves that product's CRT blocks
-    Q.plan.rn = std::min<uint64_t>(Q.plan.rn, md->n + 1);
+    // which halves that product's CRT blocks.  A cyclic Q D keeps all its
+    // limbs: they are folded modulo B^W - 1.
+    if (!md->wrap) Q.plan.rn = std::min<uint64_t>(Q.plan.rn, md->n + 1);
     mb.p1.push_back(std::move(P));
     if (Q.plan.k == 0) continue;
     I.p2 = (int)mb.p2.size();
@@ -152,7 +154,9 @@ int mod_batch_exec(ModBatch& mb, const rf_mod* const* mods, const rf_dev_entry* 
     k.d = md->d_dev;
     k.n = md->n;
     k.r = RT + I.rt_off;
-    k.t = k.r + md->n + 1;
+    k.t = k.r + (md->wrap ? md->wrap : md->n + 1);
+    k.xf = md->wrap ? k.t + md->wrap : nullptr;
+    k.wrap = I.p2 >= 0 ? md->wrap : 0;  // Q = 0: X - 0 needs no folding
     k.out = outs[i];
     ht[i] = k;
   }
@@ -220,7 +224,16 @@ int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
     E[0].nlimbs = md->n;
     E[0].sign = 1;
     E[0].bits = mpz_sizeinbase(d, 2);
-    rc = make_operand(E, 1, 1, q_bits, i_b, &md->div);
+    // Q D is needed only modulo B^W - 1 for W >= n + 1 (R < 3D < B^W - 1) and
+    // W >= m - n + 1 (Q fits): a cyclic product of about half the length,
+    // when the cost model prefers it
+    const uint64_t qn = md->m - md->n + 1;
+    RfPlan cyc{}, acyc{};
+    const bool use_cyc =
+        rf_plan_cyclic(&engine_primes(), std::max<uint64_t>(md->n + 1, qn), md->n, qn, &cyc) == 0 &&
+        (rf_plan_choose(&engine_primes(), E[0].bits, q_bits, 1, 1, &acyc) != 0 || cyc.cost < acyc.cost);
+    if (use_cyc) md->wrap = (uint64_t)cyc.t << cyc.lgN;
+    rc = make_operand(E, 1, 1, q_bits, i_b, &md->div, use_cyc ? &cyc : nullptr);
   }
   mpz_clear(I);
   mpz_clear(Bm);
@@ -236,6 +249,7 @@ int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
 }
 
 uint64_t rf_mod_limbs(const rf_mod* mod) { return mod ? mod->n : 0; }
+uint64_t rf_mod_wrap_limbs(const rf_mod* mod) { return mod ? mod->wrap : 0; }
 
 int rf_mod_mod_device(const rf_mod* const* mods, const rf_dev_entry* xs, int cnt, uint64_t* const* outs) {
   std::lock_guard<std::mutex> lk(engine_mutex());

@@ -235,6 +235,34 @@ int rf_plan_general(const RfPrimes* pr, size_t bits, unsigned terms, RfPlan* out
   return 0;
 }
 
+int rf_plan_cyclic(const RfPrimes* pr, uint64_t min_limbs, uint64_t a_limbs, uint64_t b_limbs, RfPlan* out) {
+  if (min_limbs == 0 || a_limbs == 0 || b_limbs == 0) return -1;
+  bool found = false;
+  RfPlan best;
+  memset(&best, 0, sizeof best);
+  for (int t = 1; t <= RF_TMAX; t++) {
+    int lgN = ceil_log2((min_limbs + t - 1) / t);
+    if (lgN < RF_LGN_MIN) lgN = RF_LGN_MIN;
+    if (lgN + 1 > RF_LGN_MAX) continue;
+    const uint64_t N = (uint64_t)1 << lgN;
+    if ((a_limbs + t - 1) / t > N || (b_limbs + t - 1) / t > N) continue;
+    // the acyclic plan of twice the length has the same chunks, so the same
+    // coefficient bound (prime count) and CRT shape
+    RfPlan pl;
+    if (rf_plan_fixed_k(pr, 64 * a_limbs, 64 * b_limbs, 1, 1, t, lgN + 1, 0, &pl) != 0) continue;
+    pl.lgN = lgN;
+    pl.split = 0;
+    pl.cyclic = 1;
+    pl.nout = (int)N;
+    pl.rn = (size_t)t * N + t + 1;
+    pl.cost *= 0.5;
+    if (!found || pl.cost < best.cost) best = pl, found = true;
+  }
+  if (!found) return -1;
+  *out = best;
+  return 0;
+}
+
 void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
   memset(c, 0, sizeof *c);
   for (int j = 0; j < pr->k; j++) {

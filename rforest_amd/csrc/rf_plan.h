@@ -50,6 +50,7 @@ struct RfPlan {
   size_t rn;    // output limbs per C entry (two's complement, sign included)
   double cost;  // planner's estimate, arbitrary units
   int split;    // 2^14 points as two rows of 2^13 (column + row passes) instead of one workgroup
+  int cyclic;   // 1: cyclic convolution of 2^lgN chunks, the product modulo 2^(64 t 2^lgN) - 1
 };
 
 // A tuning measurement: a square r x r product whose plan has chunk width
@@ -88,6 +89,14 @@ int rf_plan_fixed_k(const RfPrimes* pr, size_t LA, size_t LB, int r, int d, int 
 // most `terms` products x y with |x1 y1| + ... < 2^bits is recovered
 // exactly.  nA = nB = nout = the point count, rn = ceil(bits / 64) + 1.
 int rf_plan_general(const RfPrimes* pr, size_t bits, unsigned terms, RfPlan* out);
+
+// Plan for A B modulo B^W - 1 (B = 2^64) with W = t 2^lgN >= min_limbs, for
+// non-negative operands of at most a_limbs and b_limbs limbs (each <= W): a
+// cyclic convolution of the operands' 2^lgN chunks, no zero padding.  Every
+// coefficient sums at most min(nA, nB) products, the acyclic bound; the
+// recomposed sum has rn = W + t + 1 limbs, to be folded modulo B^W - 1.
+// Returns 0, or -1 if no chunk width fits.
+int rf_plan_cyclic(const RfPrimes* pr, uint64_t min_limbs, uint64_t a_limbs, uint64_t b_limbs, RfPlan* out);
 
 // ---- constant tables shared by all kernels (one copy in device memory) ----
 struct RfConsts {

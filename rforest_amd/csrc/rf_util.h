@@ -68,12 +68,14 @@ hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* m
 struct RfModTask {
   const uint64_t* x;   // |X|
   uint64_t xn;
-  const uint64_t* qd;  // Q D (non-negative), qdn limbs (0: Q = 0)
+  const uint64_t* qd;  // Q D (non-negative), qdn limbs (0: Q = 0); with wrap, a sum congruent to Q D mod B^wrap - 1
   uint64_t qdn;
   const uint64_t* d;   // D
   uint64_t n;
-  uint64_t* r;
-  uint64_t* t;
+  uint64_t* r;         // scratch: n + 1 limbs (wrap limbs with wrap)
+  uint64_t* t;         // scratch: as r
+  uint64_t* xf;        // scratch with wrap: wrap limbs (X folded)
+  uint64_t wrap;       // 0, or W: Q D was formed modulo B^W - 1 (W >= n + 1, xn and qdn <= 2 W)
   uint64_t* out;
   int32_t sign;        // sign of X: -1, 0, +1
   int32_t pad;

@@ -166,6 +166,58 @@ __device__ int sub_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uin
   return sh[12];
 }
 
+// a + b mod B^L into dst (dst may alias a or b); returns the carry out.
+// The carries into the lanes come from the same generate / propagate ballots
+// as sub_limbs' borrows: G = the lane's sum overflows, P = it is all ones.
+__device__ int add_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uint64_t L, int* sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();  // every thread has read the previous call's result
+  if (threadIdx.x == 0) sh[12] = 0;
+  __syncthreads();
+  for (uint64_t base = 0; base < L; base += RF_UTIL_THREADS) {
+    const uint64_t i = base + threadIdx.x;
+    const bool in = i < L;
+    const uint64_t x = in ? a.at(i) : 0, y = in ? b.at(i) : 0;
+    const uint64_t s = x + y;
+    const uint64_t G = __builtin_amdgcn_ballot_w64(in && s < x);
+    const uint64_t P = __builtin_amdgcn_ballot_w64(!in || s == ~(uint64_t)0);
+    const uint64_t X = G | P, Y = G;
+    const uint64_t S0 = X + Y, S1 = X + Y + 1;
+    if (lane == 0) {
+      sh[wave] = (int)(((X & Y) | ((X | Y) & ~S0)) >> 63);
+      sh[4 + wave] = (int)(((X & Y) | ((X | Y) & ~S1)) >> 63);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int c = sh[12];
+      for (int w = 0; w < RF_UTIL_THREADS / 64; w++) {
+        sh[8 + w] = c;
+        c = c ? sh[4 + w] : sh[w];
+      }
+      sh[12] = c;
+    }
+    __syncthreads();
+    const uint64_t S = sh[8 + wave] ? S1 : S0;
+    const uint64_t cin = ((S ^ X ^ Y) >> lane) & 1;
+    if (in) dst[i] = s + cin;
+    __syncthreads();  // sh[] is rewritten by the next tile
+  }
+  return sh[12];
+}
+
+// a mod (B^W - 1) into dst (W limbs) for a of at most 2 W limbs: a_lo + a_hi,
+// then the carry out (B^W = 1) added back.  a_lo + a_hi <= 2 (B^W - 1), so
+// after a carry the low part is at most B^W - 2 and adding 1 cannot carry
+// again.  The class of 0 may come out as B^W - 1.
+__device__ void fold_limbs(LimbSpan a, uint64_t* __restrict__ dst, uint64_t W, int* sh) {
+  const LimbSpan lo{a.p, a.n < W ? a.n : W};
+  const LimbSpan hi{a.n > W ? a.p + W : a.p, a.n > W ? a.n - W : 0};
+  if (add_limbs(lo, hi, dst, W, sh)) {
+    const uint64_t one = 1;
+    add_limbs(LimbSpan{dst, W}, LimbSpan{&one, 1}, dst, W, sh);
+  }
+}
+
 __global__ __launch_bounds__(RF_UTIL_THREADS) void k_mod_fix(const RfModTask* __restrict__ tasks, int ntasks) {
   __shared__ int sh[16];
   if ((int)blockIdx.x >= ntasks) return;
@@ -178,7 +230,29 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_mod_fix(const RfModTask* __
     for (uint64_t i = threadIdx.x; i < tk.n; i += RF_UTIL_THREADS) tk.out[i] = 0;
     return;
   }
-  sub_limbs(LimbSpan{tk.x, tk.xn}, LimbSpan{tk.qd, tk.qdn}, R, L, sh);  // X - QD mod B^(n+1)
+  if (tk.wrap) {
+    // R = (X - QD) mod (B^W - 1): exact, as 0 <= R < 3D < B^(n+1) - 1 <= B^W - 1
+    const uint64_t W = tk.wrap;
+    fold_limbs(LimbSpan{tk.x, tk.xn}, tk.xf, W, sh);
+    fold_limbs(LimbSpan{tk.qd, tk.qdn}, T, W, sh);
+    if (sub_limbs(LimbSpan{tk.xf, W}, LimbSpan{T, W}, R, W, sh)) {  // a borrow: add B^W - 1
+      const uint64_t one = 1;
+      sub_limbs(LimbSpan{R, W}, LimbSpan{&one, 1}, R, W, sh);
+    }
+    // B^W - 1 stands for 0; otherwise R's limbs from n + 1 on are zero
+    if (threadIdx.x == 0) sh[13] = 1;
+    __syncthreads();
+    for (uint64_t base = 0; base < W; base += RF_UTIL_THREADS) {
+      const uint64_t i = base + threadIdx.x;
+      if (__builtin_amdgcn_ballot_w64(i < W && R[i] != ~(uint64_t)0) && (threadIdx.x & 63) == 0) sh[13] = 0;
+    }
+    __syncthreads();
+    if (sh[13])
+      for (uint64_t i = threadIdx.x; i < L; i += RF_UTIL_THREADS) R[i] = 0;
+    __syncthreads();
+  } else {
+    sub_limbs(LimbSpan{tk.x, tk.xn}, LimbSpan{tk.qd, tk.qdn}, R, L, sh);  // X - QD mod B^(n+1)
+  }
   for (int k = 0; k < 2; k++) {
     const int borrow = sub_limbs(LimbSpan{R, L}, D, T, L, sh);  // R >= D iff no borrow
     if (!borrow) {

@@ -22,6 +22,8 @@ def _bind():
     L.rf_mod_create.argtypes = [ctypes.c_uint64, _vp, ctypes.POINTER(_vp)]
     L.rf_mod_limbs.restype = ctypes.c_uint64
     L.rf_mod_limbs.argtypes = [_vp]
+    L.rf_mod_wrap_limbs.restype = ctypes.c_uint64
+    L.rf_mod_wrap_limbs.argtypes = [_vp]
     L.rf_mod_mod.restype = ctypes.c_int
     L.rf_mod_mod.argtypes = [_vp, _vp, _vp]
     L.rf_mod_mod_device.restype = ctypes.c_int
@@ -46,6 +48,10 @@ class Modulus:
 
     def limbs(self):
         return int(_bind().rf_mod_limbs(self.handle))
+
+    def wrap_limbs(self):
+        """W if Q D is formed modulo 2^(64 W) - 1 (cyclic product), else 0."""
+        return int(_bind().rf_mod_wrap_limbs(self.handle))
 
     def mod(self, x):
         mx, mr = MpzArray([int(x)]), MpzArray([0])

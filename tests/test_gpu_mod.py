@@ -106,3 +106,23 @@ def test_remainder_with_long_transforms():
     M = Modulus(d, 1 << 27)
     xs = [rng.getrandbits(1 << 27) * s for s in (1, -1)]
     assert [M.mod(x) for x in xs] == mod_values(xs, [d, d])
+
+
+def test_cyclic_q_d_product():
+    """For a 1-Mbit modulus the Q D product is a cyclic convolution modulo
+    2^(64 W) - 1 (W >= limbs of D + 1), folded with X in the fix-up; random
+    and boundary dividends, one by one and as one batch, against x % d.  A
+    64-bit modulus is listed too, whichever form the cost model picks."""
+    rng = random.Random(77)
+    for dbits, xbits, cyclic in [(1 << 20, 1 << 21, True), (64, 200, None)]:
+        d = rng.getrandbits(dbits) | (1 << (dbits - 1))
+        M = Modulus(d, xbits)
+        W = M.wrap_limbs()
+        if cyclic:
+            assert W >= M.limbs() + 1
+        xs = [rng.getrandbits(xbits) * rng.choice([-1, 1]) for _ in range(3)] + \
+             [(1 << xbits) - 1, -((1 << xbits) - 1), 3 * d - 1, 2 * d, d - 1, 0, -1]
+        want = [x % d for x in xs]
+        assert [M.mod(x) for x in xs] == want
+        assert mod_batch([(M, x) for x in xs]) == want
+        M.close()
