@@ -112,17 +112,29 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_signmag(const RfConvTask* _
   }
 }
 
+// The remainder fix-up runs one remainder per workgroup of RF_FIX_THREADS
+// threads over tiles of as many limbs; the carry chain across its waves
+// keeps NW = RF_FIX_THREADS / 64 entries per kind in shared memory.
+#define RF_FIX_THREADS 1024
+#define FIX_NW (RF_FIX_THREADS / 64)
+#define FIX_C0 0
+#define FIX_C1 FIX_NW
+#define FIX_IN (2 * FIX_NW)
+#define FIX_TILE (3 * FIX_NW)
+#define FIX_FLAG (3 * FIX_NW + 1)
+#define FIX_SH (3 * FIX_NW + 2)
+
 // ---- multi-limb subtraction inside one workgroup ----
 //
 // dst = a - b over L limbs (limbs past an operand's length read 0), tiles of
-// RF_UTIL_THREADS limbs in order with the borrow carried between tiles.
+// RF_FIX_THREADS limbs in order with the borrow carried between tiles.
 // Within a wave, limb i generates a borrow (g: a_i < b_i) or propagates one
 // (p: a_i == b_i); with the wave's masks G and P (disjoint) the borrows into
 // its limbs are the carries of the addition x + y + c for x = G | P, y = G:
 // carry into bit i = bit i of (x + y + c) ^ x ^ y, carry out of bit 63 =
-// ((x & y) | ((x | y) & ~(x + y + c))) >> 63.  The four waves' carries are
-// chained by one thread.  Returns the borrow out of limb L-1.  Thread j
-// handles limbs j, j + RF_UTIL_THREADS, ... in every call, so a thread reads
+// ((x & y) | ((x | y) & ~(x + y + c))) >> 63.  The waves' carries are
+// chained by one thread (FIX_NW waves).  Returns the borrow out of limb L-1.  Thread j
+// handles limbs j, j + RF_FIX_THREADS, ... in every call, so a thread reads
 // back only what it wrote itself.
 struct LimbSpan {
   const uint64_t* p;
@@ -131,12 +143,13 @@ struct LimbSpan {
 };
 
 __device__ int sub_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uint64_t L, int* sh) {
-  // sh: [0..3] carry out with c=0, [4..7] with c=1, [8..11] wave carry-in, [12] tile borrow
+  // sh: [0, NW) carry out with c = 0, [NW, 2 NW) with c = 1, [2 NW, 3 NW) wave
+  // carry-in, [3 NW] tile borrow (NW waves)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();  // every thread has read the previous call's result
-  if (threadIdx.x == 0) sh[12] = 0;
+  if (threadIdx.x == 0) sh[FIX_TILE] = 0;
   __syncthreads();
-  for (uint64_t base = 0; base < L; base += RF_UTIL_THREADS) {
+  for (uint64_t base = 0; base < L; base += RF_FIX_THREADS) {
     const uint64_t i = base + threadIdx.x;
     const bool in = i < L;
     const uint64_t x = in ? a.at(i) : 0, y = in ? b.at(i) : 0;
@@ -145,25 +158,25 @@ __device__ int sub_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uin
     const uint64_t X = G | P, Y = G;
     const uint64_t S0 = X + Y, S1 = X + Y + 1;
     if (lane == 0) {
-      sh[wave] = (int)(((X & Y) | ((X | Y) & ~S0)) >> 63);
-      sh[4 + wave] = (int)(((X & Y) | ((X | Y) & ~S1)) >> 63);
+      sh[FIX_C0 + wave] = (int)(((X & Y) | ((X | Y) & ~S0)) >> 63);
+      sh[FIX_C1 + wave] = (int)(((X & Y) | ((X | Y) & ~S1)) >> 63);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      int c = sh[12];
-      for (int w = 0; w < RF_UTIL_THREADS / 64; w++) {
-        sh[8 + w] = c;
-        c = c ? sh[4 + w] : sh[w];
+      int c = sh[FIX_TILE];
+      for (int w = 0; w < RF_FIX_THREADS / 64; w++) {
+        sh[FIX_IN + w] = c;
+        c = c ? sh[FIX_C1 + w] : sh[FIX_C0 + w];
       }
-      sh[12] = c;
+      sh[FIX_TILE] = c;
     }
     __syncthreads();
-    const uint64_t S = sh[8 + wave] ? S1 : S0;
+    const uint64_t S = sh[FIX_IN + wave] ? S1 : S0;
     const uint64_t bin = ((S ^ X ^ Y) >> lane) & 1;
     if (in) dst[i] = x - y - bin;
     __syncthreads();  // sh[] is rewritten by the next tile
   }
-  return sh[12];
+  return sh[FIX_TILE];
 }
 
 // a + b mod B^L into dst (dst may alias a or b); returns the carry out.
@@ -172,9 +185,9 @@ __device__ int sub_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uin
 __device__ int add_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uint64_t L, int* sh) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();  // every thread has read the previous call's result
-  if (threadIdx.x == 0) sh[12] = 0;
+  if (threadIdx.x == 0) sh[FIX_TILE] = 0;
   __syncthreads();
-  for (uint64_t base = 0; base < L; base += RF_UTIL_THREADS) {
+  for (uint64_t base = 0; base < L; base += RF_FIX_THREADS) {
     const uint64_t i = base + threadIdx.x;
     const bool in = i < L;
     const uint64_t x = in ? a.at(i) : 0, y = in ? b.at(i) : 0;
@@ -184,25 +197,25 @@ __device__ int add_limbs(LimbSpan a, LimbSpan b, uint64_t* __restrict__ dst, uin
     const uint64_t X = G | P, Y = G;
     const uint64_t S0 = X + Y, S1 = X + Y + 1;
     if (lane == 0) {
-      sh[wave] = (int)(((X & Y) | ((X | Y) & ~S0)) >> 63);
-      sh[4 + wave] = (int)(((X & Y) | ((X | Y) & ~S1)) >> 63);
+      sh[FIX_C0 + wave] = (int)(((X & Y) | ((X | Y) & ~S0)) >> 63);
+      sh[FIX_C1 + wave] = (int)(((X & Y) | ((X | Y) & ~S1)) >> 63);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      int c = sh[12];
-      for (int w = 0; w < RF_UTIL_THREADS / 64; w++) {
-        sh[8 + w] = c;
-        c = c ? sh[4 + w] : sh[w];
+      int c = sh[FIX_TILE];
+      for (int w = 0; w < RF_FIX_THREADS / 64; w++) {
+        sh[FIX_IN + w] = c;
+        c = c ? sh[FIX_C1 + w] : sh[FIX_C0 + w];
       }
-      sh[12] = c;
+      sh[FIX_TILE] = c;
     }
     __syncthreads();
-    const uint64_t S = sh[8 + wave] ? S1 : S0;
+    const uint64_t S = sh[FIX_IN + wave] ? S1 : S0;
     const uint64_t cin = ((S ^ X ^ Y) >> lane) & 1;
     if (in) dst[i] = s + cin;
     __syncthreads();  // sh[] is rewritten by the next tile
   }
-  return sh[12];
+  return sh[FIX_TILE];
 }
 
 // a mod (B^W - 1) into dst (W limbs) for a of at most 2 W limbs: a_lo + a_hi,
@@ -218,8 +231,8 @@ __device__ void fold_limbs(LimbSpan a, uint64_t* __restrict__ dst, uint64_t W, i
   }
 }
 
-__global__ __launch_bounds__(RF_UTIL_THREADS) void k_mod_fix(const RfModTask* __restrict__ tasks, int ntasks) {
-  __shared__ int sh[16];
+__global__ __launch_bounds__(RF_FIX_THREADS) void k_mod_fix(const RfModTask* __restrict__ tasks, int ntasks) {
+  __shared__ int sh[FIX_SH];
   if ((int)blockIdx.x >= ntasks) return;
   const RfModTask tk = tasks[blockIdx.x];
   const uint64_t L = tk.n + 1;
@@ -227,7 +240,7 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_mod_fix(const RfModTask* __
   uint64_t* T = tk.t;
   const LimbSpan D{tk.d, tk.n};
   if (tk.sign == 0) {
-    for (uint64_t i = threadIdx.x; i < tk.n; i += RF_UTIL_THREADS) tk.out[i] = 0;
+    for (uint64_t i = threadIdx.x; i < tk.n; i += RF_FIX_THREADS) tk.out[i] = 0;
     return;
   }
   if (tk.wrap) {
@@ -240,15 +253,15 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_mod_fix(const RfModTask* __
       sub_limbs(LimbSpan{R, W}, LimbSpan{&one, 1}, R, W, sh);
     }
     // B^W - 1 stands for 0; otherwise R's limbs from n + 1 on are zero
-    if (threadIdx.x == 0) sh[13] = 1;
+    if (threadIdx.x == 0) sh[FIX_FLAG] = 1;
     __syncthreads();
-    for (uint64_t base = 0; base < W; base += RF_UTIL_THREADS) {
+    for (uint64_t base = 0; base < W; base += RF_FIX_THREADS) {
       const uint64_t i = base + threadIdx.x;
-      if (__builtin_amdgcn_ballot_w64(i < W && R[i] != ~(uint64_t)0) && (threadIdx.x & 63) == 0) sh[13] = 0;
+      if (__builtin_amdgcn_ballot_w64(i < W && R[i] != ~(uint64_t)0) && (threadIdx.x & 63) == 0) sh[FIX_FLAG] = 0;
     }
     __syncthreads();
-    if (sh[13])
-      for (uint64_t i = threadIdx.x; i < L; i += RF_UTIL_THREADS) R[i] = 0;
+    if (sh[FIX_FLAG])
+      for (uint64_t i = threadIdx.x; i < L; i += RF_FIX_THREADS) R[i] = 0;
     __syncthreads();
   } else {
     sub_limbs(LimbSpan{tk.x, tk.xn}, LimbSpan{tk.qd, tk.qdn}, R, L, sh);  // X - QD mod B^(n+1)
@@ -262,19 +275,19 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_mod_fix(const RfModTask* __
     }
   }
   if (tk.sign < 0) {
-    if (threadIdx.x == 0) sh[13] = 0;
+    if (threadIdx.x == 0) sh[FIX_FLAG] = 0;
     __syncthreads();
-    for (uint64_t base = 0; base < L; base += RF_UTIL_THREADS) {
+    for (uint64_t base = 0; base < L; base += RF_FIX_THREADS) {
       const uint64_t i = base + threadIdx.x;
-      if (__builtin_amdgcn_ballot_w64(i < L && R[i] != 0) && (threadIdx.x & 63) == 0) atomicOr(&sh[13], 1);
+      if (__builtin_amdgcn_ballot_w64(i < L && R[i] != 0) && (threadIdx.x & 63) == 0) atomicOr(&sh[FIX_FLAG], 1);
     }
     __syncthreads();
-    if (sh[13]) {
+    if (sh[FIX_FLAG]) {
       sub_limbs(D, LimbSpan{R, L}, T, L, sh);  // D - R
       R = T;
     }
   }
-  for (uint64_t i = threadIdx.x; i < tk.n; i += RF_UTIL_THREADS) tk.out[i] = R[i];
+  for (uint64_t i = threadIdx.x; i < tk.n; i += RF_FIX_THREADS) tk.out[i] = R[i];
 }
 
 __global__ void k_fill(uint64_t* __restrict__ dst, uint64_t n, uint64_t v) {
@@ -306,7 +319,7 @@ hipError_t rf_launch_fill(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t st)
 
 hipError_t rf_launch_mod_fix(const RfModTask* tasks, int ntasks, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mod_fix, dim3(ntasks), dim3(RF_UTIL_THREADS), 0, st, tasks, ntasks);
+  hipLaunchKernelGGL(k_mod_fix, dim3(ntasks), dim3(RF_FIX_THREADS), 0, st, tasks, ntasks);
   return hipGetLastError();
 }
 
