@@ -591,17 +591,23 @@ int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
 
 // Build, upload the task image through the pinned staging buffer, launch.
 // The arena must hold base + L.total() bytes (scratch starts at base).
-int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipStream_t st) {
+// stage_free: the caller guarantees that no copy in flight touches the
+// staging buffer's first L.tasks_b bytes (so no wait is needed before the
+// task image is written there), and that the buffer already holds them.
+int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipStream_t st,
+                 bool stage_free = false) {
   if (g.arena_cap < base + L.total()) return fail(RF_ERR_NOMEM, "arena too small for pipeline");
   Pipeline pp;
   int rc = build_pipeline(prods, pp);
   if (rc) return rc;
   char* sbase = g.d_arena + base;
   const ScratchPtrs S = scratch_ptrs(sbase, pp.L);
-  rc = ensure_stage(pp.image.size());
-  if (rc) return rc;
-  // Earlier copies out of the staging buffer must finish before it is reused.
-  HIPCHK(hipStreamSynchronize(st));
+  if (!stage_free) {
+    rc = ensure_stage(pp.image.size());
+    if (rc) return rc;
+    // Earlier copies out of the staging buffer must finish before it is reused.
+    HIPCHK(hipStreamSynchronize(st));
+  }
   if (!pp.image.empty()) {
     memcpy(g.h_stage, pp.image.data(), pp.image.size());
     HIPCHK(hipMemcpyAsync(S.tasks, g.h_stage, pp.image.size(), hipMemcpyHostToDevice, st));
@@ -615,11 +621,18 @@ uint64_t bitlen_limbs(const uint64_t* x, uint64_t n) {
 }
 
 // Host-array product: the engine behind every host entry point.  Entry e of
-// A is ap[e] (an[e] limbs, sign as[e]); likewise B.  Results come back as
-// two's complement, rn limbs per C entry, in out_tc.
+// A is ap[e] (an[e] limbs, sign as[e]); likewise B.  Per output entry e the
+// sink receives its rn limbs of two's complement, read from the staging
+// buffer (valid until the engine's next staging use; rn = 0: a zero entry).
+using TcSink = int (*)(void* ctx, int e, const uint64_t* tc, size_t rn);
+
+// One product through the staging buffer: inputs go after the task-image
+// region, so the inputs' copy, the task image's copy, the launches and the
+// result copy are all stream-ordered and the call waits once; the results
+// are handed to `sink` straight from the staging buffer.
 int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, const int* as,
-                 const uint64_t* const* bp, const uint64_t* bn, const int* bs,
-                 std::vector<std::vector<uint64_t>>& out_tc, const rf_operand* left = nullptr) {
+                 const uint64_t* const* bp, const uint64_t* bn, const int* bs, TcSink sink, void* ctx,
+                 const rf_operand* left = nullptr) {
   if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
   Product P;
   P.r = r;
@@ -643,19 +656,22 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
   int rc = plan_product(P);
   if (rc) return rc;
   const RfPlan pl = P.plan;
-  out_tc.assign(r * d, std::vector<uint64_t>());
-  if (pl.k == 0) return RF_OK;  // a zero operand: all products are zero
+  if (pl.k == 0) {  // a zero operand: all products are zero
+    for (int e = 0; e < r * d && !rc; e++) rc = sink(ctx, e, nullptr, 0);
+    return rc;
+  }
   std::vector<Product> prods{P};
   const Layout L = layout_of(prods);
   const size_t in_b = align256(total_limbs * 8);
   const size_t out_n = (size_t)r * d * pl.rn;
   const size_t out_b = align256(out_n * 8);
+  const size_t in_off = align256(L.tasks_b);  // staging: [task image | inputs], results over both
   rc = ensure_arena(in_b + out_b + L.total());
   if (rc) return rc;
-  rc = ensure_stage(std::max(std::max(in_b, out_b), L.tasks_b));
+  rc = ensure_stage(std::max(in_off + in_b, out_b));
   if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(g.stream));
-  uint64_t* hs = (uint64_t*)g.h_stage;
+  HIPCHK(hipStreamSynchronize(g.stream));  // an earlier call's copies out of the staging buffer
+  uint64_t* hs = (uint64_t*)(g.h_stage + in_off);
   Product& Q = prods[0];
   for (int e = left ? r * d : 0; e < r * d + d * d; e++) {
     const bool isA = e < r * d;
@@ -669,21 +685,37 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
   for (auto* v : {&Q.A, &Q.B})
     for (auto& E : *v) E.dptr = d_in + E.limb_off;
   for (int e = 0; e < r * d; e++) Q.out.push_back(d_out + (size_t)e * pl.rn);
-  rc = run_pipeline(prods, L, in_b + out_b, g.stream);
+  rc = run_pipeline(prods, L, in_b + out_b, g.stream, /*stage_free=*/true);
   if (rc) return rc;
-  // The task tables went through the staging buffer; the stream orders the
-  // result copy after them.
-  HIPCHK(hipStreamSynchronize(g.stream));
+  // stream order puts the result copy after the input and task copies that
+  // read the same staging bytes
   HIPCHK(hipMemcpyAsync(g.h_stage, d_out, out_n * 8, hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
   if (g.prof) prof_collect();
-  for (int e = 0; e < r * d; e++)
-    out_tc[e].assign((uint64_t*)g.h_stage + (size_t)e * pl.rn,
-                     (uint64_t*)g.h_stage + (size_t)(e + 1) * pl.rn);
-  return RF_OK;
+  for (int e = 0; e < r * d && !rc; e++) rc = sink(ctx, e, (const uint64_t*)g.h_stage + (size_t)e * pl.rn, pl.rn);
+  return rc;
 }
 
 // two's complement -> (magnitude limbs, negative?)
+// Sign and magnitude of the rn-limb two's complement tc into dst (rn limbs
+// of room); returns the magnitude's length without high zero limbs.
+size_t tc_signmag(const uint64_t* tc, size_t rn, uint64_t* dst, bool& neg) {
+  neg = rn && (tc[rn - 1] >> 63);
+  if (neg) {
+    uint64_t br = 1;
+    for (size_t i = 0; i < rn; i++) {
+      const uint64_t y = ~tc[i] + br;
+      br = (br && y == 0);
+      dst[i] = y;
+    }
+  } else if (dst != tc) {
+    memcpy(dst, tc, rn * 8);
+  }
+  size_t n = rn;
+  while (n && dst[n - 1] == 0) n--;
+  return n;
+}
+
 void tc_to_signmag(std::vector<uint64_t>& x, bool& neg) {
   neg = !x.empty() && (x.back() >> 63);
   if (neg) {
@@ -711,22 +743,21 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* le
     bn[e] = mpz_size(B[e]);
     bs[e] = mpz_sgn(B[e]);
   }
-  std::vector<std::vector<uint64_t>> out;
-  int rc = host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), out, left);
-  if (rc) return rc;
-  for (int e = 0; e < r * d; e++) {
-    bool neg = false;
-    tc_to_signmag(out[e], neg);
-    size_t n = out[e].size();
-    if (n == 0) {
-      mpz_set_ui(C[e], 0);
-      continue;
+  // results go straight from the staging buffer into C (no aliasing of C
+  // with A or B, src/include/hwmpz.h:128)
+  auto sink = [](void* ctx, int e, const uint64_t* tc, size_t rn) -> int {
+    mpz_t* Cv = (mpz_t*)ctx;
+    if (rn == 0) {
+      mpz_set_ui(Cv[e], 0);
+      return RF_OK;
     }
-    _mpz_realloc(C[e], (mp_size_t)n);  // src/lib/hwmpz.c:114
-    memcpy(C[e]->_mp_d, out[e].data(), n * 8);
-    C[e]->_mp_size = neg ? -(int)n : (int)n;  // src/lib/hwmpz.c:116-122
-  }
-  return RF_OK;
+    _mpz_realloc(Cv[e], (mp_size_t)rn);  // src/lib/hwmpz.c:114
+    bool neg = false;
+    const size_t n = tc_signmag(tc, rn, (uint64_t*)Cv[e]->_mp_d, neg);
+    Cv[e]->_mp_size = n == 0 ? 0 : (neg ? -(int)n : (int)n);  // src/lib/hwmpz.c:116-122
+    return RF_OK;
+  };
+  return host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), sink, C, left);
 }
 
 int engine_host_init() { return host_init_locked(); }
@@ -813,21 +844,27 @@ int rf_rmatrix_mult_limbs(uint64_t* c, const size_t* coff, const size_t* ccap, l
   std::vector<uint64_t> anv(r * d), bnv(d * d);
   for (int e = 0; e < r * d; e++) ap[e] = a + aoff[e], anv[e] = an[e];
   for (int e = 0; e < d * d; e++) bp[e] = b + boff[e], bnv[e] = bn[e];
-  std::vector<std::vector<uint64_t>> out;
-  int rc = host_product(r, d, ap.data(), anv.data(), as, bp.data(), bnv.data(), bs, out);
-  if (rc) return rc;
-  for (int e = 0; e < r * d; e++) {
-    bool neg = false;
-    if (out[e].empty()) {
-      csize[e] = 0;
-      continue;
+  struct Ctx {
+    uint64_t* c;
+    const size_t *coff, *ccap;
+    long* csize;
+    std::vector<uint64_t> tmp;
+  } cx{c, coff, ccap, csize, {}};
+  auto sink = [](void* vp, int e, const uint64_t* tc, size_t rn) -> int {
+    Ctx& x = *(Ctx*)vp;
+    if (rn == 0) {
+      x.csize[e] = 0;
+      return RF_OK;
     }
-    tc_to_signmag(out[e], neg);
-    if (out[e].size() > ccap[e]) return fail(RF_ERR_CAPACITY, "output %d needs %zu limbs", e, out[e].size());
-    memcpy(c + coff[e], out[e].data(), out[e].size() * 8);
-    csize[e] = neg ? -(long)out[e].size() : (long)out[e].size();
-  }
-  return RF_OK;
+    x.tmp.resize(rn);
+    bool neg = false;
+    const size_t n = tc_signmag(tc, rn, x.tmp.data(), neg);
+    if (n > x.ccap[e]) return fail(RF_ERR_CAPACITY, "output %d needs %zu limbs", e, n);
+    memcpy(x.c + x.coff[e], x.tmp.data(), n * 8);
+    x.csize[e] = neg ? -(long)n : (long)n;
+    return RF_OK;
+  };
+  return host_product(r, d, ap.data(), anv.data(), as, bp.data(), bnv.data(), bs, sink, &cx);
 }
 
 uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d) {
