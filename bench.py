@@ -290,7 +290,11 @@ def remainder_section(rf, workloads, reps):
     from rforest_amd.mod import ModBatch, Modulus
     pairs = workloads.remainder_level(256, seed=1007)
     t0 = time.perf_counter()
-    mods = [Modulus(d, 1 << 21) for (d, _) in pairs]
+    for (d, _) in pairs[:4]:
+        Modulus(d, 1 << 21).close()
+    t_create_one = (time.perf_counter() - t0) / 4
+    t0 = time.perf_counter()
+    mods = Modulus.batch([d for (d, _) in pairs], 1 << 21)
     t_create = (time.perf_counter() - t0) / len(mods)
     batch = ModBatch([(M, x) for M, (_, x) in zip(mods, pairs)])
     batch.run()
@@ -319,6 +323,8 @@ def remainder_section(rf, workloads, reps):
             "gmp_cpu_remainders_per_s": round(1 / t_cpu, 1),
             "gmp_cpu_sample": f"{nc} calls of mpz_mod on one host core",
             "modulus_create_ms": round(t_create * 1e3, 3),
+            "modulus_create_sample": "per modulus, the level's 256 prepared in one rf_mod_create_batch call",
+            "modulus_create_one_ms": round(t_create_one * 1e3, 3),
             "checks": {"identical_to_gmp": got == want, "one_per_call_identical": one == want[:ns]}}
 
 

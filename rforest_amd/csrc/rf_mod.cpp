@@ -17,6 +17,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "rf_internal.h"
@@ -168,24 +170,27 @@ int mod_batch_exec(ModBatch& mb, const rf_mod* const* mods, const rf_dev_entry* 
 
 }  // namespace
 
-extern "C" {
+namespace {
+using namespace rfi;
 
-int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
-  std::lock_guard<std::mutex> lk(engine_mutex());
-  if (!out) return fail(RF_ERR_ARG, "rf_mod_create: null output");
-  *out = nullptr;
-  if (!engine_ready()) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
-  if (!d || mpz_sgn(d) <= 0) return fail(RF_ERR_ARG, "rf_mod_create: the modulus must be positive");
-  if (max_bits == 0) return fail(RF_ERR_ARG, "rf_mod_create: max_bits must be positive");
-  rf_mod* md = new rf_mod();
-  md->n = mpz_size(d);
-  md->m = std::max<uint64_t>((max_bits + 63) / 64, md->n);
-  // I = floor(B^(m+1) / D), at most m - n + 2 limbs
-  mpz_t I, Bm;
-  mpz_init(I);
+// The host part of preparing a modulus: n, m and I = floor(B^(m+1) / D)
+// (GMP division; no engine state, so callers may run it in parallel).
+void mod_reciprocal(uint64_t max_bits, const mpz_t d, uint64_t* n, uint64_t* m, mpz_t I) {
+  *n = mpz_size(d);
+  *m = std::max<uint64_t>((max_bits + 63) / 64, *n);
+  mpz_t Bm;
   mpz_init(Bm);
-  mpz_setbit(Bm, 64 * (md->m + 1));
-  mpz_tdiv_q(I, Bm, d);
+  mpz_setbit(Bm, 64 * (*m + 1));
+  mpz_tdiv_q(I, Bm, d);  // at most m - n + 2 limbs
+  mpz_clear(Bm);
+}
+
+// The device part (engine mutex held): D and I uploaded and made resident
+// transforms.
+int mod_create_locked(const mpz_t d, uint64_t n, uint64_t m, const mpz_t I, rf_mod** out) {
+  rf_mod* md = new rf_mod();
+  md->n = n;
+  md->m = m;
   const uint64_t in = mpz_size(I);
   int rc = RF_OK;
   const size_t d_b = align256(md->n * 8), i_b = align256(in * 8);
@@ -235,8 +240,6 @@ int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
     if (use_cyc) md->wrap = (uint64_t)cyc.t << cyc.lgN;
     rc = make_operand(E, 1, 1, q_bits, i_b, &md->div, use_cyc ? &cyc : nullptr);
   }
-  mpz_clear(I);
-  mpz_clear(Bm);
   if (rc) {
     free_operand(md->inv);
     free_operand(md->div);
@@ -246,6 +249,73 @@ int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
   }
   *out = md;
   return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
+  if (!out) return fail(RF_ERR_ARG, "rf_mod_create: null output");
+  *out = nullptr;
+  if (!d || mpz_sgn(d) <= 0) return fail(RF_ERR_ARG, "rf_mod_create: the modulus must be positive");
+  if (max_bits == 0) return fail(RF_ERR_ARG, "rf_mod_create: max_bits must be positive");
+  {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    if (!engine_ready()) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  }
+  uint64_t n = 0, m = 0;
+  mpz_t I;
+  mpz_init(I);
+  mod_reciprocal(max_bits, d, &n, &m, I);  // outside the engine mutex
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    rc = engine_ready() ? mod_create_locked(d, n, m, I, out)
+                        : fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  }
+  mpz_clear(I);
+  return rc;
+}
+
+int rf_mod_create_batch(const uint64_t* max_bits, const mpz_t* ds, int cnt, rf_mod** outs) {
+  if (cnt < 0 || (cnt > 0 && (!max_bits || !ds || !outs))) return fail(RF_ERR_ARG, "rf_mod_create_batch: bad arguments");
+  for (int i = 0; i < cnt; i++) {
+    outs[i] = nullptr;
+    if (mpz_sgn(ds[i]) <= 0) return fail(RF_ERR_ARG, "rf_mod_create_batch: modulus %d is not positive", i);
+    if (max_bits[i] == 0) return fail(RF_ERR_ARG, "rf_mod_create_batch: max_bits[%d] must be positive", i);
+  }
+  {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    if (!engine_ready()) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  }
+  // the GMP divisions dominate a modulus's preparation: run them on up to
+  // 16 host threads, then make the resident transforms in order
+  std::vector<uint64_t> n(cnt), m(cnt);
+  std::vector<__mpz_struct> I(cnt);
+  for (int i = 0; i < cnt; i++) mpz_init(&I[i]);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int i = next++; i < cnt; i = next++) mod_reciprocal(max_bits[i], ds[i], &n[i], &m[i], &I[i]);
+  };
+  const int nth = std::max(1, std::min({cnt, 16, (int)std::thread::hardware_concurrency()}));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth; t++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  int rc = RF_OK;
+  {
+    std::lock_guard<std::mutex> lk(engine_mutex());
+    for (int i = 0; i < cnt && !rc; i++) rc = mod_create_locked(ds[i], n[i], m[i], &I[i], &outs[i]);
+  }
+  for (int i = 0; i < cnt; i++) mpz_clear(&I[i]);
+  if (rc)
+    for (int i = 0; i < cnt; i++)
+      if (outs[i]) {
+        rf_mod_destroy(outs[i]);
+        outs[i] = nullptr;
+      }
+  return rc;
 }
 
 uint64_t rf_mod_limbs(const rf_mod* mod) { return mod ? mod->n : 0; }

@@ -20,6 +20,8 @@ def _bind():
         return L
     L.rf_mod_create.restype = ctypes.c_int
     L.rf_mod_create.argtypes = [ctypes.c_uint64, _vp, ctypes.POINTER(_vp)]
+    L.rf_mod_create_batch.restype = ctypes.c_int
+    L.rf_mod_create_batch.argtypes = [ctypes.POINTER(ctypes.c_uint64), _vp, ctypes.c_int, ctypes.POINTER(_vp)]
     L.rf_mod_limbs.restype = ctypes.c_uint64
     L.rf_mod_limbs.argtypes = [_vp]
     L.rf_mod_wrap_limbs.restype = ctypes.c_uint64
@@ -36,15 +38,32 @@ def _bind():
 
 
 class Modulus:
-    def __init__(self, d, max_bits):
+    def __init__(self, d, max_bits, handle=None):
+        if handle is None:
+            L = _bind()
+            m = MpzArray([int(d)])
+            handle = _vp()
+            try:
+                check(L.rf_mod_create(int(max_bits), ctypes.cast(m.ptr(), _vp), ctypes.byref(handle)))
+            finally:
+                m.close()
+        self.handle, self.d, self.max_bits = handle, int(d), int(max_bits)
+
+    @classmethod
+    def batch(cls, ds, max_bits):
+        """rf_mod_create_batch: one Modulus per d in ds (max_bits: one value
+        or one per modulus)."""
         L = _bind()
-        m = MpzArray([int(d)])
-        h = _vp()
+        ds = [int(d) for d in ds]
+        mb = [int(max_bits)] * len(ds) if isinstance(max_bits, int) else [int(b) for b in max_bits]
+        m = MpzArray(ds)
+        hs = (_vp * max(len(ds), 1))()
         try:
-            check(L.rf_mod_create(int(max_bits), ctypes.cast(m.ptr(), _vp), ctypes.byref(h)))
+            check(L.rf_mod_create_batch((ctypes.c_uint64 * max(len(mb), 1))(*mb), ctypes.cast(m.ptr(), _vp),
+                                        len(ds), hs))
         finally:
             m.close()
-        self.handle, self.d, self.max_bits = h, int(d), int(max_bits)
+        return [cls(d, b, handle=_vp(hs[i])) for i, (d, b) in enumerate(zip(ds, mb))]
 
     def limbs(self):
         return int(_bind().rf_mod_limbs(self.handle))

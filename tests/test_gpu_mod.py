@@ -126,3 +126,24 @@ def test_cyclic_q_d_product():
         assert [M.mod(x) for x in xs] == want
         assert mod_batch([(M, x) for x in xs]) == want
         M.close()
+
+
+def test_batch_creation():
+    """rf_mod_create_batch: the same remainders as moduli created one by one
+    (mixed sizes and dividend bounds), and a bad modulus in the batch is an
+    error that leaves nothing created."""
+    rng = random.Random(91)
+    specs = [(64, 200), (1000, 2000), (1 << 20, 1 << 21), (777, 1 << 18), (4096, 4096)]
+    ds = [rng.getrandbits(b) | (1 << (b - 1)) for b, _ in specs]
+    mods = Modulus.batch(ds, [x for _, x in specs])
+    for M, d, (_, xb) in zip(mods, ds, specs):
+        xs = [rng.getrandbits(xb) * rng.choice([-1, 1]) for _ in range(3)] + [d - 1, 2 * d, -3 * d + 1]
+        xs = [x for x in xs if abs(x).bit_length() <= xb]
+        want = [x % d for x in xs]
+        assert [M.mod(x) for x in xs] == want
+        one = Modulus(d, xb)
+        assert [one.mod(x) for x in xs] == want and one.limbs() == M.limbs() and one.wrap_limbs() == M.wrap_limbs()
+        one.close()
+        M.close()
+    with pytest.raises(rf.RfError):
+        Modulus.batch([5, 0, 7], 64)
