@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -280,8 +281,8 @@ int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
 
 int rf_mod_create_batch(const uint64_t* max_bits, const mpz_t* ds, int cnt, rf_mod** outs) {
   if (cnt < 0 || (cnt > 0 && (!max_bits || !ds || !outs))) return fail(RF_ERR_ARG, "rf_mod_create_batch: bad arguments");
+  for (int i = 0; i < cnt; i++) outs[i] = nullptr;
   for (int i = 0; i < cnt; i++) {
-    outs[i] = nullptr;
     if (mpz_sgn(ds[i]) <= 0) return fail(RF_ERR_ARG, "rf_mod_create_batch: modulus %d is not positive", i);
     if (max_bits[i] == 0) return fail(RF_ERR_ARG, "rf_mod_create_batch: max_bits[%d] must be positive", i);
   }
@@ -300,12 +301,18 @@ int rf_mod_create_batch(const uint64_t* max_bits, const mpz_t* ds, int cnt, rf_m
   };
   const int nth = std::max(1, std::min({cnt, 16, (int)std::thread::hardware_concurrency()}));
   std::vector<std::thread> pool;
-  for (int t = 1; t < nth; t++) pool.emplace_back(work);
+  try {
+    for (int t = 1; t < nth; t++) pool.emplace_back(work);
+  } catch (const std::system_error&) {
+    // fewer helper threads: the calling thread takes what is left
+  }
   work();
   for (auto& t : pool) t.join();
   int rc = RF_OK;
   {
     std::lock_guard<std::mutex> lk(engine_mutex());
+    // hw_mpz_clear may have run while the divisions were unlocked
+    if (!engine_ready()) rc = fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
     for (int i = 0; i < cnt && !rc; i++) rc = mod_create_locked(ds[i], n[i], m[i], &I[i], &outs[i]);
   }
   for (int i = 0; i < cnt; i++) mpz_clear(&I[i]);
