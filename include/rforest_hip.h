@@ -220,7 +220,8 @@ int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out);
 /* mpzfft_mod_init for a whole level of moduli (a remainder tree's nodes):
    outs[i] for ds[i] with dividends |X| < 2^max_bits[i].  The reciprocal
    divisions, which dominate a modulus's preparation, run on up to 16 host
-   threads.  On an error no modulus is left created. */
+   threads.  On an error no modulus is left created and every outs[i] is
+   NULL. */
 int rf_mod_create_batch(const uint64_t* max_bits, const mpz_t* ds, int cnt, rf_mod** outs);
 /* Limbs of D, which every remainder of this modulus is written in. */
 uint64_t rf_mod_limbs(const rf_mod* mod);
