@@ -438,7 +438,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=1024, help="products per step")
-    ap.add_argument("--lanes", type=int, default=1, help="overlapping pipelines per prepared batch")
+    ap.add_argument("--lanes", type=int, default=3, help="overlapping pipelines per prepared batch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", type=float, default=0.0, help=argparse.SUPPRESS)
@@ -578,7 +578,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "products_per_step": args.batch, "r": 2, "d": 2,
                    "entry_bits": 1 << 20, "chunk_bits": 64 * t, "ntt_length": N, "primes": k,
-                   "prime_bits": 48, "parallelism": "single GPU"},
+                   "prime_bits": 48, "parallelism": "single GPU",
+                   "lanes": args.lanes},
         "roofline": {"bound": "hbm", "kernel": kern_names[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "alg_bytes_per_launch": alg[dom],
