@@ -11,13 +11,22 @@
 //   * rf_mulmod(a, w, wq): |a| < 2^53, 0 <= w < p, wq = fl(w / p)
 //       -> result == a*w (mod p) exactly, |result| <= p * (1/2 + |a| 2^-52).
 //     For |a| <= 2^51 this is |result| <= p.
+//   * rf_mulmod_var(a, b): |a|, |b| <= 2^49 -> result == a*b (mod p) exactly,
+//       |result| <= p/2 + |a b| 1.51 2^-52.
 //   * rf_reduce(x): |x| < 2^53 -> result == x (mod p), |result| <= p/2 + 8.
 #pragma once
 #include <stdint.h>
 
-#ifndef RF_HOST_ONLY
+// RF_FIELD_HOST compiles the same functions for the host (plain static
+// inline, for exhaustive checks against exact integer arithmetic);
+// RF_HOST_ONLY leaves them out.
+#if defined(RF_FIELD_HOST)
+#define RF_DEV static inline
+#elif !defined(RF_HOST_ONLY)
 #include <hip/hip_runtime.h>
 #define RF_DEV __device__ __forceinline__
+#endif
+#if defined(RF_FIELD_HOST) || !defined(RF_HOST_ONLY)
 
 // x mod p into |x| <= p/2 + 8.
 // q = rint(fl(x * pinv)) with pinv = fl(1/p): |fl(x*pinv) - x/p| <= |x/p| 2^-52
@@ -51,9 +60,13 @@ RF_DEV double rf_mulmod(double a, double w, double wq, double p) {
   return r + l;
 }
 
-// a * b mod p for two data operands; bq = fl(b * pinv) replaces fl(b / p),
-// relative error <= 2^-51, so |a b / p - q| <= 1/2 + |a| 2^-51 and the same
-// exactness argument applies for |a|, |b| <= 2^49.
+// a * b mod p for two data operands.  bq = fl(b * pinv) = (b / p)(1 + e) with
+// |e| <= 2^-52 + 2^-106, and fl(a * bq) rounds once more, so
+// |fl(a bq) - a b / p| <= |a b / p| 1.51 2^-52 and q = rint(fl(a bq)) leaves
+// |r| = |a b - q p| <= p/2 + |a b| 1.51 2^-52.  For |a|, |b| <= 2^49 that is
+// below 1.26 p < 2^49, so the exactness argument of rf_mulmod applies
+// (|l| <= 2^45, |r - l| < 2^53 an integer); for reduced operands
+// |a|, |b| <= p/2 + 8 it is below 0.53 p.
 RF_DEV double rf_mulmod_var(double a, double b, double p, double pinv) {
   double bq = b * pinv;
   double h = a * b;

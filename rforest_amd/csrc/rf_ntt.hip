@@ -20,7 +20,7 @@
 // Both directions use Cooley-Tukey butterflies (x, y) -> (x + w y, x - w y).
 // A stage adds at most p (1/2 + |y| 2^-52) to a residue's bound, or doubles it
 // where w = 1 is not multiplied (the first pass's block 0), and every eighth
-// stage reduces, so all values stay below 12.2 p + 160 < 2^51.4 (primes below
+// stage reduces, so all values stay below 12.2 p + 160 < 2^51.5 (primes below
 // 2^51 / 9; the derivation is at RF_PMAX).
 #include "rf_field.h"
 #include "rf_kernels.h"

@@ -17,7 +17,7 @@
 // twiddle is 1 and is not multiplied (block 0 of a transform's first pass, at
 // most four stages, all at the start of a window) at most doubles the bound B;
 // any other stage adds at most p (1/2 + B 2^-52).  Within a window values stay
-// below 16 (p/2 + 8) (1 + 1/18)^4 + 2.2 p < 12.2 p + 160 < 2^51.4, inside the
+// below 16 (p/2 + 8) (1 + 1/18)^4 + 2.2 p < 12.2 p + 160 < 2^51.5, inside the
 // |a| < 2^53 that rf_mulmod and rf_reduce need.
 #define RF_PMAX ((((uint64_t)1) << 51) / 9)
 #define RF_S 24           // every prime is 1 mod 2^RF_S
