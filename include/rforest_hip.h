@@ -38,7 +38,12 @@ void hw_mpz_clear(void);
 /* src/include/hwmpz.h:129 (src/lib/hwmpz.c:65-130): C (r x d) := A (r x d) *
    B (d x d), all row-major arrays of initialised mpz_t; C entries are
    reallocated as needed.  No aliasing.  w is unused scratch.  Any signs and
-   zeros.  On failure prints the reason and aborts (the reference exits). */
+   zeros, and any sizes, like the reference's CPU path: entries beyond one
+   device plan (2^32 bits) or beyond device memory are cut into pieces whose
+   products run on the device and are recombined on the host.  On failure
+   (not set up, a HIP error, host or device memory exhausted even for
+   2^16-bit pieces) prints the reason and aborts (the reference asserts or
+   exits, src/lib/hwmpz.c:69,146). */
 void mpz_rmatrix_mult_fft(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, mpz_t w);
 
 /* src/include/hwmpz.h:134 (src/lib/hwmpz.c:132-247): the reference's GPU
@@ -61,7 +66,7 @@ int mpz_rmatrix_gpu_fft_suitable(mpz_t* A, int r, mpz_t* B, int d);
 #define RF_ERR_NOMEM (-5)
 #define RF_ERR_CAPACITY (-6)    /* an output buffer is too small */
 
-/* Same as mpz_rmatrix_mult_fft; returns RF_OK or an RF_ERR_* code. */
+/* Same as mpz_rmatrix_mult_fft (any sizes); returns RF_OK or an RF_ERR_* code. */
 int rf_mpz_rmatrix_mult(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d);
 
 /* Description of the last error on this process. */
@@ -70,7 +75,9 @@ const char* rf_last_error(void);
 /* Same product on plain host limb arrays.  Entry e of A: magnitude limbs
    a + aoff[e], an[e] limbs, sign as[e] in {-1, 0, 1}; likewise B.  Output e:
    magnitude written to c + coff[e] (capacity ccap[e] limbs), signed size
-   (as _mp_size) to csize[e]. */
+   (as _mp_size) to csize[e].  One device plan per call: RF_ERR_UNSUPPORTED
+   for entries beyond 2^32 bits (the mpz_t entry points cut those into
+   pieces). */
 int rf_rmatrix_mult_limbs(uint64_t* c, const size_t* coff, const size_t* ccap, long* csize,
                           const uint64_t* a, const size_t* aoff, const size_t* an, const int* as,
                           const uint64_t* b, const size_t* boff, const size_t* bn, const int* bs,

@@ -42,6 +42,13 @@ int rf_stage_inv(const uint64_t* specA, const uint64_t* specB, int r, int d, int
    written as rn limbs of two's complement. */
 int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, uint64_t* out);
 
+/* Test hook of the drop-in (mpz_rmatrix_mult_fft, rf_mpz_rmatrix_mult):
+   products whose entries exceed `bits` bits are computed from pieces of
+   `bits` bits (the path otherwise taken only above the device plans' 2^32-bit
+   entries or when device memory runs out); 0 restores the default.  Lets the
+   piecewise recombination be checked at small sizes. */
+int rf_stage_dropin_piece_bits(uint64_t bits);
+
 /* Root of order 2^lgN used for prime index q. */
 uint64_t rf_root(int q, int lgN);
 

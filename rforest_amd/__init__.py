@@ -134,8 +134,50 @@ def lib():
     L.rf_operand_plan.argtypes = [ctypes.c_void_p, _intp, _intp, _intp]
     L.rf_operand_destroy.restype = None
     L.rf_operand_destroy.argtypes = [ctypes.c_void_p]
+    L.rf_stage_dropin_piece_bits.restype = ctypes.c_int
+    L.rf_stage_dropin_piece_bits.argtypes = [ctypes.c_uint64]
+    L.mpz_rmatrix_mult_fft.restype = None
+    L.mpz_rmatrix_mult_fft.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_void_p]
+    L.rf_mpz_rmatrix_mult.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_int]
     _lib = L
     return L
+
+
+def hip_runtime():
+    """The HIP runtime librforest_hip.so runs on (libamdhip64.so.7, loaded by
+    the library itself), for callers that pass streams of their own."""
+    lib()
+    h = ctypes.CDLL("libamdhip64.so.7")
+    h.hipStreamCreateWithFlags.restype = ctypes.c_int
+    h.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+    h.hipStreamSynchronize.restype = ctypes.c_int
+    h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    h.hipStreamDestroy.restype = ctypes.c_int
+    h.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    return h
+
+
+class HipStream:
+    """A caller-owned HIP stream (non-blocking, default priority) of the
+    library's runtime; .ptr is the hipStream_t handed to the C ABI."""
+
+    def __init__(self):
+        self._h = hip_runtime()
+        s = ctypes.c_void_p()
+        if self._h.hipStreamCreateWithFlags(ctypes.byref(s), 1) != 0:  # hipStreamNonBlocking
+            raise RfError(RF_ERR_HIP, "hipStreamCreateWithFlags failed")
+        self.ptr = s.value
+
+    def synchronize(self):
+        if self._h.hipStreamSynchronize(self.ptr) != 0:
+            raise RfError(RF_ERR_HIP, "hipStreamSynchronize failed")
+
+    def destroy(self):
+        if self.ptr:
+            self._h.hipStreamDestroy(self.ptr)
+            self.ptr = None
 
 
 def check(rc):

@@ -76,5 +76,28 @@ def build(verbose=False):
     return LIB
 
 
+C_CALLER_SRC = os.path.join(ROOT, "tests", "c", "test_dropin.c")
+C_CALLER_BIN = os.path.join(ROOT, "tests", "c", "test_dropin")
+
+
+def c_caller_cmd(out, extra=()):
+    """gcc command line for the C caller of the drop-in (tests/c/test_dropin.c):
+    C11 against include/rforest_hip.h, linked to the library and GMP."""
+    return (["gcc", "-std=gnu11", "-O2", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+             "-I" + os.path.join(GMP_PREFIX, "include")] + list(extra) +
+            [C_CALLER_SRC, "-o", out, "-L" + HERE, "-lrforest_hip", "-l:libgmp.so.10",
+             "-Wl,-rpath,$ORIGIN/../../rforest_amd"])
+
+
+def build_c_caller():
+    """Builds tests/c/test_dropin (needs the library built first)."""
+    res = subprocess.run(c_caller_cmd(C_CALLER_BIN + ".tmp"), capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building the C caller test program failed")
+    os.replace(C_CALLER_BIN + ".tmp", C_CALLER_BIN)
+    return C_CALLER_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

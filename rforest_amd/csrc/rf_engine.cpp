@@ -19,6 +19,7 @@
 #include <map>
 #include <atomic>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,10 @@ struct Ctx {
   double* d_twi = nullptr;
   RfDevTables tables{};
   hipStream_t stream = nullptr;
+  // Marks the end of the last asynchronous batch on a caller's stream; the
+  // library's stream waits on it, so every later user of the arena and the
+  // staging buffer (all ordered after the library's stream) waits too.
+  hipEvent_t foreign_ev = nullptr;
   char* d_arena = nullptr;
   size_t arena_cap = 0;
   char* h_stage = nullptr;
@@ -63,6 +68,14 @@ struct Ctx {
 
 Ctx g;
 std::mutex g_mu;
+// Incremented by every set-up: a prepared batch's captured graph holds the
+// table addresses of the set-up it was captured under, and is captured again
+// when the generation has changed.
+uint64_t g_gen = 0;
+// Held shared by request-server workers while they plan and enqueue against
+// the engine's tables without the engine mutex; hw_mpz_clear holds it
+// exclusively while it waits for the device and frees the tables.
+std::shared_mutex g_tables_rw;
 thread_local std::string g_err;
 
 // Twiddle tables of order 2^RF_LGN_MAX for transforms above 2^RF_LGN_TW
@@ -119,6 +132,7 @@ void load_tuned_plans() {
 
 int setup_locked() {
   if (g.ready) return RF_OK;
+  std::unique_lock<std::shared_mutex> wl(g_tables_rw);  // server workers see all of it or none
   int rc0 = host_init_locked();
   if (rc0) return rc0;
   RfConsts* hc = (RfConsts*)calloc(1, sizeof(RfConsts));
@@ -127,6 +141,7 @@ int setup_locked() {
   double roots[2 * RF_TW_PRIMES] = {0};
   rf_twiddle_roots(&g.primes, RF_LGN_TW, roots, roots + RF_TW_PRIMES);
   HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&g.foreign_ev, hipEventDisableTiming));
   HIPCHK(hipMalloc((void**)&g.d_consts, sizeof(RfConsts)));
   HIPCHK(hipMemcpy(g.d_consts, hc, sizeof(RfConsts), hipMemcpyHostToDevice));
   free(hc);
@@ -146,13 +161,20 @@ int setup_locked() {
   g.tables.tw_fwd = g.d_twf;
   g.tables.tw_inv = g.d_twi;
   g.ready = true;
+  g_gen++;
   load_tuned_plans();
   return RF_OK;
 }
 
+// Frees the tables and buffers.  Work the library enqueued on any stream
+// (asynchronous batches on callers' streams, prepared batches, server
+// slots) may still read them, so the whole device is waited for first;
+// server workers are held off by g_tables_rw meanwhile and find the engine
+// not ready afterwards.
 void clear_locked() {
   if (!g.ready) return;
-  hipStreamSynchronize(g.stream);
+  std::unique_lock<std::shared_mutex> wl(g_tables_rw);
+  (void)hipDeviceSynchronize();
   for (auto& e : g.pending) {
     hipEventDestroy(e.a);
     hipEventDestroy(e.b);
@@ -168,6 +190,7 @@ void clear_locked() {
   }
   if (g.d_arena) hipFree(g.d_arena);
   if (g.h_stage) hipHostFree(g.h_stage);
+  if (g.foreign_ev) hipEventDestroy(g.foreign_ev);
   hipStreamDestroy(g.stream);
   g = Ctx();
 }
@@ -605,8 +628,11 @@ int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipS
   if (!stage_free) {
     rc = ensure_stage(pp.image.size());
     if (rc) return rc;
-    // Earlier copies out of the staging buffer must finish before it is reused.
+    // Earlier copies out of the staging buffer, and kernels over the arena,
+    // must finish before either is reused: they may be on the library's
+    // stream or on st.
     HIPCHK(hipStreamSynchronize(st));
+    if (st != g.stream) HIPCHK(hipStreamSynchronize(g.stream));
   }
   if (!pp.image.empty()) {
     memcpy(g.h_stage, pp.image.data(), pp.image.size());
@@ -760,6 +786,93 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* le
   return host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), sink, C, left);
 }
 
+// ---- products beyond one device plan ----
+//
+// The reference's CPU mpz_rmatrix_mult_fft (src/lib/hwmpz.c:65-130) takes
+// entries of any size; the device plans stop at 2^32-bit entries (2^24
+// points of 512-bit chunks), and a large plan may not fit in device memory.
+// Then every entry x is cut into pieces of P bits carrying its sign,
+// x = sum_u x_u 2^(u P) with x_u = sgn(x) (floor(|x| / 2^(u P)) mod 2^P), and
+// C = sum_s 2^(s P) D_s with D_s = sum_{u+v=s} A_u B_v, where A_u B_v is an
+// ordinary (device) product of the piece matrices.  C is formed by Horner's
+// rule from the top diagonal down.
+constexpr uint64_t RF_PIECE_MAX = (uint64_t)1 << 31;  // pieces 2^31 x 2^31 bits: 2^23-point plans
+constexpr uint64_t RF_PIECE_MIN = (uint64_t)1 << 16;
+uint64_t g_piece_override = 0;  // rf_stage_dropin_piece_bits: cut above this size (0: only when needed)
+
+uint64_t max_bits(mpz_t* X, int n) {
+  uint64_t m = 0;
+  for (int e = 0; e < n; e++)
+    if (mpz_sgn(X[e])) m = std::max<uint64_t>(m, mpz_sizeinbase(X[e], 2));
+  return m;
+}
+
+struct MpzVec {
+  std::vector<__mpz_struct> v;
+  explicit MpzVec(size_t n) : v(n) {
+    for (auto& z : v) mpz_init(&z);
+  }
+  ~MpzVec() {
+    for (auto& z : v) mpz_clear(&z);
+  }
+  mpz_t* data() { return reinterpret_cast<mpz_t*>(v.data()); }
+  __mpz_struct* operator[](size_t i) { return &v[i]; }
+};
+
+// piece u (P bits, the entry's sign) of x into out
+void mpz_piece(__mpz_struct* out, const __mpz_struct* x, uint64_t u, uint64_t P, __mpz_struct* tmp) {
+  mpz_tdiv_q_2exp(tmp, x, u * P);
+  mpz_tdiv_r_2exp(out, tmp, P);
+}
+
+int mpz_product_pieces(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, uint64_t P) {
+  const uint64_t LA = max_bits(A, r * d), LB = max_bits(B, d * d);
+  const uint64_t na = (LA + P - 1) / P, nb = (LB + P - 1) / P;
+  if (na == 0 || nb == 0) {
+    for (int e = 0; e < r * d; e++) mpz_set_ui(C[e], 0);
+    return RF_OK;
+  }
+  const size_t rd = (size_t)r * d, dd = (size_t)d * d;
+  MpzVec Bp(nb * dd), Au(rd), T(rd), tmp(1);
+  for (uint64_t v = 0; v < nb; v++)
+    for (size_t e = 0; e < dd; e++) mpz_piece(Bp[v * dd + e], B[e], v, P, tmp[0]);
+  MpzVec D((na + nb - 1) * rd);  // diagonal sums D_s, entry after entry
+  for (uint64_t u = 0; u < na; u++) {
+    for (size_t e = 0; e < rd; e++) mpz_piece(Au[e], A[e], u, P, tmp[0]);
+    for (uint64_t v = 0; v < nb; v++) {
+      int rc = mpz_product(T.data(), Au.data(), r, Bp.data() + v * dd, d);
+      if (rc) return rc;
+      for (size_t e = 0; e < rd; e++) mpz_add(D[(u + v) * rd + e], D[(u + v) * rd + e], T[e]);
+    }
+  }
+  for (size_t e = 0; e < rd; e++) {
+    mpz_set(C[e], D[(na + nb - 2) * rd + e]);
+    for (uint64_t s = na + nb - 1; s-- > 0;) {
+      if (s == na + nb - 2) continue;
+      mpz_mul_2exp(C[e], C[e], P);
+      mpz_add(C[e], C[e], D[s * rd + e]);
+    }
+  }
+  return RF_OK;
+}
+
+// The drop-in: one device product when a plan exists and fits, otherwise the
+// product in pieces (halved further while the device runs out of memory).
+int mpz_product_any(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d) {
+  const uint64_t L = std::max(max_bits(A, r * d), max_bits(B, d * d));
+  uint64_t P = g_piece_override;
+  if (!P || L <= P) {
+    const int rc = mpz_product(C, A, r, B, d);
+    if (rc != RF_ERR_UNSUPPORTED && rc != RF_ERR_NOMEM) return rc;
+    P = std::min(RF_PIECE_MAX, ((L + 1) / 2 + 63) & ~(uint64_t)63);
+  }
+  for (;;) {
+    const int rc = mpz_product_pieces(C, A, r, B, d, P);
+    if (rc != RF_ERR_NOMEM || P <= RF_PIECE_MIN) return rc;
+    P = std::max(RF_PIECE_MIN, (P / 2 + 63) & ~(uint64_t)63);
+  }
+}
+
 int engine_host_init() { return host_init_locked(); }
 hipStream_t engine_stream() { return g.stream; }
 int engine_arena(size_t bytes, char** base) {
@@ -776,6 +889,7 @@ int engine_stage(size_t bytes, char** host) {
 }
 
 std::mutex& engine_mutex() { return g_mu; }
+std::shared_mutex& engine_tables_lock() { return g_tables_rw; }
 bool engine_ready() { return g.ready; }
 RfDevTables engine_tables() { return tables(); }
 int engine_long_tables(int lgN) { return ensure_long_tables(lgN); }
@@ -805,7 +919,15 @@ void hw_mpz_clear(void) {
 int rf_mpz_rmatrix_mult(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
-  return mpz_product(C, A, r, B, d);
+  if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
+  return mpz_product_any(C, A, r, B, d);
+}
+
+int rf_stage_dropin_piece_bits(uint64_t bits) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (bits && bits < 64) return fail(RF_ERR_ARG, "piece size below 64 bits");
+  g_piece_override = bits ? (bits + 63) & ~(uint64_t)63 : 0;
+  return RF_OK;
 }
 
 void mpz_rmatrix_mult_fft(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, mpz_t w) {
@@ -923,6 +1045,7 @@ struct rf_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
+  uint64_t graph_gen = 0;  // engine generation the graph was captured under
 };
 
 namespace {
@@ -973,6 +1096,11 @@ int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, i
   if (!async) {
     HIPCHK(hipStreamSynchronize(st));
     if (g.prof) prof_collect();
+  } else if (st != g.stream) {
+    // the pipeline still reads the arena and the task image: order every
+    // later arena or staging user after it
+    HIPCHK(hipEventRecord(g.foreign_ev, st));
+    HIPCHK(hipStreamWaitEvent(g.stream, g.foreign_ev, 0));
   }
   return RF_OK;
 }
@@ -1049,7 +1177,7 @@ int rf_batch_launch(rf_batch* b, void* stream, int use_graph) {
   if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
   hipStream_t st = stream ? (hipStream_t)stream : g.stream;
   if (!use_graph || g.prof) return batch_enqueue(b, st, g.prof);
-  if (b->exec && b->graph_stream != st) {
+  if (b->exec && (b->graph_stream != st || b->graph_gen != g_gen)) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -1072,6 +1200,7 @@ int rf_batch_launch(rf_batch* b, void* stream, int use_graph) {
     }
     b->graph = gr;
     b->graph_stream = st;
+    b->graph_gen = g_gen;
   }
   HIPCHK(hipGraphLaunch(b->exec, st));
   return RF_OK;

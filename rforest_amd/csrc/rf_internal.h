@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <shared_mutex>
 #include <vector>
 
 #include "../../include/rforest_hip.h"
@@ -85,6 +86,9 @@ struct ScratchPtrs {
 
 // The library's global lock, its readiness and its device tables.
 std::mutex& engine_mutex();
+// Held shared while planning and enqueueing against the tables without the
+// engine mutex (server workers); hw_mpz_clear takes it exclusively.
+std::shared_mutex& engine_tables_lock();
 bool engine_ready();
 // Device tables for a launch (a copy: long-table pointers may appear later).
 RfDevTables engine_tables();

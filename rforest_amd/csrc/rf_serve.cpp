@@ -22,6 +22,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -283,8 +284,12 @@ bool admit(rf_server* S, Slot& sl) {
       }
     }
   }
+  // the engine's tables stay in place until this batch is enqueued
+  // (hw_mpz_clear waits for the device under the exclusive lock)
+  std::shared_lock<std::shared_mutex> tl(engine_tables_lock());
   Pipeline pp;
-  int rc = build_pipeline(prods, pp);
+  int rc = engine_ready() ? build_pipeline(prods, pp)
+                          : fail(RF_ERR_NOT_INIT, "hw_mpz_clear() was called; requests wait for hw_mpz_setup()");
   // staging image: [task image | gen spans | gen blocks | sum spans | sum blocks]
   const size_t o_gs = align256(pp.image.size());
   const size_t o_gb = o_gs + align256(spans.size() * sizeof(RfSpan));
@@ -479,7 +484,11 @@ int rf_server_submit(rf_server* S, const rf_request* rq, uint64_t* id) {
     if (E.nlimbs && q->input == RF_IN_SYNTH) q->gen_spans++, q->gen_blocks += blocks_of(E.nlimbs);
     (e < r * d ? q->P.A : q->P.B).push_back(E);
   }
-  int rc = plan_product(q->P);
+  int rc;
+  {
+    std::shared_lock<std::shared_mutex> tl(engine_tables_lock());
+    rc = engine_ready() ? plan_product(q->P) : fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  }
   if (rc) {
     delete q;
     return rc;

@@ -221,9 +221,11 @@ class DeviceBatch:
                     slots.append(pi)
         return checksums_device(ptrs, ns, ents, slots, len(self.layout))
 
-    def run(self, sync=True):
-        """One-shot: plans, builds task tables and runs (rf_batch_mult_device)."""
-        check(lib().rf_batch_mult_device(self.prods, len(self.products), None, 0 if sync else 1))
+    def run(self, sync=True, stream=None):
+        """One-shot: plans, builds task tables and runs (rf_batch_mult_device),
+        on the library's stream or on `stream` (a HipStream)."""
+        check(lib().rf_batch_mult_device(self.prods, len(self.products), stream.ptr if stream else None,
+                                         0 if sync else 1))
 
     def run_one(self, i, sync=True):
         """Product i alone, one-shot (the plain per-product path)."""
@@ -239,10 +241,16 @@ class DeviceBatch:
             self.handle = h
         return self
 
-    def launch(self, graph=True):
-        """Enqueues the prepared pipeline; returns without waiting."""
+    def launch(self, graph=True, stream=None):
+        """Enqueues the prepared pipeline (on `stream`, a HipStream, or the
+        library's stream); returns without waiting."""
         self.prepare()
-        check(lib().rf_batch_launch(self.handle, None, 1 if graph else 0))
+        check(lib().rf_batch_launch(self.handle, stream.ptr if stream else None, 1 if graph else 0))
+
+    def clear_outputs(self):
+        """Zeroes the output buffer (so a later launch is seen to write it)."""
+        import numpy as np
+        self.out.upload(np.zeros(max(self.total_out, 1), dtype=np.uint64))
 
     def release(self):
         if self.handle is not None:

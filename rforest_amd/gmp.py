@@ -81,6 +81,10 @@ class MpzArray:
     def ptr(self):
         return ctypes.cast(self.arr, ctypes.c_void_p)
 
+    def ref(self, i):
+        """Pointer to entry i (an mpz_ptr)."""
+        return ctypes.byref(self.arr[i])
+
     def close(self):
         if self.arr is not None:
             g = libgmp()
@@ -93,6 +97,22 @@ class MpzArray:
             self.close()
         except Exception:
             pass
+
+
+_fn = {}
+
+
+def mpz_fn(name, argtypes, restype=None):
+    """A GMP mpz function (gmp.h's mpz_<name>, symbol __gmpz_<name>) with
+    the given argument types; for arithmetic on entries too large for
+    Python integers."""
+    f = _fn.get(name)
+    if f is None:
+        f = getattr(ctypes.CDLL(GMP_LIB), "__gmpz_" + name)
+        f.argtypes = argtypes
+        f.restype = restype
+        _fn[name] = f
+    return f
 
 
 def rmatrix_mult_schoolbook(A, r, B, d):

@@ -165,16 +165,21 @@ def test_dropin_mpz_abi():
         assert rf.lib().rf_mpz_rmatrix_mult(mC2.ptr(), mA.ptr(), r, mB.ptr(), d) == rf.RF_OK
 
 
-def test_unsupported_size_returns_error():
-    # one entry of 2^33 bits needs 2^25 points at the widest chunks (the
-    # longest plan has 2^24); a sparse entry keeps the host side cheap
-    rng = random.Random(1)
-    big = [1 << ((1 << 33) - 1) | 12345] + [rng.getrandbits(64) for _ in range(3)]
-    mA = MpzArray(big)
-    mC = MpzArray.zeros(4)
-    assert rf.lib().mpz_rmatrix_gpu_fft_suitable(mA.ptr(), 2, mA.ptr(), 2) == 0
-    rc = rf.lib().rf_mpz_rmatrix_mult(mC.ptr(), mA.ptr(), 2, mA.ptr(), 2)
-    assert rc == rf.RF_ERR_UNSUPPORTED
+def test_plain_array_path_refuses_sizes_without_a_plan():
+    """rf_rmatrix_mult_limbs (plain arrays, one device plan per call) returns
+    RF_ERR_UNSUPPORTED for a product of two 2^33-bit entries, which needs 2^25
+    points at the widest chunks (the longest plan has 2^24); the mpz_t drop-in computes
+    such products from pieces instead (test_gpu_pieces.py)."""
+    import numpy as np
+    big = np.zeros(1 << 27, dtype=np.uint64)  # 2^33 bits, sparse
+    big[-1] = 1 << 63
+    big[0] = 12345
+    with pytest.raises(rf.RfError) as ei:
+        rf.rmatrix_mult_limb_arrays([(1, big)], 1, [(-1, big)], 1)
+    assert ei.value.code == rf.RF_ERR_UNSUPPORTED
+    mA = MpzArray([1 << 40])
+    assert rf.lib().mpz_rmatrix_gpu_fft_suitable(mA.ptr(), 1, mA.ptr(), 1) == 1
+    mA.close()
 
 
 def test_device_batch_mixed_level_matches_alone():
