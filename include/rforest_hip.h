@@ -124,6 +124,22 @@ uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d);
    otherwise after completion. */
 int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, int async);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (serving loops under a fixed memory budget, harnesses that forbid
+   hipMalloc).  The same products as rf_batch_mult_device, with
+   the pipeline's scratch (spectra, coefficient residues, task tables, carry
+   codes) in a fixed device array of the library (64 MiB per GPU, held from
+   module load until the process ends) instead of the engine's arena: the
+   call makes no device or pinned allocation of any kind.  Products whose
+   scratch exceeds the array, or whose plans would need the twiddle tables
+   of transforms above 2^21 points made first, are refused with
+   RF_ERR_CAPACITY.  The array keeps the last call's contents and every call
+   in the process on that GPU shares it; calls are serialised by the
+   library's lock and each returns after its work completes on `stream`
+   (NULL: the library's stream), so they never overlap -- two calls running
+   at once on one GPU would race on it. */
+int rf_batch_mult_device_noalloc(const rf_dev_product* prods, int nprod, void* stream);
+
 /* A prepared batch: the products, their plans, task tables and scratch
    memory fixed once, for callers that run the same level shape repeatedly
    (serving loops, benchmarks, graph replay).  Inputs are read and outputs

@@ -134,6 +134,8 @@ def lib():
     L.rf_operand_plan.argtypes = [ctypes.c_void_p, _intp, _intp, _intp]
     L.rf_operand_destroy.restype = None
     L.rf_operand_destroy.argtypes = [ctypes.c_void_p]
+    L.rf_batch_mult_device_noalloc.restype = ctypes.c_int
+    L.rf_batch_mult_device_noalloc.argtypes = [ctypes.POINTER(DevProduct), ctypes.c_int, ctypes.c_void_p]
     L.rf_stage_dropin_piece_bits.restype = ctypes.c_int
     L.rf_stage_dropin_piece_bits.argtypes = [ctypes.c_uint64]
     L.mpz_rmatrix_mult_fft.restype = None

@@ -27,6 +27,7 @@
 #include "../../include/rforest_hip_stages.h"
 #include "rf_internal.h"
 #include "rf_kernels.h"
+#include "rf_noalloc.h"
 #include "rf_plan.h"
 #include "rf_util.h"
 
@@ -160,6 +161,7 @@ int setup_locked() {
   g.tables.consts = g.d_consts;
   g.tables.tw_fwd = g.d_twf;
   g.tables.tw_inv = g.d_twi;
+  HIPCHK(rf_noalloc_init());  // this device's address of the no-allocation scratch
   g.ready = true;
   g_gen++;
   load_tuned_plans();
@@ -332,7 +334,15 @@ bool tuned_plan(uint64_t LA, uint64_t LB, int r, int d, RfPlan* out) {
   return found;
 }
 
-int plan_product(Product& P) {
+int tables_for(int lgN, bool make) {
+  if (make) return ensure_long_tables(lgN);
+  if (lgN > RF_LGN_TW && !g_twi_long.load(std::memory_order_acquire))
+    return fail(RF_ERR_CAPACITY, "a plan of 2^%d points needs twiddle tables that are not made (no allocation allowed)",
+                lgN);
+  return RF_OK;
+}
+
+int plan_product(Product& P, bool make_tables) {
   uint64_t LA = 0, LB = 0;
   for (auto& e : P.A) LA = std::max(LA, e.sign ? e.bits : 0);
   for (auto& e : P.B) LB = std::max(LB, e.sign ? e.bits : 0);
@@ -357,12 +367,12 @@ int plan_product(Product& P) {
                   (unsigned long long)LB);
     return RF_OK;
   }
-  if (LA && LB && tuned_plan(LA, LB, P.r, P.d, &P.plan)) return ensure_long_tables(P.plan.lgN);
+  if (LA && LB && tuned_plan(LA, LB, P.r, P.d, &P.plan)) return tables_for(P.plan.lgN, make_tables);
   if (rf_plan_choose(&g.primes, LA, LB, P.r, P.d, &P.plan) != 0)
     return fail(RF_ERR_UNSUPPORTED,
                 "no transform plan for entries of %llu x %llu bits (r=%d, d=%d)",
                 (unsigned long long)LA, (unsigned long long)LB, P.r, P.d);
-  return ensure_long_tables(P.plan.lgN);
+  return tables_for(P.plan.lgN, make_tables);
 }
 
 uint64_t out_limbs_for(uint64_t la_bits, uint64_t lb_bits, int d) {
@@ -996,7 +1006,7 @@ uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d) {
 }  // extern "C"
 
 namespace rfi {
-int products_from_dev(const rf_dev_product* prods, int nprod, std::vector<Product>& ps) {
+int products_from_dev(const rf_dev_product* prods, int nprod, std::vector<Product>& ps, bool make_tables = true) {
   if (nprod < 0 || (nprod > 0 && !prods)) return fail(RF_ERR_ARG, "bad product list");
   ps.assign(nprod, Product());
   for (int i = 0; i < nprod; i++) {
@@ -1016,7 +1026,7 @@ int products_from_dev(const rf_dev_product* prods, int nprod, std::vector<Produc
       E.bits = E.sign ? 64 * x.nlimbs : 0;
       (e < P.r * P.d ? P.A : P.B).push_back(E);
     }
-    int rc = plan_product(P);
+    int rc = plan_product(P, make_tables);
     if (rc) return rc;
     const uint64_t rn = P.plan.k ? P.plan.rn : 0;  // a zero product writes no limbs
     for (int e = 0; e < P.r * P.d; e++) {
@@ -1102,6 +1112,33 @@ int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, i
     HIPCHK(hipEventRecord(g.foreign_ev, st));
     HIPCHK(hipStreamWaitEvent(g.stream, g.foreign_ev, 0));
   }
+  return RF_OK;
+}
+
+int rf_batch_mult_device_noalloc(const rf_dev_product* prods, int nprod, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  char* scratch = nullptr;
+  HIPCHK(rf_noalloc_scratch(&scratch));
+  hipStream_t st = stream ? (hipStream_t)stream : g.stream;
+  std::vector<Product> ps;
+  int rc = products_from_dev(prods, nprod, ps, /*make_tables=*/false);
+  if (rc) return rc;
+  Pipeline pp;
+  rc = build_pipeline(ps, pp);
+  if (rc) return rc;
+  if (pp.L.total() > RF_NOALLOC_SCRATCH_BYTES)
+    return fail(RF_ERR_CAPACITY, "products need %zu bytes of scratch; the fixed scratch holds %zu",
+                pp.L.total(), (size_t)RF_NOALLOC_SCRATCH_BYTES);
+  // the task image goes from host memory straight into the scratch's task
+  // area; the call returns after completion, so the image outlives the copy
+  if (!pp.image.empty())
+    HIPCHK(hipMemcpyAsync(scratch_ptrs(scratch, pp.L).tasks, pp.image.data(), pp.image.size(),
+                          hipMemcpyHostToDevice, st));
+  rc = launch_pipeline(pp, scratch, st, g.prof);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  if (g.prof) prof_collect();
   return RF_OK;
 }
 

@@ -119,7 +119,10 @@ hipStream_t engine_stream();
 int engine_arena(size_t bytes, char** base);
 int engine_stage(size_t bytes, char** host);
 
-int plan_product(Product& P);
+// Plans one product.  make_tables: create the twiddle tables of order 2^24
+// when the plan needs them (false: a plan that needs missing tables fails
+// with RF_ERR_CAPACITY instead, for callers that may not allocate).
+int plan_product(Product& P, bool make_tables = true);
 Layout layout_of(std::vector<Product>& prods);
 int build_pipeline(std::vector<Product>& prods, Pipeline& pp);
 ScratchPtrs scratch_ptrs(char* base, const Layout& L);

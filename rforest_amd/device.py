@@ -227,6 +227,11 @@ class DeviceBatch:
         check(lib().rf_batch_mult_device(self.prods, len(self.products), stream.ptr if stream else None,
                                          0 if sync else 1))
 
+    def run_noalloc(self, stream=None):
+        """The same products through rf_batch_mult_device_noalloc (scratch in
+        the library's fixed device array; returns after completion)."""
+        check(lib().rf_batch_mult_device_noalloc(self.prods, len(self.products), stream.ptr if stream else None))
+
     def run_one(self, i, sync=True):
         """Product i alone, one-shot (the plain per-product path)."""
         ptr = ctypes.cast(ctypes.byref(self.prods, i * ctypes.sizeof(DevProduct)), ctypes.POINTER(DevProduct))
