@@ -83,7 +83,7 @@ def cpu_baseline(products, gpu_results, budget_s):
             mismatches += int(C != gpu_results[i])
         n += 1
     distinct = min(n, len(products))
-    return {"value": n / el, "unit": "products/s", "cores": 1, "kind": kind,
+    return {"value": n / el, "unit": "products/s", "cores": 1, "kind": kind, "cpu_model": cpu_model(),
             "median_latency_ms": round(sorted(lat)[len(lat) // 2] * 1e3, 3),
             "sample": f"{n} products ({distinct} distinct) of the same workload, one thread, {el:.1f} s; {desc}",
             "gpu_results_identical": mismatches == 0}
@@ -113,11 +113,26 @@ def cpu_worker(seconds, batch):
     print(json.dumps({"n": n, "s": time.perf_counter() - t0}), flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_all_cores(seconds, batch):
-    """All-core CPU throughput: one worker process per core (at most 16),
-    started before this process touches the GPU."""
+    """All-core CPU throughput: one worker process per core this process may
+    use, at most 16 -- the host CPU share that comes with one GPU on these
+    boxes, whatever the machine's own core count -- started before this
+    process touches the GPU."""
     import subprocess
-    workers = max(1, min(16, os.cpu_count() or 1))
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = max(1, min(16, allowed))
     procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(seconds),
                                "--batch", str(batch)], stdout=subprocess.PIPE, text=True)
              for _ in range(workers)]
@@ -127,7 +142,43 @@ def cpu_all_cores(seconds, batch):
         rec = json.loads(out.strip().splitlines()[-1])
         n += rec["n"]
         wall = max(wall, rec["s"])
-    return {"workers": workers, "products_per_s": round(n / wall, 2), "products": n, "seconds": round(wall, 2)}
+    return {"workers": workers, "products_per_s": round(n / wall, 2), "products": n, "seconds": round(wall, 2),
+            "host_logical_cpus": os.cpu_count(), "affinity_cpus": allowed, "cpu_model": cpu_model(),
+            "workers_rule": "one worker per core allowed to this process, capped at the 16-core host share "
+                            "that comes with one GPU"}
+
+
+def dropin_section(rf, products, seconds):
+    """The drop-in as rforest calls it: mpz_rmatrix_mult_fft on host mpz_t,
+    one headline product per call (PCIe transfers, staging and the sign
+    epilogue included), the mpz arrays made once; per-call latency
+    percentiles over `seconds` of back-to-back calls."""
+    from rforest_amd.gmp import MpzArray
+    n = min(8, len(products))
+    arrs = [(MpzArray(A), MpzArray(B), MpzArray.zeros(4)) for (_, _, A, B) in products[:n]]
+    w = MpzArray.zeros(1)
+    L = rf.lib()
+    for i in range(2 * n):  # warm: plans, staging buffers
+        mA, mB, mC = arrs[i % n]
+        L.mpz_rmatrix_mult_fft(mC.ptr(), mA.ptr(), 2, mB.ptr(), 2, w.ptr())
+    lat = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or len(lat) < 100:
+        mA, mB, mC = arrs[len(lat) % n]
+        t0 = time.perf_counter()
+        L.mpz_rmatrix_mult_fft(mC.ptr(), mA.ptr(), 2, mB.ptr(), 2, w.ptr())
+        lat.append(time.perf_counter() - t0)
+    exact = all(arrs[i][2].values() == rf.mpz_rmatrix_mult_naive(products[i][2], 2, products[i][3], 2)
+                for i in range(min(2, n)))
+    for a in arrs:
+        for m in a:
+            m.close()
+    w.close()
+    return {"workload": "one 2x2 . 2x2 product of 1,048,576-bit entries per mpz_rmatrix_mult_fft call, "
+                        "host mpz_t in and out (PCIe included)",
+            "calls": len(lat), "products_per_s": round(len(lat) / sum(lat), 1),
+            "p50_us": round(_pct(lat, 50) * 1e6, 1), "p99_us": round(_pct(lat, 99) * 1e6, 1),
+            "checks": {"exact_vs_gmp": exact}}
 
 
 def tree_level_section(rf, workloads, reps):
@@ -442,8 +493,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", type=float, default=0.0, help=argparse.SUPPRESS)
-    ap.add_argument("--dropin", action="store_true",
-                    help="also time the host mpz drop-in (PCIe-inclusive, one product per call)")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip timing the host mpz drop-in (PCIe-inclusive, one product per call)")
     ap.add_argument("--requests", type=int, default=20000, help="requests in the served stream")
     ap.add_argument("--no-extra", action="store_true",
                     help="headline only (skip the tree-level and resident-operand sections)")
@@ -534,14 +585,9 @@ def main():
     stage_ms = {s: round(prof[s][0] / psteps, 4) for s in prof}
 
     dropin = None
-    if args.dropin:
-        # PCIe-inclusive drop-in (host mpz in, host mpz out), one product per call
-        rf.mpz_rmatrix_mult_fft(products[0][2], 2, products[0][3], 2)
-        nd = min(8, args.batch)
-        t1 = time.perf_counter()
-        for i in range(nd):
-            rf.mpz_rmatrix_mult_fft(products[i][2], 2, products[i][3], 2)
-        dropin = round(nd / (time.perf_counter() - t1), 2)
+    if not args.no_dropin:
+        log("drop-in: one product per mpz_rmatrix_mult_fft call")
+        dropin = dropin_section(rf, products, 2.0)
 
     extra = {}
     if not args.no_extra:
@@ -591,7 +637,7 @@ def main():
                                 "butterflies_per_product": bfly,
                                 "gbutterflies_per_s": round(bfly * args.batch / ntt_launch_s / 1e9, 2)},
         "stage_ms_per_step": stage_ms,
-        "dropin_pcie_products_per_s": dropin,
+        "dropin": dropin,
         "cpu_baseline": cpu,
         "checks": {"exact_product0": exact0, "mod_2^61-1_mismatches": bad},
     }

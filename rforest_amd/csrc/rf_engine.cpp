@@ -16,9 +16,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <atomic>
 #include <mutex>
+#include <unordered_map>
 #include <shared_mutex>
 #include <string>
 #include <vector>
@@ -342,6 +344,27 @@ int tables_for(int lgN, bool make) {
   return RF_OK;
 }
 
+// Plans of products without a resident operand depend only on (LA, LB, r,
+// d) and the tuning rules; repeated shapes (a drop-in called in a loop, a
+// tree level of equal sizes) reuse them.  Cleared whenever the rules change.
+struct PlanKey {
+  uint64_t la, lb;
+  int r, d;
+  bool operator==(const PlanKey& o) const { return la == o.la && lb == o.lb && r == o.r && d == o.d; }
+};
+struct PlanKeyHash {
+  size_t operator()(const PlanKey& k) const {
+    return std::hash<uint64_t>()(k.la * 0x9E3779B97F4A7C15ull ^ k.lb) ^ ((size_t)k.r << 40) ^ ((size_t)k.d << 48);
+  }
+};
+std::mutex g_plan_mu;
+std::unordered_map<PlanKey, RfPlan, PlanKeyHash> g_plan_cache;
+
+void plan_cache_clear() {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_plan_cache.clear();
+}
+
 int plan_product(Product& P, bool make_tables) {
   uint64_t LA = 0, LB = 0;
   for (auto& e : P.A) LA = std::max(LA, e.sign ? e.bits : 0);
@@ -367,11 +390,25 @@ int plan_product(Product& P, bool make_tables) {
                   (unsigned long long)LB);
     return RF_OK;
   }
-  if (LA && LB && tuned_plan(LA, LB, P.r, P.d, &P.plan)) return tables_for(P.plan.lgN, make_tables);
-  if (rf_plan_choose(&g.primes, LA, LB, P.r, P.d, &P.plan) != 0)
+  const PlanKey key{LA, LB, P.r, P.d};
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plan_cache.find(key);
+    if (it != g_plan_cache.end()) {
+      P.plan = it->second;
+      return tables_for(P.plan.lgN, make_tables);
+    }
+  }
+  if (!(LA && LB && tuned_plan(LA, LB, P.r, P.d, &P.plan)) &&
+      rf_plan_choose(&g.primes, LA, LB, P.r, P.d, &P.plan) != 0)
     return fail(RF_ERR_UNSUPPORTED,
                 "no transform plan for entries of %llu x %llu bits (r=%d, d=%d)",
                 (unsigned long long)LA, (unsigned long long)LB, P.r, P.d);
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (g_plan_cache.size() >= 4096) g_plan_cache.clear();
+    g_plan_cache[key] = P.plan;
+  }
   return tables_for(P.plan.lgN, make_tables);
 }
 
@@ -980,19 +1017,18 @@ int rf_rmatrix_mult_limbs(uint64_t* c, const size_t* coff, const size_t* ccap, l
     uint64_t* c;
     const size_t *coff, *ccap;
     long* csize;
-    std::vector<uint64_t> tmp;
-  } cx{c, coff, ccap, csize, {}};
+  } cx{c, coff, ccap, csize};
   auto sink = [](void* vp, int e, const uint64_t* tc, size_t rn) -> int {
     Ctx& x = *(Ctx*)vp;
     if (rn == 0) {
       x.csize[e] = 0;
       return RF_OK;
     }
-    x.tmp.resize(rn);
+    std::vector<uint64_t> tmp(rn);
     bool neg = false;
-    const size_t n = tc_signmag(tc, rn, x.tmp.data(), neg);
+    const size_t n = tc_signmag(tc, rn, tmp.data(), neg);
     if (n > x.ccap[e]) return fail(RF_ERR_CAPACITY, "output %d needs %zu limbs", e, n);
-    memcpy(x.c + x.coff[e], x.tmp.data(), n * 8);
+    memcpy(x.c + x.coff[e], tmp.data(), n * 8);
     x.csize[e] = neg ? -(long)n : (long)n;
     return RF_OK;
   };
@@ -1615,6 +1651,7 @@ int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int*
 int rf_tune_set(int r, int lgN, int t, int split, double cost_us) {
   if (r <= 0 || lgN < RF_LGN_MIN || lgN > RF_LGN_MAX || t < 1 || t > RF_TMAX || !(cost_us >= 0))
     return fail(RF_ERR_ARG, "rf_tune_set: bad measurement");
+  plan_cache_clear();
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (auto& x : g_tune)
     if (x.r == r && x.lgN == lgN && x.t == t && x.split == (split != 0)) {
@@ -1626,6 +1663,7 @@ int rf_tune_set(int r, int lgN, int t, int split, double cost_us) {
 }
 
 void rf_tune_clear(void) {
+  plan_cache_clear();
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune.clear();
 }
