@@ -17,6 +17,8 @@ import random
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -406,6 +408,110 @@ def _run_stream(rf, serve, reqs, cfg, rate=None):
     return {c["tag"]: c for c in comps}, t1 - t0, st
 
 
+def _run_stream_host(rf, serve, reqs, limbs, outs, cfg, rate=None):
+    """As _run_stream, with every request's entries supplied from host
+    memory (RF_IN_HOST) and its result copied back (RF_OUT_HOST) into the
+    caller's buffers `outs`: PCIe both ways."""
+    srv = serve.Server(**cfg)
+    t0 = srv.clock()
+    due = 0.0
+    comps, ids = [], []
+    out = serve.RF_OUT_HOST | serve.RF_OUT_CHECKSUM
+    for i, (r, bits, signs, seed, gap) in enumerate(reqs):
+        if rate:
+            due += gap / rate
+            while srv.clock() - t0 < due:
+                comps += srv.poll(4096, 0)
+        ids.append(srv.submit_host_limbs(r, r, limbs[i], tag=i, output=out, out=outs[i]))
+    comps += srv.wait_all(len(reqs) - len(comps))
+    t1 = max(c["t_done"] for c in comps)
+    for rid in ids:  # drop the host result buffers
+        srv._keep.pop(rid, None)
+    srv.close()
+    return {c["tag"]: c for c in comps}, t1 - t0
+
+
+def serving_host_section(rf, workloads, sat, n_host):
+    """The first n_host requests of the served stream with host-memory
+    inputs and outputs (the PCIe-inclusive served figure), saturated and
+    open loop at 0.7 x the saturated rate; each result's checksum must equal
+    the synthetic run's (sat) for the same request."""
+    from rforest_amd import serve
+    reqs = workloads.served_stream(n_host, seed=1005)
+    limbs = [[(sg, serve.gen_limbs(b, seed, e), b) for e, (b, sg) in enumerate(zip(bits, signs))]
+             for (r, bits, signs, seed, _) in reqs]
+    nbytes = sum(a.nbytes for ents in limbs for (_, a, _) in ents)
+    # the caller's result buffers, allocated and touched once (a serving
+    # loop reuses its buffers; first-touch page faults are not the server's)
+    sizer = serve.Server(1 << 30, streams=1)
+    outs = [np.ones(r * r * sizer.out_limbs_for(r, r, ents), dtype=np.uint64)
+            for (r, _, _, _, _), ents in zip(reqs, limbs)]
+    sizer.close()
+    cfg = dict(hbm_budget=64 * 10**9, streams=3, max_batch=256)
+    got, t_sat = _run_stream_host(rf, serve, reqs, limbs, outs, cfg)
+    same = all(got[i]["checksum"] == sat[i]["checksum"] and got[i]["status"] == rf.RF_OK for i in range(n_host))
+    rate = 0.7 * n_host / t_sat
+    op, _ = _run_stream_host(rf, serve, reqs, limbs, outs, cfg, rate)
+    lat = [c["t_done"] - c["t_submit"] for c in op.values()]
+    return {"workload": f"first {n_host} requests of the served stream, entries in host memory (RF_IN_HOST), "
+                        f"results copied back (RF_OUT_HOST): {nbytes / 1e9:.2f} GB of input over PCIe",
+            "products_per_s": round(n_host / t_sat, 1),
+            "open_loop": {"rate_per_s": round(rate, 1), "p50_ms": round(_pct(lat, 50) * 1e3, 3),
+                          "p99_ms": round(_pct(lat, 99) * 1e3, 3)},
+            "checks": {"identical_to_synthetic_run": same}}
+
+
+def serving_resident_section(rf, workloads, n_req, n_ops):
+    """Requests that reuse left operands kept resident in the NTT domain
+    inside the server (rf_request.left), against the same requests carrying
+    the left matrix themselves: n_ops resident 3x3 operands of
+    1,048,576-bit entries, n_req requests each naming one (seeded), right
+    3x3 operands of 1,048,576-bit synthetic entries generated in HBM."""
+    from rforest_amd import serve
+    rng = workloads.GmpRandom(1008)
+    As = [[rng.signed(1 << 20) for _ in range(9)] for _ in range(n_ops)]
+    pick = [rng.urandomb(16) % n_ops for _ in range(n_req)]
+    rng.close()
+    ops = [rf.ResidentOperand(A, 3, 3, 1 << 20) for A in As]
+    bits = [1 << 20] * 9
+    cfg = dict(hbm_budget=64 * 10**9, streams=3, max_batch=256)
+
+    def run(resident):
+        srv = serve.Server(**cfg)
+        t0 = srv.clock()
+        for i, j in enumerate(pick):
+            if resident:
+                srv.submit_synth(3, 3, bits, [1] * 9, 1008 * (1 << 20) + i, tag=i, left=ops[j])
+            else:  # A generated in HBM too (the same sizes; the operand's values are not needed for timing)
+                srv.submit_synth(3, 3, bits + bits, [1] * 18, 1008 * (1 << 20) + i, tag=i)
+        comps = srv.wait_all(n_req)
+        el = max(c["t_done"] for c in comps) - t0
+        ok = all(c["status"] == rf.RF_OK for c in comps)
+        srv.close()
+        return n_req / el, ok, {c["tag"]: c for c in comps}
+
+    res_rate, ok1, comps = run(True)
+    plain_rate, ok2, _ = run(False)
+    # exactness: a sample of resident results against GMP on the host-generated right operands
+    exact = True
+    srv = serve.Server(**cfg)
+    for i in range(0, n_req, max(1, n_req // 4)):
+        B = [serve.gen_value(1 << 20, 1, 1008 * (1 << 20) + i, e) for e in range(9)]
+        rid = srv.submit_host(3, 3, None, B, tag=i, left=ops[pick[i]])
+        (c,) = srv.wait_all(1)
+        C = srv.host_result(rid, c["out_limbs"])
+        exact = exact and C == rf.mpz_rmatrix_mult_naive(As[pick[i]], 3, B, 3)
+    srv.close()
+    for op in ops:
+        op.release()
+    return {"workload": f"{n_req} requests 3x3 . 3x3 with 1,048,576-bit entries, each naming one of {n_ops} left "
+                        "operands kept resident in the NTT domain (GMP seed 1008); right operands generated in HBM",
+            "resident_products_per_s": round(res_rate, 1),
+            "left_supplied_products_per_s": round(plain_rate, 1),
+            "speedup": round(res_rate / plain_rate, 2),
+            "checks": {"all_ok": ok1 and ok2, "exact_sample_vs_gmp": exact}}
+
+
 def serving_section(rf, workloads, n_req, n_plain, n_dropin, n_open):
     """The mixed request stream through the request server (continuous
     batching, 64 GB HBM budget, synthetic entries generated in device memory
@@ -463,8 +569,10 @@ def serving_section(rf, workloads, n_req, n_plain, n_dropin, n_open):
     pp, _, _ = _run_stream(rf, serve, reqs[:n_open], plain_cfg, rate)
     lat = [c["t_done"] - c["t_submit"] for c in op.values()]
     plat = [c["t_done"] - c["t_submit"] for c in pp.values()]
+    host = serving_host_section(rf, workloads, sat, min(1000, n_req))
     return {"workload": f"{n_req} requests, r in {{2,3,4}} w.p. {{0.6,0.3,0.1}}, entries log-uniform in [2^12, 2^24] bits, "
                         "random signs, GMP seed 1005; entries generated in HBM at admission; 64 GB HBM budget",
+            "host_fed": host,
             "served_products_per_s": round(served_rate, 1),
             "plain_products_per_s": round(plain_rate, 1),
             "plain_sample": f"first {n_plain} requests, same server with one slot and one product per batch",
@@ -602,6 +710,8 @@ def main():
         log("serving: mixed request stream")
         extra["serving"] = serving_section(rf, workloads, args.requests, min(2000, args.requests), 50,
                                            min(5000, args.requests))
+        log("serving: resident left operands")
+        extra["serving_resident"] = serving_resident_section(rf, workloads, 512, 16)
 
     cpu = None
     if not args.no_cpu:

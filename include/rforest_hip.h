@@ -278,7 +278,8 @@ int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d);
 
 /* ---------------------------------------------------------------------------
  * Request server (no reference counterpart; the reference serves one
- * synchronous call at a time).  Continuous batching: a worker thread admits
+ * synchronous call at a time; its transform-domain reuse is mpzfft_fft,
+ * src/lib/mpzfft.c:10-23).  Continuous batching: a worker thread admits
  * queued requests in arrival order whenever a stream slot is free and the
  * request's device memory (inputs, outputs and its share of the batch's
  * spectra, coefficients and task tables) fits in the HBM budget, runs each
@@ -305,12 +306,18 @@ typedef struct rf_request {
   int32_t r, d;
   int32_t input;               /* RF_IN_* */
   int32_t output;              /* RF_OUT_* flags */
-  const rf_serve_entry* A;     /* r*d, copied at submission */
+  const rf_serve_entry* A;     /* r*d, copied at submission (ignored when left is set) */
   const rf_serve_entry* B;     /* d*d */
   uint64_t seed;               /* RF_IN_SYNTH */
   uint64_t* out_host;          /* RF_OUT_HOST */
   uint64_t out_cap;            /* RF_OUT_HOST: limbs per C entry */
   uint64_t tag;                /* returned in the completion */
+  /* NULL, or a resident left operand (rf_operand_create*, same r and d) used
+     as A: its NTT-domain spectra are read in place by every request naming
+     it, so only B is transferred and transformed.  The caller keeps it alive
+     until every request naming it has completed; B must fit the operand's
+     plan (RF_ERR_ARG at submission otherwise). */
+  const rf_operand* left;
 } rf_request;
 
 typedef struct rf_completion {

@@ -19,8 +19,10 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <functional>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <shared_mutex>
 #include <thread>
@@ -83,6 +85,80 @@ class Pool {
 
 constexpr size_t NONE = ~(size_t)0;
 
+// Copy threads of a server: the host copies of RF_IN_HOST inputs into the
+// pinned staging buffer and of RF_OUT_HOST results out of it run on these
+// and the worker thread together (one thread copies far below the host's
+// memory bandwidth).  Small copies stay on the worker.
+class CopyPool {
+ public:
+  explicit CopyPool(int n) {
+    for (int i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // f(i) for i in [0, n) on the pool and the calling thread; returns when done.
+  void run(size_t n, const std::function<void(size_t)>& f, size_t bytes) {
+    if (n <= 1 || th_.empty() || bytes < ((size_t)8 << 20)) {
+      for (size_t i = 0; i < n; i++) f(i);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(mu_);
+    job_ = &f;
+    n_ = n;
+    next_ = 0;
+    left_ = n;
+    gen_++;
+    cv_.notify_all();
+    lk.unlock();
+    work();
+    lk.lock();
+    done_.wait(lk, [&] { return left_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+      }
+      work();
+    }
+  }
+  void work() {
+    for (;;) {
+      const std::function<void(size_t)>* f;
+      size_t i;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!job_ || next_ >= n_) return;
+        f = job_;
+        i = next_++;
+      }
+      (*f)(i);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--left_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(size_t)>* job_ = nullptr;
+  size_t n_ = 0, next_ = 0, left_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
 struct Req {
   uint64_t id = 0, tag = 0, seed = 0;
   int r = 0, d = 0, input = 0, output = 0;
@@ -93,6 +169,7 @@ struct Req {
   uint64_t rn = 0;  // limbs per C entry (0: the product is zero)
   size_t in_bytes = 0, out_bytes = 0, alone_bytes = 0;
   size_t in_off = NONE, out_off = NONE;
+  size_t st_in = NONE, st_out = NONE;  // host inputs / outputs in the slot's pinned staging buffer
   size_t gen_spans = 0, gen_blocks = 0, sum_spans = 0, sum_blocks = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
   uint64_t checksum = 0;
@@ -113,6 +190,12 @@ struct Slot {
 
 size_t blocks_of(uint64_t limbs) { return (size_t)((limbs + RF_UTIL_LIMBS - 1) / RF_UTIL_LIMBS); }
 
+// Host inputs and results pass through the slot's pinned staging buffer
+// (one host copy and one DMA transfer per request each way, instead of a
+// transfer per entry from pageable memory); a batch stages at most this
+// many bytes of them, unless its first request alone needs more.
+constexpr size_t kSlotHostBytes = (size_t)512 << 20;
+
 // Device bytes of the span/block tables a set of requests needs (four
 // 256-aligned arrays: generation spans and blocks, checksum spans and blocks).
 size_t util_bytes(size_t spans, size_t blocks) {
@@ -132,6 +215,7 @@ struct rf_server {
   std::unordered_map<uint64_t, Req*> kept;
   std::vector<rf_completion> done;
   std::vector<Slot> slots;
+  std::unique_ptr<CopyPool> copiers;
   std::thread worker;
   bool stop = false;
   uint64_t next_id = 1;
@@ -164,6 +248,20 @@ void free_req_mem(rf_server* S, Req* q, bool keep_out) {
 }
 
 void retire(rf_server* S, Slot& sl, int status) {
+  // host results: from the slot's staging buffer to the callers' arrays
+  // (the batch has completed, and the slot is not reused before this returns)
+  if (status == RF_OK) {
+    std::vector<std::pair<Req*, int>> jobs;  // (request, C entry)
+    size_t bytes = 0;
+    for (Req* q : sl.reqs)
+      if (q->status == RF_OK && q->st_out != NONE)
+        for (int e = 0; e < q->r * q->d; e++) jobs.push_back({q, e}), bytes += q->rn * 8;
+    S->copiers->run(jobs.size(), [&](size_t i) {
+      Req* q = jobs[i].first;
+      const int e = jobs[i].second;
+      memcpy(q->out_host + (size_t)e * q->out_cap, sl.h_stage + q->st_out + (size_t)e * q->rn * 8, q->rn * 8);
+    }, bytes);
+  }
   const double t = S->now();
   std::lock_guard<std::mutex> lk(S->mu);
   for (size_t i = 0; i < sl.reqs.size(); i++) {
@@ -189,13 +287,22 @@ void retire(rf_server* S, Slot& sl, int status) {
 
 int ensure_slot_stage(Slot& sl, size_t bytes) {
   if (bytes <= sl.stage_cap) return RF_OK;
+  const size_t old_cap = sl.stage_cap;
   if (sl.h_stage) HIPCHK(hipHostFree(sl.h_stage));
   sl.h_stage = nullptr;
+  sl.stage_cap = old_cap;  // the growth below starts from the old size
+  // pinning memory is slow (about 0.1 s per GB here): grow geometrically so
+  // a slot re-pins only a few times
+  const size_t cap = std::max({bytes, 2 * sl.stage_cap, (size_t)1 << 20});
   sl.stage_cap = 0;
-  const size_t cap = std::max(bytes, (size_t)1 << 20);
   HIPCHK(hipHostMalloc((void**)&sl.h_stage, cap, hipHostMallocDefault));
   sl.stage_cap = cap;
   return RF_OK;
+}
+
+size_t host_io_bytes(const Req* q) {
+  return (q->input == RF_IN_HOST ? align256(q->in_bytes) : 0) +
+         ((q->output & RF_OUT_HOST) && q->rn ? align256(q->out_bytes) : 0);
 }
 
 // Takes queued requests for a free slot and enqueues their batch.  Returns
@@ -207,10 +314,13 @@ bool admit(rf_server* S, Slot& sl) {
   size_t ub = 0, soff = NONE, sbytes = 0;
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    size_t bytes = 0;
+    size_t bytes = 0, hbytes = 0;
     while (!S->queue.empty() && (int)take.size() < S->cfg.max_batch) {
       Req* q = S->queue.front();
       if (!take.empty() && bytes + q->alone_bytes > S->cfg.max_batch_bytes) break;
+      const size_t qh = host_io_bytes(q);
+      if (!take.empty() && hbytes + qh > kSlotHostBytes) break;
+      hbytes += qh;
       size_t io = NONE, oo = NONE;
       if (q->in_bytes && !S->pool.alloc(q->in_bytes, &io)) break;
       if (q->out_bytes && !S->pool.alloc(q->out_bytes, &oo)) {
@@ -296,7 +406,13 @@ bool admit(rf_server* S, Slot& sl) {
   const size_t o_ss = o_gb + align256(blocks.size() * sizeof(RfBlock));
   const size_t o_sb = o_ss + align256(sspans.size() * sizeof(RfSpan));
   const size_t o_end = o_sb + align256(sblocks.size() * sizeof(RfBlock));
-  if (!rc) rc = ensure_slot_stage(sl, o_end);
+  // then [host inputs | host results] of each request that has them
+  size_t o_host = o_end;
+  for (Req* q : take) {
+    if (q->input == RF_IN_HOST && q->in_bytes) q->st_in = o_host, o_host += align256(q->in_bytes);
+    if ((q->output & RF_OUT_HOST) && q->rn) q->st_out = o_host, o_host += align256(q->out_bytes);
+  }
+  if (!rc) rc = ensure_slot_stage(sl, o_host);
   if (!rc) {
     char* h = sl.h_stage;
     memcpy(h, pp.image.data(), pp.image.size());
@@ -304,6 +420,23 @@ bool admit(rf_server* S, Slot& sl) {
     if (!blocks.empty()) memcpy(h + o_gb, blocks.data(), blocks.size() * sizeof(RfBlock));
     if (!sspans.empty()) memcpy(h + o_ss, sspans.data(), sspans.size() * sizeof(RfSpan));
     if (!sblocks.empty()) memcpy(h + o_sb, sblocks.data(), sblocks.size() * sizeof(RfBlock));
+    struct Cp {
+      char* dst;
+      const void* src;
+      size_t n;
+    };
+    std::vector<Cp> cps;
+    size_t bytes = 0;
+    for (Req* q : take) {
+      if (q->st_in == NONE) continue;
+      uint64_t pos = 0;
+      for (size_t e = 0; e < q->ents.size(); e++) {
+        const uint64_t n = q->ents[e].sign ? (q->ents[e].bits + 63) / 64 : 0;
+        if (n) cps.push_back(Cp{h + q->st_in + pos * 8, q->ents[e].limbs, n * 8}), bytes += n * 8;
+        pos += n;
+      }
+    }
+    S->copiers->run(cps.size(), [&](size_t i) { memcpy(cps[i].dst, cps[i].src, cps[i].n); }, bytes);
   }
   // device side: the task image sits in the layout's task area, the util
   // tables after the layout
@@ -314,18 +447,9 @@ bool admit(rf_server* S, Slot& sl) {
     if (!pp.image.empty()) HIPCHK(hipMemcpyAsync(SP.tasks, sl.h_stage, pp.image.size(), hipMemcpyHostToDevice, sl.st));
     if (o_end > o_gs)
       HIPCHK(hipMemcpyAsync(d_util, sl.h_stage + o_gs, o_end - o_gs, hipMemcpyHostToDevice, sl.st));
-    for (size_t i = 0; i < take.size(); i++) {
-      Req* q = take[i];
-      if (q->input != RF_IN_HOST) continue;
-      uint64_t pos = 0;
-      for (size_t e = 0; e < q->ents.size(); e++) {
-        const uint64_t n = q->ents[e].sign ? (q->ents[e].bits + 63) / 64 : 0;
-        if (n)
-          HIPCHK(hipMemcpyAsync((uint64_t*)(base + q->in_off) + pos, q->ents[e].limbs, n * 8,
-                                hipMemcpyHostToDevice, sl.st));
-        pos += n;
-      }
-    }
+    for (Req* q : take)
+      if (q->st_in != NONE)
+        HIPCHK(hipMemcpyAsync(base + q->in_off, sl.h_stage + q->st_in, q->in_bytes, hipMemcpyHostToDevice, sl.st));
     HIPCHK(rf_launch_gen((const RfSpan*)(d_util), (const RfBlock*)(d_util + (o_gb - o_gs)), (int)blocks.size(),
                          sl.st));
     int r2 = launch_pipeline(pp, sbase, sl.st, false);
@@ -336,12 +460,10 @@ bool admit(rf_server* S, Slot& sl) {
                                 (int)sblocks.size(), sl.d_sums, sl.st));
       HIPCHK(hipMemcpyAsync(sl.h_sums, sl.d_sums, take.size() * 8, hipMemcpyDeviceToHost, sl.st));
     }
-    for (size_t i = 0; i < take.size(); i++) {
+    for (size_t i = 0; i < take.size(); i++) {  // results to staging; to the callers at retirement
       Req* q = take[i];
-      if (!(q->output & RF_OUT_HOST) || !q->rn) continue;
-      for (int e = 0; e < q->r * q->d; e++)
-        HIPCHK(hipMemcpyAsync(q->out_host + (size_t)e * q->out_cap, prods[i].out[e], q->rn * 8,
-                              hipMemcpyDeviceToHost, sl.st));
+      if (q->st_out != NONE)
+        HIPCHK(hipMemcpyAsync(sl.h_stage + q->st_out, prods[i].out[0], q->out_bytes, hipMemcpyDeviceToHost, sl.st));
     }
     HIPCHK(hipEventRecord(sl.ev, sl.st));
     return RF_OK;
@@ -444,6 +566,7 @@ int rf_server_create(const rf_server_config* cfg, rf_server** out) {
       return rc;
     }
   }
+  S->copiers.reset(new CopyPool(3));
   S->worker = std::thread(worker_main, S);
   *out = S;
   return RF_OK;
@@ -452,7 +575,7 @@ int rf_server_create(const rf_server_config* cfg, rf_server** out) {
 int rf_server_submit(rf_server* S, const rf_request* rq, uint64_t* id) {
   if (!S || !rq) return fail(RF_ERR_ARG, "rf_server_submit: null argument");
   const int r = rq->r, d = rq->d;
-  if (r <= 0 || d <= 0 || !rq->A || !rq->B) return fail(RF_ERR_ARG, "bad request r=%d d=%d", r, d);
+  if (r <= 0 || d <= 0 || (!rq->A && !rq->left) || !rq->B) return fail(RF_ERR_ARG, "bad request r=%d d=%d", r, d);
   if (rq->input != RF_IN_HOST && rq->input != RF_IN_SYNTH) return fail(RF_ERR_ARG, "bad input mode %d", rq->input);
   if ((rq->output & RF_OUT_HOST) && !rq->out_host) return fail(RF_ERR_ARG, "RF_OUT_HOST without out_host");
   Req* q = new Req();
@@ -466,9 +589,12 @@ int rf_server_submit(rf_server* S, const rf_request* rq, uint64_t* id) {
   q->out_cap = rq->out_cap;
   q->P.r = r;
   q->P.d = d;
+  // A resident left operand replaces A: its entries stand as zeros here (no
+  // limbs, nothing generated or copied) and its spectra are read in place.
+  q->P.left = rq->left;
   uint64_t in_limbs = 0;
   for (int e = 0; e < r * d + d * d; e++) {
-    rf_serve_entry x = e < r * d ? rq->A[e] : rq->B[e - r * d];
+    rf_serve_entry x = e < r * d ? (rq->left ? rf_serve_entry{nullptr, 0, 0, 0} : rq->A[e]) : rq->B[e - r * d];
     if (x.bits == 0) x.sign = 0;
     x.sign = x.sign > 0 ? 1 : (x.sign < 0 ? -1 : 0);
     if (q->input == RF_IN_HOST && x.sign && !x.limbs) {

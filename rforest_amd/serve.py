@@ -26,7 +26,8 @@ class ServeEntry(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("r", ctypes.c_int32), ("d", ctypes.c_int32), ("input", ctypes.c_int32), ("output", ctypes.c_int32),
                 ("A", ctypes.POINTER(ServeEntry)), ("B", ctypes.POINTER(ServeEntry)), ("seed", ctypes.c_uint64),
-                ("out_host", ctypes.c_void_p), ("out_cap", ctypes.c_uint64), ("tag", ctypes.c_uint64)]
+                ("out_host", ctypes.c_void_p), ("out_cap", ctypes.c_uint64), ("tag", ctypes.c_uint64),
+                ("left", ctypes.c_void_p)]
 
 
 class Completion(ctypes.Structure):
@@ -151,46 +152,73 @@ class Server:
         self.h = h
         self._keep = {}
 
-    def _submit(self, r, d, ents, input_mode, seed, output, tag, out=None, out_cap=0, keep=None):
+    def _submit(self, r, d, ents, input_mode, seed, output, tag, out=None, out_cap=0, keep=None, left=None):
+        """ents: A's r*d entries then B's d*d as (limb pointer, bits, sign);
+        with a resident left operand, B's d*d only."""
         L = _bind()
         A = (ServeEntry * (r * d))()
         B = (ServeEntry * (d * d))()
+        na = 0 if left is not None else r * d
         for e, (ptr, bits, sign) in enumerate(ents):
-            tgt = A[e] if e < r * d else B[e - r * d]
+            tgt = A[e] if e < na else B[e - na]
             tgt.limbs = ptr
             tgt.bits = bits
             tgt.sign = sign
         req = Request(r, d, input_mode, output, ctypes.cast(A, ctypes.POINTER(ServeEntry)),
                       ctypes.cast(B, ctypes.POINTER(ServeEntry)), seed & _M64,
-                      out.ctypes.data if out is not None else None, out_cap, tag)
+                      out.ctypes.data if out is not None else None, out_cap, tag,
+                      left.handle.value if left is not None else None)
         rid = ctypes.c_uint64()
         check(L.rf_server_submit(self.h, ctypes.byref(req), ctypes.byref(rid)))
         if keep is not None or out is not None:
             self._keep[rid.value] = (keep, out)
         return rid.value
 
-    def submit_synth(self, r, d, bits, signs, seed, tag=0, output=RF_OUT_CHECKSUM):
+    def submit_synth(self, r, d, bits, signs, seed, tag=0, output=RF_OUT_CHECKSUM, left=None):
         """Entries generated in device memory at admission: entry e has
-        bits[e] bits and sign signs[e] (A's r*d entries, then B's d*d)."""
+        bits[e] bits and sign signs[e] (A's r*d entries, then B's d*d; B's
+        only when `left`, a ResidentOperand, stands for A)."""
         ents = [(None, int(b), int(s)) for b, s in zip(bits, signs)]
-        return self._submit(r, d, ents, RF_IN_SYNTH, seed, output, tag)
+        return self._submit(r, d, ents, RF_IN_SYNTH, seed, output, tag, left=left)
 
-    def submit_host(self, r, d, A, B, tag=0, output=RF_OUT_HOST | RF_OUT_CHECKSUM):
-        """Entries from Python ints, copied at admission.  With RF_OUT_HOST the
-        result is copied back at completion (results(id) decodes it)."""
-        bufs, ents = [], []
-        for x in list(A) + list(B):
+    def submit_host(self, r, d, A, B, tag=0, output=RF_OUT_HOST | RF_OUT_CHECKSUM, left=None):
+        """Entries from Python ints, copied at admission (A is ignored when
+        `left`, a ResidentOperand, stands for it).  With RF_OUT_HOST the
+        result is copied back at completion (host_result(id) decodes it)."""
+        limbs = []
+        for x in ([] if left is not None else list(A)) + list(B):
             x = int(x)
             m = abs(x)
             n = (m.bit_length() + 63) // 64
-            a = np.frombuffer(m.to_bytes(8 * max(n, 1), "little"), dtype=np.uint64).copy()
-            bufs.append(a)
-            ents.append((a.ctypes.data, m.bit_length(), (x > 0) - (x < 0)))
-        la = max([e[1] for e in ents[:r * d]] or [0])
-        lb = max([e[1] for e in ents[r * d:]] or [0])
+            limbs.append(((x > 0) - (x < 0), np.frombuffer(m.to_bytes(8 * max(n, 1), "little"), dtype=np.uint64).copy(),
+                          m.bit_length()))
+        return self.submit_host_limbs(r, d, limbs, tag, output, left)
+
+    def out_limbs_for(self, r, d, ents, left=None):
+        """Limbs per C entry that a host result of these entries needs."""
+        na = 0 if left is not None else r * d
+        la = left.la_limbs * 64 if left is not None else max([b if sg else 0 for (sg, _, b) in ents[:na]] or [0])
+        lb = max([b if sg else 0 for (sg, _, b) in ents[na:]] or [0])
+        return max(int(lib().rf_out_limbs((la + 63) // 64, (lb + 63) // 64, d)), 1)
+
+    def submit_host_limbs(self, r, d, ents, tag=0, output=RF_OUT_HOST | RF_OUT_CHECKSUM, left=None, out=None):
+        """Entries as (sign, uint64 limb array, bits) triples, copied at
+        admission; A's r*d then B's d*d (B's only with `left`).  `out`: a
+        caller's uint64 array of r*d*out_limbs_for(...) limbs for the result
+        (a new one when None)."""
+        bufs = [a for (_, a, _) in ents]
+        cents = [(a.ctypes.data, int(bits) if sg else 0, int(sg) if bits else 0) for (sg, a, bits) in ents]
+        na = 0 if left is not None else r * d
+        la = left.la_limbs * 64 if left is not None else max([e[1] for e in cents[:na]] or [0])
+        lb = max([e[1] for e in cents[na:]] or [0])
         cap = max(int(lib().rf_out_limbs((la + 63) // 64, (lb + 63) // 64, d)), 1)
-        out = np.zeros(r * d * cap, dtype=np.uint64) if output & RF_OUT_HOST else None
-        rid = self._submit(r, d, ents, RF_IN_HOST, 0, output, tag, out, cap if out is not None else 0, bufs)
+        if not output & RF_OUT_HOST:
+            out = None
+        elif out is None:
+            out = np.zeros(r * d * cap, dtype=np.uint64)
+        else:
+            assert out.dtype == np.uint64 and out.size >= r * d * cap
+        rid = self._submit(r, d, cents, RF_IN_HOST, 0, output, tag, out, cap if out is not None else 0, bufs, left)
         if out is not None:
             self._keep[rid] = (bufs, out, r, d, cap)
         return rid

@@ -95,22 +95,24 @@ def test_server_mix_equals_alone_and_gmp():
     mix = {c["tag"]: c for c in srv.wait_all(len(ids))}
     st = srv.stats()
     assert st["batches"] < len(ids)  # requests shared batches
+    full = [srv.read(ids[i], r, r, mix[i]["out_limbs"]) for i, (r, _, _, _) in enumerate(stream)]
     # a sample against GMP schoolbook on the host-generated inputs
     for i in range(0, len(stream), 15):
         r, bits, signs, seed = stream[i]
-        C = srv.read(ids[i], r, r, mix[i]["out_limbs"])
         A, B = _values(r, bits, signs, seed)
-        assert C == naive(A, r, B, r)
+        assert full[i] == naive(A, r, B, r)
     for rid in ids:
         srv.release(rid)
     srv.close()
-    # each request alone in a fresh server
+    # each request alone in a fresh server: the same full value (and checksum)
     srv = serve.Server(8 << 30, streams=1, max_batch=1)
     for i, (r, bits, signs, seed) in enumerate(stream):
-        srv.submit_synth(r, r, bits, signs, seed, tag=i)
+        rid = srv.submit_synth(r, r, bits, signs, seed, tag=i, output=serve.RF_OUT_CHECKSUM | serve.RF_OUT_KEEP)
         (c,) = srv.wait_all(1)
         assert c["status"] == rf.RF_OK
         assert c["checksum"] == mix[i]["checksum"] and c["out_limbs"] == mix[i]["out_limbs"], i
+        assert srv.read(rid, r, r, c["out_limbs"]) == full[i], i
+        srv.release(rid)
     srv.close()
 
 
