@@ -30,7 +30,9 @@ extern int hw_disable_fft;
 extern int mpzfft_threads;
 
 /* src/include/hwmpz.h:74 (src/lib/hwmpz.c:24-40, :51): one-time setup of the
-   prime tables and GPU state; idempotent. */
+   prime tables and GPU state (38 MB of twiddle tables for transforms up to
+   2^16 points; longer transforms add 1.2 GB, then 9.7 GB, on the first plan
+   that needs them); idempotent. */
 void hw_mpz_setup(void);
 /* src/include/hwmpz.h:75 (src/lib/hwmpz.c:42-52): releases GPU state. */
 void hw_mpz_clear(void);
@@ -131,9 +133,9 @@ int rf_batch_mult_device(const rf_dev_product* prods, int nprod, void* stream, i
    codes) in a fixed device array of the library (64 MiB per GPU, held from
    module load until the process ends) instead of the engine's arena: the
    call makes no device or pinned allocation of any kind.  Products whose
-   scratch exceeds the array, or whose plans would need the twiddle tables
-   of transforms above 2^21 points made first, are refused with
-   RF_ERR_CAPACITY.  The array keeps the last call's contents and every call
+   scratch exceeds the array, or whose plans would need twiddle tables not
+   made yet (those of transforms above 2^16 points are made by the first
+   allocating call that plans one), are refused with RF_ERR_CAPACITY.  The array keeps the last call's contents and every call
    in the process on that GPU shares it; calls are serialised by the
    library's lock and each returns after its work completes on `stream`
    (NULL: the library's stream), so they never overlap -- two calls running

@@ -81,15 +81,17 @@ uint64_t g_gen = 0;
 std::shared_mutex g_tables_rw;
 thread_local std::string g_err;
 
-// Twiddle tables of order 2^RF_LGN_MAX for transforms above 2^RF_LGN_TW
-// points (RF_TW_PRIMES x (2^23 + 2^24) pairs, 9.7 GB): made on the first
-// plan that needs them and kept until hw_mpz_clear.  Server workers launch
-// without the engine mutex, so the pointers are atomics and g_tw_mu alone
-// serialises their creation.  Their entries agree with the resident tables'
-// on every index both hold (both come from powers of the same root of order
-// 2^RF_S), so a transform may mix row passes over one with column passes
-// over the other.
+// Twiddle tables beyond the set-up ones (2^RF_LGN_TW points, 38 MB): order
+// 2^RF_LGN_MID for transforms up to that length (RF_TW_PRIMES x (2^20 +
+// 2^21) pairs, 1.2 GB) and order 2^RF_LGN_MAX above it (x (2^23 + 2^24), 9.7
+// GB), each made on the first plan that needs it and kept until
+// hw_mpz_clear.  Server workers launch without the engine mutex, so the
+// pointers are atomics and g_tw_mu alone serialises their creation.  All
+// tables are powers of one root of order 2^RF_S per prime, so they agree on
+// every index they share and a transform may mix row passes over one with
+// column passes over another.
 std::mutex g_tw_mu;
+std::atomic<double*> g_twf_mid{nullptr}, g_twi_mid{nullptr};
 std::atomic<double*> g_twf_long{nullptr}, g_twi_long{nullptr};
 
 int fail(int code, const char* fmt, ...) {
@@ -148,8 +150,9 @@ int setup_locked() {
   HIPCHK(hipMalloc((void**)&g.d_consts, sizeof(RfConsts)));
   HIPCHK(hipMemcpy(g.d_consts, hc, sizeof(RfConsts), hipMemcpyHostToDevice));
   free(hc);
-  // Twiddle tables (RF_TW_PRIMES x (2^20 + 2^21) pairs, 1.1 GiB) are
-  // generated on the device from each prime's root of order 2^RF_LGN_TW.
+  // Twiddle tables for transforms up to 2^RF_LGN_TW points (RF_TW_PRIMES x
+  // (2^15 + 2^16) pairs, 38 MB) are generated on the device from each
+  // prime's root of order 2^RF_LGN_TW; longer ones are made on first use.
   const size_t twf_n = (size_t)RF_TW_PRIMES * RF_FWD_TW * 2, twi_n = (size_t)RF_TW_PRIMES * RF_INV_TW * 2;
   HIPCHK(hipMalloc((void**)&g.d_twf, twf_n * sizeof(double)));
   HIPCHK(hipMalloc((void**)&g.d_twi, twi_n * sizeof(double)));
@@ -189,6 +192,8 @@ void clear_locked() {
   hipFree(g.d_twi);
   {
     std::lock_guard<std::mutex> lk(g_tw_mu);
+    hipFree(g_twf_mid.exchange(nullptr));
+    hipFree(g_twi_mid.exchange(nullptr));
     hipFree(g_twf_long.exchange(nullptr));
     hipFree(g_twi_long.exchange(nullptr));
   }
@@ -199,23 +204,37 @@ void clear_locked() {
   g = Ctx();
 }
 
-// The tables a launch passes to its kernels (long-table pointers null until
-// made).
+// The tables a launch passes to its kernels (middle and long pointers null
+// until made).
 RfDevTables tables() {
   RfDevTables t = g.tables;
+  t.tw_fwd_mid = g_twf_mid.load(std::memory_order_acquire);
+  t.tw_inv_mid = g_twi_mid.load(std::memory_order_acquire);
   t.tw_fwd_long = g_twf_long.load(std::memory_order_acquire);
   t.tw_inv_long = g_twi_long.load(std::memory_order_acquire);
   return t;
 }
 
+// Whether the tables a transform of 2^lgN points reads exist.
+bool tables_exist(int lgN) {
+  if (lgN > RF_LGN_MID) return g_twi_long.load(std::memory_order_acquire) != nullptr;
+  if (lgN > RF_LGN_TW) return g_twi_mid.load(std::memory_order_acquire) != nullptr;
+  return true;
+}
+
 int ensure_long_tables(int lgN) {
-  if (lgN <= RF_LGN_TW || g_twi_long.load(std::memory_order_acquire)) return RF_OK;
+  if (tables_exist(lgN)) return RF_OK;
   std::lock_guard<std::mutex> lk(g_tw_mu);
-  if (g_twi_long.load(std::memory_order_acquire)) return RF_OK;
+  if (tables_exist(lgN)) return RF_OK;
   if (!g.ready) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  const bool mid = lgN <= RF_LGN_MID;
+  const int lgmax = mid ? RF_LGN_MID : RF_LGN_MAX;
+  std::atomic<double*>& pf = mid ? g_twf_mid : g_twf_long;
+  std::atomic<double*>& pi = mid ? g_twi_mid : g_twi_long;
   double roots[2 * RF_TW_PRIMES] = {0};
-  rf_twiddle_roots(&g.primes, RF_LGN_MAX, roots, roots + RF_TW_PRIMES);
-  const size_t twf_n = (size_t)RF_TW_PRIMES * RF_FWD_TW_LONG * 2, twi_n = (size_t)RF_TW_PRIMES * RF_INV_TW_LONG * 2;
+  rf_twiddle_roots(&g.primes, lgmax, roots, roots + RF_TW_PRIMES);
+  const size_t twf_n = (size_t)RF_TW_PRIMES * ((size_t)1 << (lgmax - 1)) * 2,
+               twi_n = (size_t)RF_TW_PRIMES * ((size_t)1 << lgmax) * 2;
   double *f = nullptr, *i = nullptr, *d_roots = nullptr;
   hipStream_t st = nullptr;
   hipError_t e = hipMalloc((void**)&f, twf_n * sizeof(double));
@@ -223,7 +242,7 @@ int ensure_long_tables(int lgN) {
   if (e == hipSuccess) e = hipMalloc((void**)&d_roots, sizeof roots);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMemcpyAsync(d_roots, roots, sizeof roots, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = rf_launch_twiddles(g.d_consts, d_roots, d_roots + RF_TW_PRIMES, RF_TW_PRIMES, RF_LGN_MAX, f, i, st);
+  if (e == hipSuccess) e = rf_launch_twiddles(g.d_consts, d_roots, d_roots + RF_TW_PRIMES, RF_TW_PRIMES, lgmax, f, i, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (st) hipStreamDestroy(st);
   hipFree(d_roots);
@@ -233,8 +252,8 @@ int ensure_long_tables(int lgN) {
     return fail(e == hipErrorOutOfMemory ? RF_ERR_NOMEM : RF_ERR_HIP,
                 "twiddle tables for transforms of 2^%d points: %s", lgN, hipGetErrorString(e));
   }
-  g_twf_long.store(f, std::memory_order_release);
-  g_twi_long.store(i, std::memory_order_release);
+  pf.store(f, std::memory_order_release);
+  pi.store(i, std::memory_order_release);
   return RF_OK;
 }
 
@@ -338,7 +357,7 @@ bool tuned_plan(uint64_t LA, uint64_t LB, int r, int d, RfPlan* out) {
 
 int tables_for(int lgN, bool make) {
   if (make) return ensure_long_tables(lgN);
-  if (lgN > RF_LGN_TW && !g_twi_long.load(std::memory_order_acquire))
+  if (!tables_exist(lgN))
     return fail(RF_ERR_CAPACITY, "a plan of 2^%d points needs twiddle tables that are not made (no allocation allowed)",
                 lgN);
   return RF_OK;

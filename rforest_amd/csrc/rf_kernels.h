@@ -102,7 +102,9 @@ struct RfDevTables {
   const RfConsts* consts;
   const double* tw_fwd;  // RF_TW_PRIMES * RF_FWD_TW * 2
   const double* tw_inv;  // RF_TW_PRIMES * RF_INV_TW * 2
-  const double* tw_fwd_long;  // RF_TW_PRIMES * RF_FWD_TW_LONG * 2 (null until a transform above 2^RF_LGN_TW)
+  const double* tw_fwd_mid;   // RF_TW_PRIMES * RF_FWD_TW_MID * 2 (null until a transform above 2^RF_LGN_TW)
+  const double* tw_inv_mid;   // RF_TW_PRIMES * RF_INV_TW_MID * 2
+  const double* tw_fwd_long;  // RF_TW_PRIMES * RF_FWD_TW_LONG * 2 (null until a transform above 2^RF_LGN_MID)
   const double* tw_inv_long;  // RF_TW_PRIMES * RF_INV_TW_LONG * 2
 };
 

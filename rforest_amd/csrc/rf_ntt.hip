@@ -810,16 +810,24 @@ __global__ __launch_bounds__(256) void k_twiddles(const RfConsts* __restrict__ C
   }
 }
 
-// The twiddle table of prime q for a transform of 2^lgN points: the resident
-// table up to 2^RF_LGN_TW, the long one above (they agree on every index
-// both hold).
+// The twiddle table of prime q for a transform of 2^lgN points: the set-up
+// table up to 2^RF_LGN_TW, the middle one up to 2^RF_LGN_MID, the long one
+// above (they agree on every index they share).
 __device__ __forceinline__ const double2* fwd_table(const RfDevTables& tb, int q, int lgN) {
-  return lgN > RF_LGN_TW ? (const double2*)(tb.tw_fwd_long + (size_t)q * RF_FWD_TW_LONG * 2)
-                         : (const double2*)(tb.tw_fwd + (size_t)q * RF_FWD_TW * 2);
+  return lgN > RF_LGN_MID  ? (const double2*)(tb.tw_fwd_long + (size_t)q * RF_FWD_TW_LONG * 2)
+         : lgN > RF_LGN_TW ? (const double2*)(tb.tw_fwd_mid + (size_t)q * RF_FWD_TW_MID * 2)
+                           : (const double2*)(tb.tw_fwd + (size_t)q * RF_FWD_TW * 2);
 }
 __device__ __forceinline__ const double2* inv_table(const RfDevTables& tb, int q, int lgN) {
-  return lgN > RF_LGN_TW ? (const double2*)(tb.tw_inv_long + (size_t)q * RF_INV_TW_LONG * 2)
-                         : (const double2*)(tb.tw_inv + (size_t)q * RF_INV_TW * 2);
+  return lgN > RF_LGN_MID  ? (const double2*)(tb.tw_inv_long + (size_t)q * RF_INV_TW_LONG * 2)
+         : lgN > RF_LGN_TW ? (const double2*)(tb.tw_inv_mid + (size_t)q * RF_INV_TW_MID * 2)
+                           : (const double2*)(tb.tw_inv + (size_t)q * RF_INV_TW * 2);
+}
+// Whether the tables a transform of 2^lgN points reads have been made.
+static bool tables_made(const RfDevTables& tb, int lgN) {
+  if (lgN > RF_LGN_MID) return tb.tw_fwd_long && tb.tw_inv_long;
+  if (lgN > RF_LGN_TW) return tb.tw_fwd_mid && tb.tw_inv_mid;
+  return tb.tw_fwd && tb.tw_inv;
 }
 
 // ---- transforms longer than one workgroup: N = 2^(RF_LGN_ROW + N1) ----
@@ -1131,7 +1139,7 @@ hipError_t rf_launch_fwd_col(const RfDevTables& tb, int lgN, int t, const RfFwdT
   if (ntasks <= 0) return hipSuccess;
   const int n1 = lgN - RF_LGN_ROW;
   if (n1 < 1 || lgN > RF_LGN_MAX) return hipErrorInvalidValue;
-  if (lgN > RF_LGN_TW && (!tb.tw_fwd_long || !tb.tw_inv_long)) return hipErrorInvalidValue;  // tables not made
+  if (!tables_made(tb, lgN)) return hipErrorInvalidValue;
   const int na = n1 - 4 * ((n1 - 1) / 4);  // bits of the pack-fused column pass
   const int lgc = lgN - na;
   const dim3 grid(ntasks * ((1 << lgc) / RF_COL_THREADS)), blk(RF_COL_THREADS);
@@ -1182,7 +1190,7 @@ hipError_t rf_launch_inv_col(const RfDevTables& tb, int lgN, const RfRowTask* ta
   if (ntasks <= 0) return hipSuccess;
   const int n1 = lgN - RF_LGN_ROW;
   if (n1 < 1 || lgN > RF_LGN_MAX) return hipErrorInvalidValue;
-  if (lgN > RF_LGN_TW && !tb.tw_inv_long) return hipErrorInvalidValue;  // tables not made
+  if (!tables_made(tb, lgN)) return hipErrorInvalidValue;
   hipError_t e = hipSuccess;
   for (int b0 = 0; e == hipSuccess && b0 < n1; b0 += 4)
     e = launch_inv_col_pass(tb, lgN, b0, n1 - b0 < 4 ? n1 - b0 : 4, tasks, ntasks, coef, st);

@@ -25,8 +25,8 @@
 #define RF_LGN_LDS 14     // largest one-workgroup transform (2^14 doubles = 128 KiB of LDS)
 #define RF_LGN_ROW 13     // longer transforms: columns of 2^(lgN-13) in registers, rows of 2^13 in LDS
 #define RF_LGN_MAX 24     // largest transform: 2^11 rows of 2^13 (column passes of <= 16 residues)
-#define RF_LGN_TW 21      // twiddle tables kept from set-up on; longer transforms use tables of 2^RF_LGN_MAX
-                          // generated on first use
+#define RF_LGN_TW 16      // twiddle tables made at set-up; longer transforms use tables of 2^RF_LGN_MID
+#define RF_LGN_MID 21     // or 2^RF_LGN_MAX, each generated on the first plan that needs it
 #define RF_KMAX 24        // most primes a plan may use (the CRT kernel's instantiations)
 #define RF_TW_PRIMES RF_KMAX  // primes with twiddle tables
 #define RF_TMAX 8         // chunk width up to 8 limbs = 512 bits
@@ -116,13 +116,16 @@ struct RfConsts {
 };
 
 // Twiddle tables, per prime, as (w, fl(w/p)) pairs, for a maximum length
-// 2^L (L = RF_LGN_TW for the resident tables, RF_LGN_MAX for the long ones):
+// 2^L (L = RF_LGN_TW for the tables made at set-up, RF_LGN_MID and
+// RF_LGN_MAX for the ones made when a plan first needs them):
 //   fwd[q][m] = omega^(rev_{L-1}(m)), m < 2^(L-1), omega of order 2^L; the
 //     prefix of length N/2 serves length N, so both tables agree where both
 //     exist.
 //   inv[q][len + j] = omega_{2 len}^(-j), len = 2^e < 2^L, j < len.
 #define RF_FWD_TW (1 << (RF_LGN_TW - 1))
 #define RF_INV_TW (1 << RF_LGN_TW)
+#define RF_FWD_TW_MID (1 << (RF_LGN_MID - 1))
+#define RF_INV_TW_MID (1 << RF_LGN_MID)
 #define RF_FWD_TW_LONG (1 << (RF_LGN_MAX - 1))
 #define RF_INV_TW_LONG (1 << RF_LGN_MAX)
 
