@@ -49,6 +49,15 @@ int rf_stage_crt(const uint64_t* res, uint32_t nout, int k, int t, uint64_t rn, 
    piecewise recombination be checked at small sizes. */
 int rf_stage_dropin_piece_bits(uint64_t bits);
 
+/* The reciprocal a remainder modulus keeps (rf_mod_create): for D (dn
+   limbs, top limb nonzero) and dividends of at most max_bits bits (m =
+   max(ceil(max_bits / 64), dn) limbs), I with
+   floor(2^(64 (m+1)) / D) - 2 <= I <= floor(2^(64 (m+1)) / D), computed by the
+   device's Newton recursion; written to out (cap limbs), its length
+   (m - dn + 2 limbs, high zeros included) to *nlimbs. */
+int rf_stage_mod_reciprocal(const uint64_t* d, uint64_t dn, uint64_t max_bits, uint64_t* out, uint64_t cap,
+                            uint64_t* nlimbs);
+
 /* Root of order 2^lgN used for prime index q. */
 uint64_t rf_root(int q, int lgN);
 

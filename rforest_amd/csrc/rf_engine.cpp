@@ -1461,7 +1461,10 @@ void rf_operand_destroy(rf_operand* op) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!op) return;
   if (g.ready) (void)hipDeviceSynchronize();
-  if (op->spec) (void)hipFree(op->spec);
+  if (op->slab)
+    op->slab.reset();
+  else if (op->spec)
+    (void)hipFree(op->spec);
   delete op;
 }
 
@@ -2010,7 +2013,139 @@ int pipeline_run(std::vector<Product>& prods, const Layout& L, size_t base) {
 void free_operand(rf_operand* op) {
   if (!op) return;
   if (g.ready) (void)hipStreamSynchronize(g.stream);
-  if (op->spec) (void)hipFree(op->spec);
+  if (op->slab)
+    op->slab.reset();
+  else if (op->spec)
+    (void)hipFree(op->spec);
   delete op;
+}
+
+std::shared_ptr<void> device_slab(size_t bytes, void** ptr) {
+  *ptr = nullptr;
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
+  *ptr = p;
+  return std::shared_ptr<void>(p, [](void* q) { (void)hipFree(q); });
+}
+
+size_t operands_batch_task_bound(size_t nent) {
+  const size_t rows = (size_t)1 << (RF_LGN_MAX - RF_LGN_ROW);
+  // per fwd_key group alignment: at most 2 * RF_TMAX * (RF_LGN_MAX + 1) groups
+  return align256(nent * RF_KMAX * sizeof(RfFwdTask)) + align256(nent * RF_KMAX * rows * sizeof(RfRowTask)) +
+         (size_t)512 * RF_TMAX * (RF_LGN_MAX + 2);
+}
+
+int make_operands_batch(const std::vector<Entry>& ents, const std::vector<uint64_t>& b_max_bits,
+                        const std::vector<const RfPlan*>& fixed, size_t base, std::vector<rf_operand*>& out) {
+  out.clear();
+  const size_t cnt = ents.size();
+  std::vector<rf_operand*> ops(cnt, nullptr);
+  std::vector<size_t> off(cnt, 0);
+  std::map<std::pair<uint64_t, uint64_t>, RfPlan> planned;
+  size_t total = 0;
+  int rc = RF_OK;
+  for (size_t i = 0; i < cnt && !rc; i++) {
+    rf_operand* op = ops[i] = new rf_operand();
+    op->r = op->d = 1;
+    op->la_bits = ents[i].sign ? ents[i].bits : 0;
+    op->signs.push_back(ents[i].sign);
+    if (!op->la_bits || !b_max_bits[i]) {
+      op->la_bits = 0;  // a zero operand
+      continue;
+    }
+    if (fixed[i]) {
+      op->plan = *fixed[i];
+    } else {
+      // operands made together mostly share their sizes: plan each size once
+      const auto key = std::make_pair(op->la_bits, b_max_bits[i]);
+      auto it = planned.find(key);
+      if (it != planned.end()) {
+        op->plan = it->second;
+      } else if (!tuned_plan(op->la_bits, b_max_bits[i], 1, 1, &op->plan) &&
+                 rf_plan_choose(&g.primes, op->la_bits, b_max_bits[i], 1, 1, &op->plan) != 0) {
+        rc = fail(RF_ERR_UNSUPPORTED, "no transform plan for an operand of %llu bits and right entries of %llu bits",
+                  (unsigned long long)op->la_bits, (unsigned long long)b_max_bits[i]);
+      } else {
+        planned[key] = op->plan;
+      }
+    }
+    if (!rc) rc = ensure_long_tables(op->plan.lgN);
+    off[i] = total;
+    total += align256((size_t)op->plan.k * ((size_t)8 << op->plan.lgN));
+  }
+  std::shared_ptr<void> slab;
+  char* sp = nullptr;
+  if (!rc && total) {
+    slab = device_slab(total, (void**)&sp);
+    if (!slab) rc = fail(RF_ERR_NOMEM, "operand spectra of %zu bytes", total);
+  }
+  // forward tasks grouped like the product pipeline's (one launch per group)
+  std::map<int, std::vector<RfFwdTask>> fwd;
+  std::vector<RfRowTask> rows;
+  for (size_t i = 0; i < cnt && !rc; i++) {
+    rf_operand* op = ops[i];
+    if (!op->la_bits) continue;
+    op->slab = slab;
+    op->spec = (double*)(sp + off[i]);
+    const RfPlan& pl = op->plan;
+    const uint64_t N = (uint64_t)1 << pl.lgN;
+    const uint64_t spec0 = (uint64_t)(off[i] / 8);  // doubles from the slab's start
+    for (int q = 0; q < pl.k; q++) {
+      RfFwdTask t{};
+      t.limbs = ents[i].dptr;
+      t.spec_off = spec0 + (uint64_t)q * N;
+      t.nlimbs = (uint32_t)ents[i].nlimbs;
+      t.nchunk = (uint32_t)std::min<uint64_t>((ents[i].nlimbs + pl.t - 1) / pl.t, N);
+      t.sign = ents[i].sign;
+      t.q = (uint16_t)q;
+      t.t = (uint8_t)pl.t;
+      fwd[fwd_key(pl)].push_back(t);
+      if (plan_split(pl))
+        for (uint32_t h = 0; h < (1u << (pl.lgN - RF_LGN_ROW)); h++)
+          rows.push_back(RfRowTask{t.spec_off + ((uint64_t)h << RF_LGN_ROW), h, (uint16_t)q, (uint16_t)pl.lgN});
+    }
+  }
+  if (!rc && !fwd.empty()) {
+    std::vector<char> img;
+    std::vector<std::pair<int, size_t>> groups;  // (fwd_key, image offset)
+    for (auto& kv : fwd) groups.push_back({kv.first, append_tasks(img, kv.second)});
+    const size_t roff = append_tasks(img, rows);
+    if (base + img.size() > g.arena_cap || img.size() > operands_batch_task_bound(cnt)) {
+      rc = fail(RF_ERR_ARG, "internal: operand task tables exceed their room");
+    } else {
+      rc = ensure_stage(img.size());
+      if (!rc) {
+        hipError_t e = hipStreamSynchronize(g.stream);
+        if (e == hipSuccess) {
+          memcpy(g.h_stage, img.data(), img.size());
+          e = hipMemcpyAsync(g.d_arena + base, g.h_stage, img.size(), hipMemcpyHostToDevice, g.stream);
+        }
+        const RfDevTables tb = tables();
+        for (size_t gi = 0; gi < groups.size() && e == hipSuccess; gi++) {
+          const int key = groups[gi].first;
+          const RfFwdTask* tt = (const RfFwdTask*)(g.d_arena + base + groups[gi].second);
+          const int nt = (int)fwd[key].size();
+          const int lgN = (key >> 4) & 255, t = key & 15;
+          e = (key >> 12) ? rf_launch_fwd_col(tb, lgN, t, tt, nt, (double*)sp, g.stream)
+                          : rf_launch_fwd(tb, lgN, t, tt, nt, (double*)sp, g.stream);
+        }
+        if (e == hipSuccess && !rows.empty())
+          e = rf_launch_fwd_row(tables(), (const RfRowTask*)(g.d_arena + base + roff), (int)rows.size(), (double*)sp,
+                                g.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+        if (e != hipSuccess) rc = fail(RF_ERR_HIP, "operand transforms: %s", hipGetErrorString(e));
+      }
+    }
+  }
+  if (rc) {
+    for (rf_operand* op : ops) {
+      if (!op) continue;
+      op->slab.reset();
+      delete op;
+    }
+    return rc;
+  }
+  out = ops;
+  return RF_OK;
 }
 }  // namespace rfi

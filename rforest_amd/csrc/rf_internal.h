@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <mutex>
 #include <shared_mutex>
 #include <vector>
@@ -21,6 +22,9 @@ struct rf_operand {
   uint64_t la_bits = 0;     // largest entry, bits (0: the operand is zero)
   std::vector<int> signs;   // per entry
   double* spec = nullptr;   // device, r*d*k*N doubles
+  // set when spec lies in an allocation shared by the operands made together
+  // (freed with the last of them); otherwise spec is the operand's own
+  std::shared_ptr<void> slab;
 };
 
 namespace rfi {
@@ -110,6 +114,18 @@ int make_operand(std::vector<Entry>& A, int r, int d, uint64_t b_max_bits, size_
 size_t operand_task_bound(int nent);
 int pipeline_run(std::vector<Product>& prods, const Layout& L, size_t base);
 void free_operand(rf_operand* op);
+// Resident 1x1 operands for many device entries at once: ents[i] under the
+// plan fixed[i] when given, otherwise planned for right operands of
+// b_max_bits[i] bits; the spectra share one device allocation and are
+// transformed by one set of launches (task tables at arena byte `base`).
+// On failure nothing is left allocated and out is empty.
+int make_operands_batch(const std::vector<Entry>& ents, const std::vector<uint64_t>& b_max_bits,
+                        const std::vector<const RfPlan*>& fixed, size_t base, std::vector<rf_operand*>& out);
+// Arena bytes make_operands_batch needs for the task tables of nent entries
+// of at most k primes (an upper bound).
+size_t operands_batch_task_bound(size_t nent);
+// A device allocation shared by several owners (hipFree with the last).
+std::shared_ptr<void> device_slab(size_t bytes, void** ptr);
 // Host-side set-up only (primes; no device): lets planners run without a GPU.
 int engine_host_init();
 // The engine's stream, and its device arena and pinned staging buffer grown

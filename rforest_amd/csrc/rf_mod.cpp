@@ -18,6 +18,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <map>
+#include <memory>
+#include <tuple>
 #include <system_error>
 #include <thread>
 #include <vector>
@@ -31,6 +34,7 @@ struct rf_mod {
   rf_operand* inv = nullptr;
   rf_operand* div = nullptr;
   uint64_t* d_dev = nullptr;  // D, n limbs
+  std::shared_ptr<void> d_slab;  // the allocation d_dev lies in (shared by a batch's moduli)
 };
 
 namespace {
@@ -174,82 +178,301 @@ int mod_batch_exec(ModBatch& mb, const rf_mod* const* mods, const rf_dev_entry* 
 namespace {
 using namespace rfi;
 
-// The host part of preparing a modulus: n, m and I = floor(B^(m+1) / D)
-// (GMP division; no engine state, so callers may run it in parallel).
-void mod_reciprocal(uint64_t max_bits, const mpz_t d, uint64_t* n, uint64_t* m, mpz_t I) {
-  *n = mpz_size(d);
-  *m = std::max<uint64_t>((max_bits + 63) / 64, *n);
-  mpz_t Bm;
-  mpz_init(Bm);
-  mpz_setbit(Bm, 64 * (*m + 1));
-  mpz_tdiv_q(I, Bm, d);  // at most m - n + 2 limbs
-  mpz_clear(Bm);
+// ---- preparing moduli: the reciprocal I on the device ----
+//
+// For D of n limbs and dividends of at most m limbs, I approximates
+// B^(m+1) / D from below: with D' = D 2^s (top bit set) padded with zero
+// limbs to n' = max(n, m + 2 - n) limbs, D'' = D' B^(n'-n), the
+// approximate-reciprocal recursion (Brent & Zimmermann, Modern Computer
+// Arithmetic, Algorithm 3.5; Lemma 3.6: A X < B^(2n') <= A (X + 2)) gives X
+// on n' + 1 limbs, and I = floor(X 2^s / B^o) with o = n' + n - m - 1 >= 1
+// satisfies floor(B^(m+1)/D) - 2 <= I <= floor(B^(m+1)/D): X 2^s / B^o is
+// within 2 2^s / B^o <= 1 below B^(m+1) / D.  That keeps the quotient
+// estimate's bound X/D - 3 < Q <= X/D (the error term X_hi (e + 2) /
+// B^(m-n+2) stays below 1) -- the reference's remainder step also works
+// from an approximate inverse (zz_invertappr, src/lib/mpnfft_mod.c:13-90).
+// The recursion runs bottom-up, all moduli of a batch level by level: the
+// base (at most kRecipBase limbs) by GMP on the host, every level as two
+// batched device products (A X_h and Tm X_h) and two multi-limb kernels.
+constexpr uint64_t kRecipBase = 64;
+
+struct ModPrep {
+  uint64_t n = 0, m = 0, np = 0, il = 0;
+  uint32_t s = 0, o = 0;
+  std::vector<uint64_t> c;           // size chain: c[0] = n', c[j+1] = c[j] - floor((c[j] - 1) / 2)
+  std::vector<uint64_t> dnorm;       // D'' (n' limbs)
+  std::vector<uint64_t> xbase;       // X of the base level (c.back() + 1 limbs)
+  // arena offsets in limbs
+  uint64_t dn_off = 0, t_off = 0, tm_off = 0, u_off = 0, i_off = 0, tn = 0, un = 0;
+  std::vector<uint64_t> x_off;       // X_j, j = 0 .. c.size() - 1 (c[j] + 1 limbs)
+};
+
+// Host part (no engine state; callers may run it in parallel).
+void mod_prep(uint64_t max_bits, const mpz_t d, ModPrep& P) {
+  P.n = mpz_size(d);
+  P.m = std::max<uint64_t>((max_bits + 63) / 64, P.n);
+  const uint64_t top = d->_mp_d[P.n - 1];
+  P.s = (uint32_t)__builtin_clzll(top);
+  P.np = std::max<uint64_t>(P.n, P.m + 2 - P.n);
+  P.o = (uint32_t)(P.np + P.n - P.m - 1);
+  P.il = P.m - P.n + 2;
+  P.c.assign(1, P.np);
+  while (P.c.back() > kRecipBase) P.c.push_back(P.c.back() - (P.c.back() - 1) / 2);
+  // D'' = (D << s) B^(n'-n)
+  P.dnorm.assign(P.np, 0);
+  const uint64_t* dl = (const uint64_t*)d->_mp_d;
+  for (uint64_t i = 0; i < P.n; i++)
+    P.dnorm[P.np - P.n + i] = P.s ? (dl[i] << P.s) | (i ? dl[i - 1] >> (64 - P.s) : 0) : dl[i];
+  // base: X = floor((B^(2 c) - 1) / A) for the top c limbs A of D''
+  const uint64_t cb = P.c.back();
+  mpz_t A, num, X;
+  mpz_init(A);
+  mpz_init(num);
+  mpz_init(X);
+  mpz_import(A, cb, -1, 8, 0, 0, P.dnorm.data() + (P.np - cb));
+  mpz_setbit(num, 128 * cb);
+  mpz_sub_ui(num, num, 1);
+  mpz_tdiv_q(X, num, A);
+  P.xbase.assign(cb + 1, 0);
+  size_t cnt = 0;
+  mpz_export(P.xbase.data(), &cnt, -1, 8, 0, 0, X);
+  mpz_clear(A);
+  mpz_clear(num);
+  mpz_clear(X);
 }
 
-// The device part (engine mutex held): D and I uploaded and made resident
-// transforms.
-int mod_create_locked(const mpz_t d, uint64_t n, uint64_t m, const mpz_t I, rf_mod** out) {
-  rf_mod* md = new rf_mod();
-  md->n = n;
-  md->m = m;
-  const uint64_t in = mpz_size(I);
-  int rc = RF_OK;
-  const size_t d_b = align256(md->n * 8), i_b = align256(in * 8);
-  char* d_base = nullptr;
-  char* h_base = nullptr;
-  rc = engine_arena(i_b + operand_task_bound(1), &d_base);  // the transforms' tables follow I
-  if (!rc) rc = engine_stage(d_b + i_b, &h_base);
-  if (!rc) {
-    hipError_t e = hipMalloc((void**)&md->d_dev, md->n * 8);
-    if (e != hipSuccess) rc = fail(RF_ERR_NOMEM, "modulus of %llu limbs: %s", (unsigned long long)md->n,
-                                   hipGetErrorString(e));
-  }
-  hipStream_t st = engine_stream();
-  if (!rc) {
-    hipError_t e = hipStreamSynchronize(st);
-    if (e == hipSuccess) {
-      memcpy(h_base, d->_mp_d, md->n * 8);
-      memcpy(h_base + d_b, I->_mp_d, in * 8);
-      e = hipMemcpyAsync(md->d_dev, h_base, md->n * 8, hipMemcpyHostToDevice, st);
+// Device part (engine mutex held): the reciprocals of all moduli, then D and
+// I as resident transforms.
+// recips (tests): the reciprocals I are copied out instead, and no modulus is made.
+int mod_create_batch_locked(const mpz_t* ds, std::vector<ModPrep>& prep, rf_mod** outs,
+                            std::vector<std::vector<uint64_t>>* recips) {
+  const int cnt = (int)prep.size();
+  size_t levels = 0;
+  for (auto& P : prep) levels = std::max(levels, P.c.size() - 1);
+  // plan every level's two products first: their sizes fix the scratch
+  std::vector<std::vector<Product>> p1(levels), p2(levels);
+  std::vector<std::vector<int>> who(levels);  // modulus of each product, per level
+  for (int i = 0; i < cnt; i++) {
+    ModPrep& P = prep[i];
+    for (size_t j = 0; j + 1 < P.c.size(); j++) {
+      const uint64_t h = P.c[j + 1];
+      Product a;
+      a.r = a.d = 1;
+      a.A.push_back(Entry{0, nullptr, P.c[j], 64 * P.c[j], 1});
+      a.B.push_back(Entry{0, nullptr, h + 1, 64 * (h + 1), 1});
+      int rc = plan_product(a);
+      if (rc) return rc;
+      Product b;
+      b.r = b.d = 1;
+      b.A.push_back(Entry{0, nullptr, h + 1, 64 * (h + 1), 1});
+      b.B.push_back(Entry{0, nullptr, h + 1, 64 * (h + 1), 1});
+      rc = plan_product(b);
+      if (rc) return rc;
+      P.tn = std::max<uint64_t>(P.tn, a.plan.rn);
+      P.un = std::max<uint64_t>(P.un, b.plan.rn);
+      p1[j].push_back(std::move(a));
+      p2[j].push_back(std::move(b));
+      who[j].push_back(i);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(d_base, h_base + d_b, in * 8, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) rc = fail(RF_ERR_HIP, "uploading the modulus: %s", hipGetErrorString(e));
   }
-  const uint64_t q_bits = 64 * (md->m - md->n + 1);  // X_hi and Q limbs
-  if (!rc) {
-    std::vector<Entry> E(1);
-    E[0].dptr = (const uint64_t*)d_base;
-    E[0].nlimbs = in;
-    E[0].sign = 1;
-    E[0].bits = 64 * in;
-    rc = make_operand(E, 1, 1, q_bits, i_b, &md->inv);
+  // arena layout (limbs): D'' and the base X's (uploaded), X levels, T, Tm,
+  // U, I per modulus; then task tables; then the pipelines' scratch
+  uint64_t lim = 0;
+  auto take = [&](uint64_t n) {
+    const uint64_t o = lim;
+    lim += (n + 31) & ~(uint64_t)31;  // 256-byte steps
+    return o;
+  };
+  for (auto& P : prep) P.dn_off = take(P.np);
+  for (auto& P : prep) {
+    P.x_off.assign(P.c.size(), 0);
+    P.x_off.back() = take(P.c.back() + 1);
   }
-  if (!rc) {
-    std::vector<Entry> E(1);
-    E[0].dptr = md->d_dev;
-    E[0].nlimbs = md->n;
-    E[0].sign = 1;
-    E[0].bits = mpz_sizeinbase(d, 2);
+  const uint64_t up_limbs = lim;  // uploaded region
+  for (auto& P : prep) {
+    for (size_t j = 0; j + 1 < P.c.size(); j++) P.x_off[j] = take(P.c[j] + 1);
+    P.t_off = take(std::max<uint64_t>(P.tn, 1));
+    P.tm_off = take(P.c.size() > 1 ? P.c[1] + 1 : 1);
+    P.u_off = take(std::max<uint64_t>(P.un, 1));
+    P.i_off = take(P.il);
+  }
+  size_t nrt = 0;
+  for (size_t j = 0; j < levels; j++) nrt += who[j].size();
+  const size_t rt_b = align256(nrt * sizeof(RfRecipTask)) + align256(cnt * sizeof(RfRecipFinal));
+  size_t pipe_b = operands_batch_task_bound(2 * (size_t)cnt);
+  for (size_t j = 0; j < levels; j++) {
+    pipe_b = std::max(pipe_b, layout_of(p1[j]).total());
+    pipe_b = std::max(pipe_b, layout_of(p2[j]).total());
+  }
+  const size_t limbs_b = align256(lim * 8);
+  char* A = nullptr;
+  int rc = engine_arena(limbs_b + rt_b + pipe_b, &A);
+  if (rc) return rc;
+  uint64_t* L = (uint64_t*)A;
+  RfRecipTask* d_rt = (RfRecipTask*)(A + limbs_b);
+  RfRecipFinal* d_fin = (RfRecipFinal*)(A + limbs_b + align256(nrt * sizeof(RfRecipTask)));
+  const size_t pipe = limbs_b + rt_b;
+  // D limbs (kept for the fix-up) in one allocation shared by the moduli
+  uint64_t d_total = 0;
+  std::vector<uint64_t> d_off(cnt);
+  for (int i = 0; i < cnt; i++) d_off[i] = d_total, d_total += (prep[i].n + 31) & ~(uint64_t)31;
+  uint64_t* dslab_p = nullptr;
+  std::shared_ptr<void> dslab = device_slab(d_total * 8, (void**)&dslab_p);
+  if (!dslab) return fail(RF_ERR_NOMEM, "moduli of %llu limbs", (unsigned long long)d_total);
+  // host image: [uploaded limbs | recip tasks | final tasks | D limbs]
+  std::vector<RfRecipTask> rts;
+  std::vector<size_t> rt_first(levels + 1, 0);
+  for (size_t j = levels; j-- > 0;) {  // bottom-up order, the order of the launches
+    rt_first[j] = rts.size();
+    for (int i : who[j]) {
+      const ModPrep& P = prep[i];
+      const uint64_t h = P.c[j + 1];
+      RfRecipTask t{};
+      t.a = L + P.dn_off + (P.np - P.c[j]);
+      t.xh = L + P.x_off[j + 1];
+      t.t = L + P.t_off;
+      t.tm = L + P.tm_off;
+      t.u = L + P.u_off;
+      t.x = L + P.x_off[j];
+      t.n = P.c[j];
+      t.h = h;
+      t.l = P.c[j] - h;
+      rts.push_back(t);
+    }
+  }
+  std::vector<RfRecipFinal> fins(cnt);
+  uint64_t max_il = 0;
+  for (int i = 0; i < cnt; i++) {
+    const ModPrep& P = prep[i];
+    fins[i] = RfRecipFinal{L + P.x_off[0], P.c[0] + 1, L + P.i_off, P.il, P.s, P.o};
+    max_il = std::max(max_il, P.il);
+  }
+  const size_t img_b = limbs_b + rt_b + align256(d_total * 8);
+  char* H = nullptr;
+  rc = engine_stage(img_b, &H);
+  if (rc) return rc;
+  hipStream_t st = engine_stream();
+  HIPCHK(hipStreamSynchronize(st));
+  for (auto& P : prep) {
+    memcpy(H + P.dn_off * 8, P.dnorm.data(), P.np * 8);
+    memcpy(H + P.x_off.back() * 8, P.xbase.data(), P.xbase.size() * 8);
+  }
+  if (!rts.empty()) memcpy(H + limbs_b, rts.data(), rts.size() * sizeof(RfRecipTask));
+  memcpy(H + limbs_b + align256(nrt * sizeof(RfRecipTask)), fins.data(), fins.size() * sizeof(RfRecipFinal));
+  for (int i = 0; i < cnt; i++) memcpy(H + limbs_b + rt_b + d_off[i] * 8, ds[i]->_mp_d, prep[i].n * 8);
+  HIPCHK(hipMemcpyAsync(A, H, up_limbs * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(A + limbs_b, H + limbs_b, rt_b, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dslab_p, H + limbs_b + rt_b, d_total * 8, hipMemcpyHostToDevice, st));
+  // the levels, bottom-up
+  for (size_t j = levels; j-- > 0;) {
+    for (size_t k = 0; k < who[j].size(); k++) {
+      const ModPrep& P = prep[who[j][k]];
+      Product& a = p1[j][k];
+      a.A[0].dptr = L + P.dn_off + (P.np - P.c[j]);
+      a.B[0].dptr = L + P.x_off[j + 1];
+      a.out.assign(1, L + P.t_off);
+      Product& b = p2[j][k];
+      b.A[0].dptr = L + P.tm_off;
+      b.B[0].dptr = L + P.x_off[j + 1];
+      b.out.assign(1, L + P.u_off);
+      rts[rt_first[j] + k].tn = a.plan.rn;
+      rts[rt_first[j] + k].un = b.plan.rn;
+    }
+    // the lengths tn / un come from the plans: patch them into the uploaded table
+    HIPCHK(hipStreamSynchronize(st));
+    rc = engine_stage(who[j].size() * sizeof(RfRecipTask), &H);
+    if (rc) return rc;
+    memcpy(H, rts.data() + rt_first[j], who[j].size() * sizeof(RfRecipTask));
+    HIPCHK(hipMemcpyAsync(d_rt + rt_first[j], H, who[j].size() * sizeof(RfRecipTask), hipMemcpyHostToDevice, st));
+    rc = pipeline_run(p1[j], layout_of(p1[j]), pipe);
+    if (rc) return rc;
+    HIPCHK(rf_launch_recip_adjust(d_rt + rt_first[j], (int)who[j].size(), st));
+    rc = pipeline_run(p2[j], layout_of(p2[j]), pipe);
+    if (rc) return rc;
+    HIPCHK(rf_launch_recip_combine(d_rt + rt_first[j], (int)who[j].size(), st));
+  }
+  HIPCHK(rf_launch_recip_final(d_fin, cnt, max_il, st));
+  if (recips) {
+    HIPCHK(hipStreamSynchronize(st));
+    recips->assign(cnt, {});
+    for (int i = 0; i < cnt; i++) {
+      (*recips)[i].resize(prep[i].il);
+      HIPCHK(hipMemcpy((*recips)[i].data(), L + prep[i].i_off, prep[i].il * 8, hipMemcpyDeviceToHost));
+    }
+    return RF_OK;
+  }
+  // D and I as resident transforms, all in one set of launches each way
+  std::vector<Entry> ents;
+  std::vector<uint64_t> bmax;
+  std::vector<RfPlan> cyc(cnt);
+  std::vector<const RfPlan*> fixed;
+  std::vector<uint64_t> wrap(cnt, 0);
+  // the cyclic-or-whole decision depends on (n, m, bits of D) only: moduli
+  // of a level mostly share it
+  std::map<std::tuple<uint64_t, uint64_t, uint64_t>, std::pair<bool, RfPlan>> decided;
+  for (int i = 0; i < cnt; i++) {
+    const ModPrep& P = prep[i];
+    const uint64_t q_bits = 64 * (P.m - P.n + 1);  // X_hi and Q limbs
+    ents.push_back(Entry{0, L + P.i_off, P.il, 64 * P.il, 1});
+    bmax.push_back(q_bits);
+    fixed.push_back(nullptr);
     // Q D is needed only modulo B^W - 1 for W >= n + 1 (R < 3D < B^W - 1) and
     // W >= m - n + 1 (Q fits): a cyclic product of about half the length,
     // when the cost model prefers it
-    const uint64_t qn = md->m - md->n + 1;
-    RfPlan cyc{}, acyc{};
-    const bool use_cyc =
-        rf_plan_cyclic(&engine_primes(), std::max<uint64_t>(md->n + 1, qn), md->n, qn, &cyc) == 0 &&
-        (rf_plan_choose(&engine_primes(), E[0].bits, q_bits, 1, 1, &acyc) != 0 || cyc.cost < acyc.cost);
-    if (use_cyc) md->wrap = (uint64_t)cyc.t << cyc.lgN;
-    rc = make_operand(E, 1, 1, q_bits, i_b, &md->div, use_cyc ? &cyc : nullptr);
+    const uint64_t qn = P.m - P.n + 1, dbits = mpz_sizeinbase(ds[i], 2);
+    const auto key = std::make_tuple(P.n, P.m, dbits);
+    auto it = decided.find(key);
+    if (it == decided.end()) {
+      RfPlan c{}, acyc{};
+      const bool u = rf_plan_cyclic(&engine_primes(), std::max<uint64_t>(P.n + 1, qn), P.n, qn, &c) == 0 &&
+                     (rf_plan_choose(&engine_primes(), dbits, q_bits, 1, 1, &acyc) != 0 || c.cost < acyc.cost);
+      it = decided.emplace(key, std::make_pair(u, c)).first;
+    }
+    const bool use_cyc = it->second.first;
+    cyc[i] = it->second.second;
+    if (use_cyc) wrap[i] = (uint64_t)cyc[i].t << cyc[i].lgN;
+    ents.push_back(Entry{0, dslab_p + d_off[i], P.n, dbits, 1});
+    bmax.push_back(q_bits);
+    fixed.push_back(use_cyc ? &cyc[i] : nullptr);
   }
-  if (rc) {
-    free_operand(md->inv);
-    free_operand(md->div);
-    if (md->d_dev) (void)hipFree(md->d_dev);
-    delete md;
-    return rc;
+  std::vector<rf_operand*> ops;
+  rc = make_operands_batch(ents, bmax, fixed, pipe, ops);  // waits for the reciprocals first (same stream)
+  if (rc) return rc;
+  for (int i = 0; i < cnt; i++) {
+    rf_mod* md = new rf_mod();
+    md->n = prep[i].n;
+    md->m = prep[i].m;
+    md->wrap = wrap[i];
+    md->inv = ops[2 * i];
+    md->div = ops[2 * i + 1];
+    md->d_dev = dslab_p + d_off[i];
+    md->d_slab = dslab;
+    outs[i] = md;
   }
-  *out = md;
   return RF_OK;
+}
+
+int mod_create_batch(const uint64_t* max_bits, const mpz_t* ds, int cnt, rf_mod** outs,
+                     std::vector<std::vector<uint64_t>>* recips) {
+  std::vector<ModPrep> prep(cnt);
+  // host preparation (GMP base cases) on up to 16 threads for large levels
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int i = next++; i < cnt; i = next++) mod_prep(max_bits[i], ds[i], prep[i]);
+  };
+  const int nth = cnt < 32 ? 1 : std::max(1, std::min({cnt / 16, 16, (int)std::thread::hardware_concurrency()}));
+  std::vector<std::thread> pool;
+  try {
+    for (int t = 1; t < nth; t++) pool.emplace_back(work);
+  } catch (const std::system_error&) {
+    // fewer helper threads: the calling thread takes what is left
+  }
+  work();
+  for (auto& t : pool) t.join();
+  std::lock_guard<std::mutex> lk(engine_mutex());
+  // hw_mpz_clear may have run meanwhile
+  if (!engine_ready()) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
+  return mod_create_batch_locked(ds, prep, outs, recips);
 }
 
 }  // namespace
@@ -261,22 +484,7 @@ int rf_mod_create(uint64_t max_bits, const mpz_t d, rf_mod** out) {
   *out = nullptr;
   if (!d || mpz_sgn(d) <= 0) return fail(RF_ERR_ARG, "rf_mod_create: the modulus must be positive");
   if (max_bits == 0) return fail(RF_ERR_ARG, "rf_mod_create: max_bits must be positive");
-  {
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    if (!engine_ready()) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
-  }
-  uint64_t n = 0, m = 0;
-  mpz_t I;
-  mpz_init(I);
-  mod_reciprocal(max_bits, d, &n, &m, I);  // outside the engine mutex
-  int rc;
-  {
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    rc = engine_ready() ? mod_create_locked(d, n, m, I, out)
-                        : fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
-  }
-  mpz_clear(I);
-  return rc;
+  return mod_create_batch(&max_bits, (const mpz_t*)d, 1, out, nullptr);
 }
 
 int rf_mod_create_batch(const uint64_t* max_bits, const mpz_t* ds, int cnt, rf_mod** outs) {
@@ -286,43 +494,25 @@ int rf_mod_create_batch(const uint64_t* max_bits, const mpz_t* ds, int cnt, rf_m
     if (mpz_sgn(ds[i]) <= 0) return fail(RF_ERR_ARG, "rf_mod_create_batch: modulus %d is not positive", i);
     if (max_bits[i] == 0) return fail(RF_ERR_ARG, "rf_mod_create_batch: max_bits[%d] must be positive", i);
   }
-  {
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    if (!engine_ready()) return fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
-  }
-  // the GMP divisions dominate a modulus's preparation: run them on up to
-  // 16 host threads, then make the resident transforms in order
-  std::vector<uint64_t> n(cnt), m(cnt);
-  std::vector<__mpz_struct> I(cnt);
-  for (int i = 0; i < cnt; i++) mpz_init(&I[i]);
-  std::atomic<int> next{0};
-  auto work = [&]() {
-    for (int i = next++; i < cnt; i = next++) mod_reciprocal(max_bits[i], ds[i], &n[i], &m[i], &I[i]);
-  };
-  const int nth = std::max(1, std::min({cnt, 16, (int)std::thread::hardware_concurrency()}));
-  std::vector<std::thread> pool;
-  try {
-    for (int t = 1; t < nth; t++) pool.emplace_back(work);
-  } catch (const std::system_error&) {
-    // fewer helper threads: the calling thread takes what is left
-  }
-  work();
-  for (auto& t : pool) t.join();
-  int rc = RF_OK;
-  {
-    std::lock_guard<std::mutex> lk(engine_mutex());
-    // hw_mpz_clear may have run while the divisions were unlocked
-    if (!engine_ready()) rc = fail(RF_ERR_NOT_INIT, "hw_mpz_setup() has not been called");
-    for (int i = 0; i < cnt && !rc; i++) rc = mod_create_locked(ds[i], n[i], m[i], &I[i], &outs[i]);
-  }
-  for (int i = 0; i < cnt; i++) mpz_clear(&I[i]);
-  if (rc)
-    for (int i = 0; i < cnt; i++)
-      if (outs[i]) {
-        rf_mod_destroy(outs[i]);
-        outs[i] = nullptr;
-      }
-  return rc;
+  if (cnt == 0) return RF_OK;
+  return mod_create_batch(max_bits, ds, cnt, outs, nullptr);
+}
+
+int rf_stage_mod_reciprocal(const uint64_t* d, uint64_t dn, uint64_t max_bits, uint64_t* out, uint64_t cap,
+                            uint64_t* nlimbs) {
+  if (!d || dn == 0 || d[dn - 1] == 0 || max_bits == 0 || !out || !nlimbs)
+    return fail(RF_ERR_ARG, "rf_stage_mod_reciprocal: bad arguments");
+  mpz_t D;
+  mpz_init(D);
+  mpz_import(D, dn, -1, 8, 0, 0, d);
+  std::vector<std::vector<uint64_t>> rec;
+  const int rc = mod_create_batch(&max_bits, (const mpz_t*)D, 1, nullptr, &rec);
+  mpz_clear(D);
+  if (rc) return rc;
+  if (rec[0].size() > cap) return fail(RF_ERR_CAPACITY, "the reciprocal has %zu limbs", rec[0].size());
+  memcpy(out, rec[0].data(), rec[0].size() * 8);
+  *nlimbs = rec[0].size();
+  return RF_OK;
 }
 
 uint64_t rf_mod_limbs(const rf_mod* mod) { return mod ? mod->n : 0; }
@@ -393,7 +583,7 @@ void rf_mod_destroy(rf_mod* mod) {
   std::lock_guard<std::mutex> lk(engine_mutex());
   free_operand(mod->inv);
   free_operand(mod->div);
-  if (mod->d_dev) (void)hipFree(mod->d_dev);
+  mod->d_slab.reset();  // D's limbs, freed with the batch's last modulus
   delete mod;
 }
 

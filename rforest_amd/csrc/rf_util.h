@@ -82,6 +82,35 @@ struct RfModTask {
 };
 hipError_t rf_launch_mod_fix(const RfModTask* tasks, int ntasks, hipStream_t st);
 
+// One level of the approximate-reciprocal recursion (Brent & Zimmermann,
+// Modern Computer Arithmetic, Algorithm 3.5) for a normalised A of n = l + h
+// limbs, given X_h ~ B^(2h) / A_h for its top h limbs (h + 1 limbs) and
+// T = A X_h (the first product, tn limbs):
+//   k_recip_adjust: while T >= B^(n+h): X_h -= 1, T -= A; then
+//                   Tm = floor((B^(n+h) - T) / B^l)  (h + 1 limbs)
+//   k_recip_combine (after the second product U = Tm X_h, un limbs):
+//                   X = X_h B^l + floor(U / B^(2h-l))  (n + 1 limbs)
+struct RfRecipTask {
+  const uint64_t* a;  // A, n limbs (normalised: top bit set)
+  uint64_t* xh;       // X_h, h + 1 limbs (adjusted in place)
+  uint64_t* t;        // T = A X_h, tn limbs (overwritten)
+  uint64_t* tm;       // Tm, h + 1 limbs
+  const uint64_t* u;  // U = Tm X_h, un limbs
+  uint64_t* x;        // X, n + 1 limbs
+  uint64_t n, h, l, tn, un;
+};
+hipError_t rf_launch_recip_adjust(const RfRecipTask* tasks, int ntasks, hipStream_t st);
+hipError_t rf_launch_recip_combine(const RfRecipTask* tasks, int ntasks, hipStream_t st);
+// I = floor(X 2^s / B^o) into out (il limbs): X of xn limbs, 0 <= s < 64, o >= 1.
+struct RfRecipFinal {
+  const uint64_t* x;
+  uint64_t xn;
+  uint64_t* out;
+  uint64_t il;
+  uint32_t s, o;
+};
+hipError_t rf_launch_recip_final(const RfRecipFinal* tasks, int ntasks, uint64_t max_il, hipStream_t st);
+
 hipError_t rf_launch_gen(const RfSpan* spans, const RfBlock* blocks, int nblocks, hipStream_t st);
 hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nblocks, uint64_t* sums,
                               hipStream_t st);

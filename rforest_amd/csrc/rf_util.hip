@@ -290,6 +290,56 @@ __global__ __launch_bounds__(RF_FIX_THREADS) void k_mod_fix(const RfModTask* __r
   for (uint64_t i = threadIdx.x; i < tk.n; i += RF_FIX_THREADS) tk.out[i] = R[i];
 }
 
+// ---- the approximate-reciprocal recursion (one workgroup per modulus) ----
+//
+// With A_h X_h < B^(2h) <= A_h (X_h + 2) (the lower level's guarantee) and
+// A = A_h B^l + A_l, T = A X_h < B^(n+h) + 2 B^n, so T >= B^(n+h) means limb
+// n + h of T is 1 and at most a few subtractions of A (A >= B^n / 2) bring T
+// below B^(n+h); the loop stops on the flag, bounded at 8 rounds.  Then
+// 0 < B^(n+h) - T <= 2A < 2 B^n, so Tm = floor((B^(n+h) - T) / B^l) < 2 B^h
+// fits h + 1 limbs.
+__global__ __launch_bounds__(RF_FIX_THREADS) void k_recip_adjust(const RfRecipTask* __restrict__ tasks, int ntasks) {
+  __shared__ int sh[FIX_SH];
+  if ((int)blockIdx.x >= ntasks) return;
+  const RfRecipTask tk = tasks[blockIdx.x];
+  const uint64_t nh = tk.n + tk.h;
+  const uint64_t one = 1;
+  for (int round = 0; round < 8; round++) {
+    __syncthreads();
+    const bool over = nh < tk.tn && tk.t[nh] != 0;  // every thread reads the same limb
+    if (!over) break;
+    sub_limbs(LimbSpan{tk.t, tk.tn}, LimbSpan{tk.a, tk.n}, tk.t, tk.tn, sh);
+    sub_limbs(LimbSpan{tk.xh, tk.h + 1}, LimbSpan{&one, 1}, tk.xh, tk.h + 1, sh);
+    __syncthreads();
+  }
+  // B^(n+h) - T = -T mod B^(n+h) (T > 0), written over T
+  const uint64_t zero = 0;
+  sub_limbs(LimbSpan{&zero, 1}, LimbSpan{tk.t, nh}, tk.t, nh, sh);
+  __syncthreads();
+  for (uint64_t i = threadIdx.x; i <= tk.h; i += RF_FIX_THREADS) tk.tm[i] = tk.l + i < nh ? tk.t[tk.l + i] : 0;
+}
+
+// X = X_h B^l + floor(U / B^(2h-l)): the low l limbs are U's limbs
+// [2h - l, 2h), the rest X_h plus U's limbs from 2h on (U < 4 B^(2h)).
+__global__ __launch_bounds__(RF_FIX_THREADS) void k_recip_combine(const RfRecipTask* __restrict__ tasks, int ntasks) {
+  __shared__ int sh[FIX_SH];
+  if ((int)blockIdx.x >= ntasks) return;
+  const RfRecipTask tk = tasks[blockIdx.x];
+  const uint64_t lo = 2 * tk.h - tk.l;
+  for (uint64_t i = threadIdx.x; i < tk.l; i += RF_FIX_THREADS) tk.x[i] = lo + i < tk.un ? tk.u[lo + i] : 0;
+  const LimbSpan uh{tk.un > 2 * tk.h ? tk.u + 2 * tk.h : tk.u, tk.un > 2 * tk.h ? tk.un - 2 * tk.h : 0};
+  add_limbs(LimbSpan{tk.xh, tk.h + 1}, uh, tk.x + tk.l, tk.h + 1, sh);
+}
+
+__global__ __launch_bounds__(256) void k_recip_final(const RfRecipFinal* __restrict__ tasks, int ntasks) {
+  const RfRecipFinal tk = tasks[blockIdx.y];
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= tk.il) return;
+  const uint64_t hi = j + tk.o < tk.xn ? tk.x[j + tk.o] : 0;
+  const uint64_t lo = j + tk.o - 1 < tk.xn ? tk.x[j + tk.o - 1] : 0;
+  tk.out[j] = tk.s ? (hi << tk.s) | (lo >> (64 - tk.s)) : hi;
+}
+
 __global__ void k_fill(uint64_t* __restrict__ dst, uint64_t n, uint64_t v) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = v;
@@ -320,6 +370,26 @@ hipError_t rf_launch_fill(uint64_t* dst, uint64_t n, uint64_t v, hipStream_t st)
 hipError_t rf_launch_mod_fix(const RfModTask* tasks, int ntasks, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_mod_fix, dim3(ntasks), dim3(RF_FIX_THREADS), 0, st, tasks, ntasks);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_recip_adjust(const RfRecipTask* tasks, int ntasks, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_recip_adjust, dim3(ntasks), dim3(RF_FIX_THREADS), 0, st, tasks, ntasks);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_recip_combine(const RfRecipTask* tasks, int ntasks, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_recip_combine, dim3(ntasks), dim3(RF_FIX_THREADS), 0, st, tasks, ntasks);
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_recip_final(const RfRecipFinal* tasks, int ntasks, uint64_t max_il, hipStream_t st) {
+  if (ntasks <= 0 || max_il == 0) return hipSuccess;
+  // the grid covers the longest output; shorter ones stop early
+  hipLaunchKernelGGL(k_recip_final, dim3((unsigned)((max_il + 255) / 256), (unsigned)ntasks), dim3(256), 0, st, tasks,
+                     ntasks);
   return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 """Remainders X mod D (rf_mod_*, the reference's mpzfft_mod interface,
 src/lib/mpzfft.c:59-100) against Python's x % d, which is in [0, d) for
 d > 0 like the reference's output."""
+import ctypes
 import random
 
 import pytest
@@ -147,3 +148,38 @@ def test_batch_creation():
         M.close()
     with pytest.raises(rf.RfError):
         Modulus.batch([5, 0, 7], 64)
+
+
+def _recip(d, max_bits):
+    import numpy as np
+    L = rf.lib()
+    L.rf_stage_mod_reciprocal.restype = ctypes.c_int
+    L.rf_stage_mod_reciprocal.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p,
+                                          ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    dl = np.frombuffer(d.to_bytes(8 * ((d.bit_length() + 63) // 64), "little"), dtype=np.uint64).copy()
+    m = max((max_bits + 63) // 64, len(dl))
+    out = np.zeros(m - len(dl) + 2, dtype=np.uint64)
+    n = ctypes.c_uint64()
+    rf.check(L.rf_stage_mod_reciprocal(dl.ctypes.data, len(dl), max_bits, out.ctypes.data, len(out), ctypes.byref(n)))
+    assert n.value == len(out)
+    return int.from_bytes(out.tobytes(), "little"), m
+
+
+@pytest.mark.parametrize("dbits,xbits", [(64, 64), (100, 300), (4096, 8192), (4096 + 1, 8192), (64 * 64, 64 * 64 * 2),
+                                         (64 * 65 - 3, 64 * 130), (200000, 400000), (200000, 1 << 20),
+                                         (1 << 20, 1 << 21), ((1 << 20) + 7, 1 << 21)])
+def test_device_reciprocal_within_two_of_the_floor(dbits, xbits):
+    """The reciprocal a modulus keeps, computed by the device's Newton
+    recursion, lies in [floor(B^(m+1)/D) - 2, floor(B^(m+1)/D)] (the
+    remainder step's quotient estimate needs exactly that), for moduli with
+    the top bit set or clear, powers of two, all-ones, and sizes below, at
+    and above the recursion's 64-limb host base."""
+    rng = random.Random(dbits * 7 + xbits)
+    cases = [rng.getrandbits(dbits) | (1 << (dbits - 1)), 1 << (dbits - 1), (1 << dbits) - 1,
+             (1 << (dbits - 1)) + 1]
+    if dbits > 64:
+        cases.append(rng.getrandbits(dbits - 5) | (1 << (dbits - 6)))  # top limb small
+    for d in cases:
+        I, m = _recip(d, xbits)
+        exact = (1 << (64 * (m + 1))) // d
+        assert exact - 2 <= I <= exact, (dbits, xbits, exact - I)
