@@ -3,11 +3,12 @@
 // src/include/mpnfft_mod.h:13-75).
 //
 // With n = limbs of D, m = most limbs of |X| and B = 2^64, creation computes
-// I = floor(B^(m+1) / D) once and keeps I and D as resident transforms.  For
-// X = X_hi B^(n-1) + X_lo:
+// I with floor(B^(m+1) / D) - 2 <= I <= floor(B^(m+1) / D) once and keeps I
+// and D as resident transforms.  For X = X_hi B^(n-1) + X_lo:
 //   Q = floor(X_hi I / B^(m-n+2))  satisfies  X/D - 3 < Q <= X/D,
 // since X_hi I / B^(m-n+2) = (X - X_lo)/D - X_hi e / B^(m-n+2) with
-// e = B^(m+1)/D - I in [0, 1), X_lo / D < 1 and X_hi < B^(m-n+1).  So
+// e = B^(m+1)/D - I in [0, 3), X_lo / D < 1 and X_hi < B^(m-n+1), so the
+// middle term is below 3 / B.  So
 // R = X - Q D lies in [0, 3D) and fits n + 1 limbs, two corrections at most
 // bring it into [0, D), and a negative X takes D - R.  Per remainder: one
 // product with the resident I (its high limbs are Q, read in place), one with
