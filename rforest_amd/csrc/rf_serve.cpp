@@ -459,6 +459,10 @@ bool admit(rf_server* S, Slot& sl) {
       HIPCHK(rf_launch_checksum((const RfSpan*)(d_util + (o_ss - o_gs)), (const RfBlock*)(d_util + (o_sb - o_gs)),
                                 (int)sblocks.size(), sl.d_sums, sl.st));
       HIPCHK(hipMemcpyAsync(sl.h_sums, sl.d_sums, take.size() * 8, hipMemcpyDeviceToHost, sl.st));
+    } else {
+      // no result limbs to sum (every product zero): the checksums are 0,
+      // not what an earlier batch of this slot left in h_sums
+      std::fill(sl.h_sums, sl.h_sums + take.size(), (uint64_t)0);
     }
     for (size_t i = 0; i < take.size(); i++) {  // results to staging; to the callers at retirement
       Req* q = take[i];

@@ -87,6 +87,19 @@ def test_server_host_inputs_exact():
     srv.close()
 
 
+def test_zero_result_alone_in_its_batch_has_checksum_zero():
+    # A batch whose every result has no limbs sums nothing on the device; its
+    # checksums must be 0, not what the slot's previous batch left behind.
+    srv = serve.Server(1 << 30, streams=1, max_batch=1)
+    for i, (A, B) in enumerate([([3 << 200], [5 << 100]), ([0], [7]), ([1 << 64], [1]), ([5], [0])]):
+        rid = srv.submit_host(1, 1, A, B, tag=i)
+        (c,) = srv.wait_all(1)
+        C = srv.host_result(rid, c["out_limbs"])
+        assert C == [A[0] * B[0]]
+        assert c["checksum"] == serve.checksum_values(C, c["out_limbs"]), i
+    srv.close()
+
+
 def test_server_mix_equals_alone_and_gmp():
     stream = _mixed_stream(150, 77)
     srv = serve.Server(8 << 30, streams=3, max_batch=64)
