@@ -22,39 +22,40 @@ __device__ __forceinline__ int fn_apply(uint32_t code, int c) {
   return (int)((code >> (2 * (c + 1))) & 3u) - 1;
 }
 __device__ __forceinline__ uint32_t fn_compose(uint32_t g, uint32_t f) {  // g o f
+  // field c of g o f is field (field c of f) of g: both hold carry + 1
   uint32_t h = 0;
 #pragma unroll
-  for (int c = -1; c <= 1; c++) h |= (uint32_t)(fn_apply(g, fn_apply(f, c)) + 1) << (2 * (c + 1));
+  for (int c = 0; c < 3; c++) h |= __builtin_amdgcn_ubfe(g, 2 * __builtin_amdgcn_ubfe(f, 2 * c, 2), 2) << (2 * c);
   return h;
 }
 constexpr uint32_t FN_ID = (0u << 0) | (1u << 2) | (2u << 4);
+// Whether f maps every incoming carry to the same outgoing one.
+__device__ __forceinline__ bool fn_constant(uint32_t f) { return (f & 15u) == ((f >> 2) & 15u); }
 
 // Garner's algorithm on residues res[j * N] (|x| <= p_j/2 + 8) into the
 // S-limb two's complement of the representative in (-P/2, P/2)
-// (src/lib/crt.c:11-94 states the same CRT on 62-bit primes).
-// Mixed-radix digits v_j in [0, p_j): v_j = (x_j - [v_0 + p_0 (v_1 + ...)]) /
-// (p_0 ... p_{j-1}) mod p_j; all fp64 values stay below 2^50.
+// (src/lib/crt.c:11-94 states the same CRT on 62-bit primes, as nested
+// Horner steps).  Mixed-radix digits v_j in [0, p_j) with
+// X = v_0 + p_0 (v_1 + p_1 (v_2 + ...)): y_m starts as x_m, and once v_j is
+// known every later y_m <- (y_m - v_j) p_j^-1 mod p_m, which leaves y_m the
+// residue modulo p_m of (X - v_0 - ... - v_j p_0...p_{j-1}) / (p_0...p_j), so
+// v_{j+1} = y_{j+1} mod p_{j+1}.  The updates of one step are independent of
+// each other (the nested form is one chain of dependent products).
+// Bounds: primes below 2^47.9, so |y_m - v_j| < 0.625 p_m + p_j < 2^49 and
+// the product is at most p_m (1/2 + 2^49 2^-52) = 0.625 p_m; starting from
+// |x_m| <= p_m/2 + 8 every y_m stays within 0.625 p_m < p_m, the range
+// rf_canon1 takes.
 template <int K, int S>
 __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t N,
                                        const RfConsts* __restrict__ C, uint64_t (&X)[S]) {
-  double v[K];
+  double y[K], v[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) y[j] = res[(size_t)j * N];
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    const double p = C->p[j];
-    const double x = res[(size_t)j * N];
-    double r;
-    if (j == 0) {
-      r = x;
-    } else {
-      double acc = v[j - 1];
+    v[j] = rf_canon1(y[j], C->p[j], C->pinv[j]);
 #pragma unroll
-      for (int l = j - 2; l >= 0; l--) acc = rf_mulmod(acc, C->pm[j][l][0], C->pm[j][l][1], p) + v[l];
-      r = rf_mulmod(x - acc, C->ci[j][0], C->ci[j][1], p);
-    }
-    // |r| < p: x is reduced (|x| <= p/2 + 8); acc, a product plus a
-    // canonical digit, stays below 1.5 p + 8, so |x - acc| < 2^50 and the
-    // product r satisfies |r| <= p (1/2 + 2^50 2^-52) = 0.75 p
-    v[j] = rf_canon1(r, p, C->pinv[j]);
+    for (int m = j + 1; m < K; m++) y[m] = rf_mulmod(y[m] - v[j], C->gi[m][j][0], C->gi[m][j][1], C->p[m]);
   }
   // Mixed radix to binary: X = v_{K-1}; X = X p_j + v_j for j = K-2 .. 0, in
   // 32-bit limbs.  p_j = phi 2^32 + plo with phi < 2^16 and, as p_j = 1 mod
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
   // limb 0 kept) and carry out E_i
   const uint32_t ndig = (tk.rn + t - 1) / t;
   const bool own = tid >= H && (uint32_t)i < ndig;
-  uint64_t* dst = tk.out + (size_t)(own ? i : 0) * t;
+  uint64_t RF_GLOBAL* dst = (uint64_t RF_GLOBAL*)(tk.out + (size_t)(own ? i : 0) * t);
   const uint32_t lim = own ? tk.rn - (uint32_t)i * t : 0u;  // limbs of the digit inside the output
   uint64_t v0 = 0;
   bool hz = true, ho = true;  // limbs 1 .. t-1 all zero / all ones
@@ -270,12 +271,18 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
     }
   }
   // Scan by composition: inclusive f_tid o ... o f_0 within each wave by
-  // shuffles, then each wave composes the earlier waves' aggregates.
+  // shuffles, then each wave composes the earlier waves' aggregates.  A
+  // constant function composed after anything is itself, so in a wave whose
+  // functions are all constant (a digit that absorbs any incoming carry: the
+  // usual case) every lane's inclusive prefix is its own function and the
+  // shuffle scan is skipped.
   const int lane = tid & 63, wave = tid >> 6;
+  if (__builtin_amdgcn_ballot_w64(!fn_constant(f)) != 0) {
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t g = __shfl_up(f, off);
-    if (lane >= off) f = fn_compose(f, g);
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t g = __shfl_up(f, off);
+      if (lane >= off) f = fn_compose(f, g);
+    }
   }
   if (lane == 63) code[wave] = f;
   __syncthreads();
@@ -321,7 +328,7 @@ __global__ __launch_bounds__(64) void k_apply(const RfCrtTask* __restrict__ task
     if (i >= ndig) continue;
     const int c = fn_apply(pre[tk.pre_off + tid], cin);
     if (c == 0) continue;
-    uint64_t* dst = tk.out + (size_t)i * t;
+    uint64_t RF_GLOBAL* dst = (uint64_t RF_GLOBAL*)(tk.out + (size_t)i * t);
     const uint32_t lim = min((uint32_t)t, tk.rn - i * t);
     for (uint32_t l = 0; l < lim; l++) {
       const uint64_t x = dst[l];

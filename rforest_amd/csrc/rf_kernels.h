@@ -13,6 +13,39 @@
 
 #include "rf_plan.h"
 
+// Global-memory views of device pointers.  Pointers that arrive inside task
+// descriptors or table structs are generic to the compiler, and loads through
+// them become flat_* instructions, whose completion the LDS wait counter also
+// tracks: every wait for an LDS exchange would then wait for outstanding
+// twiddle or limb loads too.  Loads through these types are global_*.
+// (The host pass of a .hip source parses device code too; it has no such
+// address space, and the qualifier applies to the device pass only.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RF_GLOBAL __attribute__((address_space(1)))
+#else
+#define RF_GLOBAL
+#endif
+typedef const double2 RF_GLOBAL* rf_gtw;       // twiddle tables (w, w / p)
+typedef const uint64_t RF_GLOBAL* rf_glimbs;   // entry magnitudes
+typedef const ulonglong2 RF_GLOBAL* rf_glimbs2;
+typedef const double RF_GLOBAL* rf_gres;       // stored residues
+template <typename T>
+__device__ __forceinline__ T rf_ld(const T RF_GLOBAL* p) {
+  return *p;
+}
+// Stores of results written once and read by a later launch (spectra,
+// coefficient residues), marked non-temporal: they stream past the L2
+// instead of evicting the twiddle tables and limbs the running kernel
+// re-reads from it.
+typedef double rf_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void rf_st_nt(double* p, double x) {
+  __builtin_nontemporal_store(x, (double RF_GLOBAL*)p);
+}
+__device__ __forceinline__ void rf_st_nt2(double* p, double x, double y) {
+  const rf_d2v v = {x, y};
+  __builtin_nontemporal_store(v, (rf_d2v RF_GLOBAL*)p);
+}
+
 // One forward task: pack chunks of one entry modulo prime q, transform.
 struct RfFwdTask {
   const uint64_t* limbs;  // entry magnitude (device pointer)

@@ -172,7 +172,7 @@ __device__ __forceinline__ int inv_index(int g, int j) {
 // transform (ROW) prefixes its row index hb to the block index:
 // T[(hb 2^L0 + H) 2^s + (j >> (R-s))].
 template <int LGN, int P, int S0 = 0, bool ROW = false>
-__device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __restrict__ tw,
+__device__ __forceinline__ void fwd_pass(double* v, int tid, rf_gtw __restrict__ tw,
                                          double p, int hb = 0) {
   using Q = Pass<LGN, P>;
   constexpr int NLO = LGN - Q::L0 - Q::R;
@@ -188,7 +188,7 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __re
         // in pass 0 of a whole transform (H = 0) block 0 uses T[0] = 1: no
         // multiplication
         const bool one = !ROW && (P == 0) && (b == 0);
-        const double2 w = one ? make_double2(1.0, 0.0) : tw[(H << s) + b];
+        const double2 w = one ? make_double2(1.0, 0.0) : rf_ld(tw + ((H << s) + b));
 #pragma unroll
         for (int jj = 0; jj < half; jj++) {
           const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, const double2* __re
 // R inverse (decimation-in-time) stages: at stage L = L0 + s the pair
 // (j, j + 2^s) uses V[2^L + (j mod 2^s) 2^L0 + lo] = omega_{2^(L+1)}^-(i mod 2^L).
 template <int LGN, int P, int R0 = inv_r0<LGN>()>
-__device__ __forceinline__ void inv_pass(double* v, int tid, const double2* __restrict__ tw,
+__device__ __forceinline__ void inv_pass(double* v, int tid, rf_gtw __restrict__ tw,
                                          double p) {
   using Q = Pass<LGN, P, R0>;
 #pragma unroll
@@ -220,7 +220,7 @@ __device__ __forceinline__ void inv_pass(double* v, int tid, const double2* __re
       for (int jj = 0; jj < half; jj++) {
         // in pass 0 (lo = 0) the pair with jj = 0 uses V[len] = 1
         const bool one = (P == 0) && (jj == 0);
-        const double2 w = one ? make_double2(1.0, 0.0) : tw[len + (jj << Q::L0) + lo];
+        const double2 w = one ? make_double2(1.0, 0.0) : rf_ld(tw + (len + (jj << Q::L0) + lo));
 #pragma unroll
         for (int b = 0; b < (Q::GS >> (s + 1)); b++) {
           const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
@@ -254,12 +254,12 @@ __device__ __forceinline__ void fwd_store(double* __restrict__ out, const double
     if constexpr (NLO == 0 && QL::GS >= 2) {
 #pragma unroll
       for (int j = 0; j < QL::GS; j += 2)
-        *reinterpret_cast<double2*>(out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j)) =
-            make_double2(rf_reduce(v[u * QL::GS + j], p, pinv), rf_reduce(v[u * QL::GS + j + 1], p, pinv));
+        rf_st_nt2(out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j),
+                  rf_reduce(v[u * QL::GS + j], p, pinv), rf_reduce(v[u * QL::GS + j + 1], p, pinv));
     } else {
 #pragma unroll
       for (int j = 0; j < QL::GS; j++)
-        out[fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+        rf_st_nt(out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j), rf_reduce(v[u * QL::GS + j], p, pinv));
     }
   }
 }
@@ -267,7 +267,7 @@ __device__ __forceinline__ void fwd_store(double* __restrict__ out, const double
 // Passes 1 .. NP-1 of the forward transform: LDS exchange, optional
 // reduction, stages.
 template <int LGN, int P, bool ROW = false>
-__device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const double2* tw,
+__device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, rf_gtw tw,
                                          double p, double pinv, int hb = 0) {
   if constexpr (P < Shape<LGN>::NP) {
     using Q = Pass<LGN, P>;
@@ -296,7 +296,7 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, const d
 }
 
 template <int LGN, int P>
-__device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const double2* tw,
+__device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, rf_gtw tw,
                                          double p, double pinv) {
   constexpr int R0 = inv_r0<LGN>();
   if constexpr (P < npasses<LGN, R0>()) {
@@ -334,7 +334,7 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, const d
 // checks; otherwise loads past the entry's end read limb 0 and are
 // discarded.  Control flow depends only on the uniform t and that test.
 template <int NCH>
-__device__ __forceinline__ void pack_chunks(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+__device__ __forceinline__ void pack_chunks(rf_glimbs __restrict__ limbs, uint32_t nlimbs,
                                             const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
                                             double p, double pinv,
                                             double (&out)[NCH]) {
@@ -411,7 +411,7 @@ constexpr int pack_sb() {
 }
 
 template <int NCH, int T>
-__device__ __forceinline__ void pack_full16(const uint64_t* __restrict__ limbs, const uint32_t (&ci)[NCH],
+__device__ __forceinline__ void pack_full16(rf_glimbs __restrict__ limbs, const uint32_t (&ci)[NCH],
                                             const double (*pk)[3], double p, double pinv,
                                             double (&out)[NCH]) {
   constexpr int SB = pack_sb<NCH, T>();
@@ -421,7 +421,7 @@ __device__ __forceinline__ void pack_full16(const uint64_t* __restrict__ limbs, 
     uint64_t x[SB][T];
 #pragma unroll
     for (int c = 0; c < SB; c++) {
-      const ulonglong2* src = reinterpret_cast<const ulonglong2*>(limbs + (size_t)ci[c0 + c] * T);
+      rf_glimbs2 src = reinterpret_cast<rf_glimbs2>(limbs + (size_t)ci[c0 + c] * T);
 #pragma unroll
       for (int l = 0; l < T / 2; l++) {
         const ulonglong2 y = src[l];
@@ -435,7 +435,7 @@ __device__ __forceinline__ void pack_full16(const uint64_t* __restrict__ limbs, 
 }
 
 template <int NCH, int T>
-__device__ __forceinline__ void pack_any8(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+__device__ __forceinline__ void pack_any8(rf_glimbs __restrict__ limbs, uint32_t nlimbs,
                                           const uint32_t (&ci)[NCH], const double (*pk)[3],
                                           double p, double pinv, double (&out)[NCH]) {
   constexpr int SB = pack_sb<NCH, T>();
@@ -458,7 +458,7 @@ __device__ __forceinline__ void pack_any8(const uint64_t* __restrict__ limbs, ui
 
 // T > 0: chunk width fixed at compile time (t == T); T == 0: run-time t.
 template <int NCH, int T>
-__device__ __forceinline__ void pack_dispatch(const uint64_t* __restrict__ limbs, uint32_t nlimbs,
+__device__ __forceinline__ void pack_dispatch(rf_glimbs __restrict__ limbs, uint32_t nlimbs,
                                               const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
                                               double p, double pinv, double (&out)[NCH]) {
   if constexpr (T == 0) {
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
   const double(*pk)[3] = C->pk[tk.q][tk.scale];
-  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  rf_gtw tw = (rf_gtw)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
   const double sg = tk.sign < 0 ? -1.0 : 1.0;
   const uint32_t nchunk = tk.sign != 0 ? tk.nchunk : 0u;
   double v[S::E];
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {  // a zero entry reads no limbs at all
-          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
+          pack_dispatch<CB, T>((rf_glimbs)tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {
-          pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
+          pack_dispatch<CB, T>((rf_glimbs)tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -554,8 +554,8 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 // d at run time and reduces every eighth term (|term| <= 0.6 p); D == -1
 // (a task with d = 0) copies one stored spectrum, already a product, in.
 template <int LGN, int D>
-__device__ __forceinline__ void pointwise(double* v, int tid, const double* __restrict__ pa,
-                                          const double* __restrict__ pb, uint32_t as, uint32_t bs, int d,
+__device__ __forceinline__ void pointwise(double* v, int tid, rf_gres __restrict__ pa,
+                                          rf_gres __restrict__ pb, uint32_t as, uint32_t bs, int d,
                                           double p, double pinv) {
   using S = Shape<LGN>;
   if constexpr (D < 0) {
@@ -589,8 +589,8 @@ __device__ __forceinline__ void pointwise(double* v, int tid, const double* __re
       double acc[EB];
 #pragma unroll
       for (int e = 0; e < EB; e++) acc[e] = 0.0;
-      const double* qa = pa;
-      const double* qb = pb;
+      rf_gres qa = pa;
+      rf_gres qb = pb;
       for (int l = 0; l < d; l++) {
         double a[EB], b[EB];
 #pragma unroll
@@ -616,8 +616,8 @@ __device__ __forceinline__ void pointwise(double* v, int tid, const double* __re
 // of the thread holds the 2^R0 consecutive residues (g << R0) + j,
 // g = tid + T u, loaded as 16-byte pairs.  D as in pointwise.
 template <int LGN, int D, int R0>
-__device__ __forceinline__ void pointwise_grp(double* v, int tid, const double* __restrict__ pa,
-                                              const double* __restrict__ pb, uint32_t as, uint32_t bs, int d,
+__device__ __forceinline__ void pointwise_grp(double* v, int tid, rf_gres __restrict__ pa,
+                                              rf_gres __restrict__ pb, uint32_t as, uint32_t bs, int d,
                                               double p, double pinv) {
   using S = Shape<LGN>;
   constexpr int GS = 1 << R0, NG = S::E >> R0, PR = GS / 2;  // 16-byte pairs per group
@@ -631,7 +631,7 @@ __device__ __forceinline__ void pointwise_grp(double* v, int tid, const double* 
 #pragma unroll
         for (int k = 0; k < PR; k++) {
           const double2 a =
-              *reinterpret_cast<const double2*>(pa + (size_t)((tid + S::T * (u0 + u)) << R0) + 2 * k);
+              rf_ld(reinterpret_cast<rf_gtw>(pa + (size_t)((tid + S::T * (u0 + u)) << R0) + 2 * k));
           v[(u0 + u) * GS + 2 * k] = a.x;
           v[(u0 + u) * GS + 2 * k + 1] = a.y;
         }
@@ -644,8 +644,8 @@ __device__ __forceinline__ void pointwise_grp(double* v, int tid, const double* 
 #pragma unroll
           for (int k = 0; k < PR; k++) {
             const size_t i = (size_t)((tid + S::T * (u0 + u)) << R0) + 2 * k;
-            a[l][u][k] = *reinterpret_cast<const double2*>(pa + (size_t)l * as + i);
-            b[l][u][k] = *reinterpret_cast<const double2*>(pb + (size_t)l * bs + i);
+            a[l][u][k] = rf_ld(reinterpret_cast<rf_gtw>(pa + (size_t)l * as + i));
+            b[l][u][k] = rf_ld(reinterpret_cast<rf_gtw>(pb + (size_t)l * bs + i));
           }
 #pragma unroll
       for (int u = 0; u < GB; u++)
@@ -674,8 +674,8 @@ __device__ __forceinline__ void pointwise_grp(double* v, int tid, const double* 
 #pragma unroll
           for (int k = 0; k < PR; k++) {
             const size_t i = (size_t)((tid + S::T * (u0 + u)) << R0) + 2 * k;
-            a[u][k] = *reinterpret_cast<const double2*>(pa + (size_t)l * as + i);
-            b[u][k] = *reinterpret_cast<const double2*>(pb + (size_t)l * bs + i);
+            a[u][k] = rf_ld(reinterpret_cast<rf_gtw>(pa + (size_t)l * as + i));
+            b[u][k] = rf_ld(reinterpret_cast<rf_gtw>(pb + (size_t)l * bs + i));
           }
 #pragma unroll
         for (int u = 0; u < GB; u++)
@@ -714,10 +714,10 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
   double* sh = sh_all + sub * S::LDSN;
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = (const double2*)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
+  rf_gtw tw = (rf_gtw)(tb.tw_inv + (size_t)tk.q * RF_INV_TW * 2);
   double v[S::E];
-  const double* pa = (tk.a_base ? tk.a_base : aspec) + tk.a_off;
-  const double* pb = bspec + tk.b_off;
+  const rf_gres pa = (rf_gres)((tk.a_base ? tk.a_base : aspec) + tk.a_off);
+  const rf_gres pb = (rf_gres)(bspec + tk.b_off);
   constexpr int R0 = inv_r0<LGN>();
   if constexpr (R0 <= 2) {
     // the first pass's groups of 2 or 4 consecutive frequencies come
@@ -763,7 +763,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
   for (int u = 0; u < QL::NG; u++)
 #pragma unroll
     for (int j = 0; j < QL::GS; j++)
-      out[inv_index<LGN, NPI - 1>(tid + S::T * u, j)] = rf_reduce(v[u * QL::GS + j], p, pinv);
+      rf_st_nt(out + inv_index<LGN, NPI - 1>(tid + S::T * u, j), rf_reduce(v[u * QL::GS + j], p, pinv));
 }
 
 // ---- twiddle tables, generated on the device at set-up ----
@@ -813,15 +813,15 @@ __global__ __launch_bounds__(256) void k_twiddles(const RfConsts* __restrict__ C
 // The twiddle table of prime q for a transform of 2^lgN points: the set-up
 // table up to 2^RF_LGN_TW, the middle one up to 2^RF_LGN_MID, the long one
 // above (they agree on every index they share).
-__device__ __forceinline__ const double2* fwd_table(const RfDevTables& tb, int q, int lgN) {
-  return lgN > RF_LGN_MID  ? (const double2*)(tb.tw_fwd_long + (size_t)q * RF_FWD_TW_LONG * 2)
-         : lgN > RF_LGN_TW ? (const double2*)(tb.tw_fwd_mid + (size_t)q * RF_FWD_TW_MID * 2)
-                           : (const double2*)(tb.tw_fwd + (size_t)q * RF_FWD_TW * 2);
+__device__ __forceinline__ rf_gtw fwd_table(const RfDevTables& tb, int q, int lgN) {
+  return lgN > RF_LGN_MID  ? (rf_gtw)(tb.tw_fwd_long + (size_t)q * RF_FWD_TW_LONG * 2)
+         : lgN > RF_LGN_TW ? (rf_gtw)(tb.tw_fwd_mid + (size_t)q * RF_FWD_TW_MID * 2)
+                           : (rf_gtw)(tb.tw_fwd + (size_t)q * RF_FWD_TW * 2);
 }
-__device__ __forceinline__ const double2* inv_table(const RfDevTables& tb, int q, int lgN) {
-  return lgN > RF_LGN_MID  ? (const double2*)(tb.tw_inv_long + (size_t)q * RF_INV_TW_LONG * 2)
-         : lgN > RF_LGN_TW ? (const double2*)(tb.tw_inv_mid + (size_t)q * RF_INV_TW_MID * 2)
-                           : (const double2*)(tb.tw_inv + (size_t)q * RF_INV_TW * 2);
+__device__ __forceinline__ rf_gtw inv_table(const RfDevTables& tb, int q, int lgN) {
+  return lgN > RF_LGN_MID  ? (rf_gtw)(tb.tw_inv_long + (size_t)q * RF_INV_TW_LONG * 2)
+         : lgN > RF_LGN_TW ? (rf_gtw)(tb.tw_inv_mid + (size_t)q * RF_INV_TW_MID * 2)
+                           : (rf_gtw)(tb.tw_inv + (size_t)q * RF_INV_TW * 2);
 }
 // Whether the tables a transform of 2^lgN points reads have been made.
 static bool tables_made(const RfDevTables& tb, int lgN) {
@@ -851,7 +851,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
   const double(*pk)[3] = C->pk[tk.q][tk.scale];
-  const double2* tw = (const double2*)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
+  rf_gtw tw = (rf_gtw)(tb.tw_fwd + (size_t)tk.q * RF_FWD_TW * 2);
   const double sg = tk.sign < 0 ? -1.0 : 1.0;
   const uint32_t nchunk = tk.sign != 0 ? tk.nchunk : 0u;
   const bool half = nchunk <= N / 2;  // rows >= ROWS/2 are zero: stage 0 copies
@@ -864,7 +864,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
 #pragma unroll
     for (int c = 0; c < CB; c++) ci[c] = (uint32_t)(r0 + c) * COLS + (uint32_t)col;
     if (nchunk && !(half && r0 >= ROWS / 2)) {
-      pack_dispatch<CB, T>(tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
+      pack_dispatch<CB, T>((rf_glimbs)tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
     } else {
 #pragma unroll
       for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -888,7 +888,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
     const int hf = ROWS >> (st + 1);
 #pragma unroll
     for (int b = 0; b < (1 << st); b++) {
-      const double2 wt = tw[b];
+      const double2 wt = rf_ld(tw + (b));
 #pragma unroll
       for (int jj = 0; jj < hf; jj++) {
         const int j0 = b * 2 * hf + jj, j1 = j0 + hf;
@@ -920,7 +920,7 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
   const RfRowTask tk = rf_wave_uniform_load(tasks + wg);
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = fwd_table(tb, tk.q, tk.lg);
+  rf_gtw tw = fwd_table(tb, tk.q, tk.lg);
   double* a = spec + tk.off;
   double v[S::E];
 #pragma unroll
@@ -953,7 +953,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_mid(RfDevTables tb, c
   const int lo = (th >> RF_LGN_ROW) & ((1 << b0) - 1), hi = th >> (RF_LGN_ROW + b0);
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = fwd_table(tb, tk.q, lgN);
+  rf_gtw tw = fwd_table(tb, tk.q, lgN);
   const size_t stride = (size_t)1 << (RF_LGN_ROW + b0);
   double* a = spec + tk.spec_off + ((((size_t)hi << (b0 + NB)) + lo) << RF_LGN_ROW) + col;
   double v[J];
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_mid(RfDevTables tb, c
     const int hf = J >> (s + 1);
 #pragma unroll
     for (int b = 0; b < (1 << s); b++) {
-      const double2 wt = tw[(hi << s) + b];
+      const double2 wt = rf_ld(tw + ((hi << s) + b));
 #pragma unroll
       for (int jj = 0; jj < hf; jj++) {
         const int j0 = b * 2 * hf + jj, j1 = j0 + hf;
@@ -1000,8 +1000,8 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
   const int lo = (th >> RF_LGN_ROW) & ((1 << b0) - 1), hi = th >> (RF_LGN_ROW + b0);
   const RfConsts* C = tb.consts;
   const double p = C->p[tk.q], pinv = C->pinv[tk.q];
-  const double2* tw = inv_table(tb, tk.q, lgN);
-  double* a = coef + tk.off + ((((size_t)hi << (b0 + NB)) + lo) << RF_LGN_ROW) + col;
+  rf_gtw tw = inv_table(tb, tk.q, lgN);
+  double RF_GLOBAL* a = (double RF_GLOBAL*)(coef + tk.off + ((((size_t)hi << (b0 + NB)) + lo) << RF_LGN_ROW) + col);
   const size_t stride = (size_t)1 << (RF_LGN_ROW + b0);
   const int low = (lo << RF_LGN_ROW) + col;
   double v[J];
@@ -1013,7 +1013,7 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
     const int L = RF_LGN_ROW + b0 + st;
 #pragma unroll
     for (int jj = 0; jj < hf; jj++) {
-      const double2 wt = tw[(1 << L) + (jj << (RF_LGN_ROW + b0)) + low];
+      const double2 wt = rf_ld(tw + ((1 << L) + (jj << (RF_LGN_ROW + b0)) + low));
 #pragma unroll
       for (int b = 0; b < (J >> (st + 1)); b++) {
         const int j0 = b * 2 * hf + jj, j1 = j0 + hf;
