@@ -34,28 +34,30 @@ __device__ __forceinline__ bool fn_constant(uint32_t f) { return (f & 15u) == ((
 
 // Garner's algorithm on residues res[j * N] (|x| <= p_j/2 + 8) into the
 // S-limb two's complement of the representative in (-P/2, P/2)
-// (src/lib/crt.c:11-94 states the same CRT on 62-bit primes, as nested
-// Horner steps).  Mixed-radix digits v_j in [0, p_j) with
-// X = v_0 + p_0 (v_1 + p_1 (v_2 + ...)): y_m starts as x_m, and once v_j is
-// known every later y_m <- (y_m - v_j) p_j^-1 mod p_m, which leaves y_m the
-// residue modulo p_m of (X - v_0 - ... - v_j p_0...p_{j-1}) / (p_0...p_j), so
-// v_{j+1} = y_{j+1} mod p_{j+1}.  The updates of one step are independent of
-// each other (the nested form is one chain of dependent products).
-// Bounds: primes below 2^47.9, so |y_m - v_j| < 0.625 p_m + p_j < 2^49 and
-// the product is at most p_m (1/2 + 2^49 2^-52) = 0.625 p_m; starting from
-// |x_m| <= p_m/2 + 8 every y_m stays within 0.625 p_m < p_m, the range
-// rf_canon1 takes.
+// (src/lib/crt.c:11-94 states the same CRT on 62-bit primes).
+// Mixed-radix digits v_j in [0, p_j): v_j = (x_j - [v_0 + p_0 (v_1 + ...)]) /
+// (p_0 ... p_{j-1}) mod p_j; all fp64 values stay below 2^50.
 template <int K, int S>
 __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t N,
                                        const RfConsts* __restrict__ C, uint64_t (&X)[S]) {
-  double y[K], v[K];
-#pragma unroll
-  for (int j = 0; j < K; j++) y[j] = res[(size_t)j * N];
+  double v[K];
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    v[j] = rf_canon1(y[j], C->p[j], C->pinv[j]);
+    const double p = C->p[j];
+    const double x = res[(size_t)j * N];
+    double r;
+    if (j == 0) {
+      r = x;
+    } else {
+      double acc = v[j - 1];
 #pragma unroll
-    for (int m = j + 1; m < K; m++) y[m] = rf_mulmod(y[m] - v[j], C->gi[m][j][0], C->gi[m][j][1], C->p[m]);
+      for (int l = j - 2; l >= 0; l--) acc = rf_mulmod(acc, C->pm[j][l][0], C->pm[j][l][1], p) + v[l];
+      r = rf_mulmod(x - acc, C->ci[j][0], C->ci[j][1], p);
+    }
+    // |r| < p: x is reduced (|x| <= p/2 + 8); acc, a product plus a
+    // canonical digit, stays below 1.5 p + 8, so |x - acc| < 2^50 and the
+    // product r satisfies |r| <= p (1/2 + 2^50 2^-52) = 0.75 p
+    v[j] = rf_canon1(r, p, C->pinv[j]);
   }
   // Mixed radix to binary: X = v_{K-1}; X = X p_j + v_j for j = K-2 .. 0, in
   // 32-bit limbs.  p_j = phi 2^32 + plo with phi < 2^16 and, as p_j = 1 mod

@@ -612,15 +612,6 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
     for (int e = 0; e < r * d; e++)
       crt_tasks(pl, P.coef_off + (uint64_t)e * k * N, P.out[e], npre, nblk_total, crt[k]);
   }
-  // Inverse tasks prime by prime: an XCD runs a contiguous range of a
-  // launch's tasks, so its L2 then holds the inverse twiddle tables of one
-  // or two primes instead of all of them (the tables, 128 KiB per prime at
-  // 2^13 points, were otherwise evicted by the streaming spectra and
-  // fetched again).  The outputs of one product at one prime stay adjacent,
-  // so the spectra they share are still read while in L2.
-  for (auto& kv : inv)
-    std::stable_sort(kv.second.begin(), kv.second.end(),
-                     [](const RfInvTask& a, const RfInvTask& b) { return a.q < b.q; });
   // Stream order: every forward pass (column passes before the row pass),
   // then every inverse pass (row passes before the column passes), CRT,
   // carry application.

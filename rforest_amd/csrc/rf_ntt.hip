@@ -166,16 +166,76 @@ __device__ __forceinline__ int inv_index(int g, int j) {
   return (hi << (Q::L0 + Q::R)) | (j << Q::L0) | lo;
 }
 
+// Twiddles derived instead of loaded.  In a pass of R stages a group needs
+// 2^s twiddles at stage s; all but one of them are products of the group's
+// first one and a constant shared by every lane:
+//   forward  T[H 2^s + b] = T[H 2^s] T[b]           (b < 2^s: the reversal of
+//            H 2^s + b is the sum of the reversals of its two disjoint parts)
+//   inverse  V[2^L + jj 2^L0 + lo] = V[2^L + lo] V[2^s + jj]
+//            (omega_{2^(L+1)}^-(jj 2^L0 + lo), L = L0 + s, jj < 2^s)
+// so a group loads R twiddles per pass and forms the other 2^R - 1 - R as
+// exact modular products.  A derived twiddle w = rf_mulmod(base, c) is not
+// canonical but |w| <= p (1/2 + p 2^-52) < 0.56 p, and its quotient
+// wq = fl(w pinv) (two roundings) is within 0.6 2^-52 (1 + 2^-53) of w / p;
+// fl(a wq) is then within |a| 0.9 2^-52 of a w / p, inside the |a| 2^-52
+// that rf_mulmod's bound assumes for a canonical w, so every residue bound
+// of the transforms holds unchanged.
+#ifndef RF_DERIVE_FWD
+#define RF_DERIVE_FWD 1
+#endif
+#ifndef RF_DERIVE_INV
+#define RF_DERIVE_INV 1
+#endif
+#ifndef RF_DERIVE_INV_L0
+#define RF_DERIVE_INV_L0 1  // inverse passes starting at stage >= this derive
+#endif
+#ifndef RF_FWD_BASES_R
+#define RF_FWD_BASES_R 2  // forward passes of >= this many stages load bases early
+#endif
+// Shorter transforms keep loading every twiddle and storing through the
+// caches: their tables stay in L1, and a batch of them often keeps its
+// spectra in the Infinity Cache between the forward and inverse launches.
+#ifndef RF_BIG_LGN
+#define RF_BIG_LGN 13
+#endif
+__device__ __forceinline__ double2 derive_tw(double2 base, double2 c, double p, double pinv) {
+  const double w = rf_mulmod(base.x, c.x, c.y, p);
+  return make_double2(w, w * pinv);
+}
+
 // R forward stages on NG groups held in v[u * 2^R + j].  At stage L0 + s the
 // pair (j, j + 2^(R-1-s)) of group (H, lo) uses twiddle T[H 2^s + (j >> (R-s))]
 // (T[m] = omega^(rev(m)), shared prefix for all lengths).  A row of a longer
 // transform (ROW) prefixes its row index hb to the block index:
-// T[(hb 2^L0 + H) 2^s + (j >> (R-s))].
-template <int LGN, int P, int S0 = 0, bool ROW = false>
-__device__ __forceinline__ void fwd_pass(double* v, int tid, rf_gtw __restrict__ tw,
-                                         double p, int hb = 0) {
+// T[(hb 2^L0 + H) 2^s + (j >> (R-s))].  Passes after the first (and rows)
+// derive the twiddles with b > 0 from T[H 2^s] and the uniform T[b].
+template <int LGN, int P, bool ROW = false>
+constexpr bool fwd_derive() {
+  return RF_DERIVE_FWD && LGN >= RF_BIG_LGN && (ROW || P > 0) && Pass<LGN, P>::R >= RF_FWD_BASES_R;
+}
+// The R base twiddles T[H 2^s] of each of the thread's groups in forward
+// pass P (loaded before the exchange into the pass, so that they arrive
+// while it runs).
+template <int LGN, int P, bool ROW = false>
+__device__ __forceinline__ void fwd_bases(double2* bs, int tid, rf_gtw __restrict__ tw, int hb = 0) {
   using Q = Pass<LGN, P>;
   constexpr int NLO = LGN - Q::L0 - Q::R;
+  if constexpr (fwd_derive<LGN, P, ROW>()) {
+#pragma unroll
+    for (int u = 0; u < Q::NG; u++) {
+      const int H = (ROW ? (hb << Q::L0) : 0) + (fwd_group<LGN, P>(tid, u) >> NLO);
+#pragma unroll
+      for (int s = 0; s < Q::R; s++) bs[u * Q::R + s] = rf_ld(tw + (H << s));
+    }
+  }
+}
+
+template <int LGN, int P, int S0 = 0, bool ROW = false>
+__device__ __forceinline__ void fwd_pass(double* v, int tid, rf_gtw __restrict__ tw,
+                                         double p, double pinv, const double2* bs, int hb = 0) {
+  using Q = Pass<LGN, P>;
+  constexpr int NLO = LGN - Q::L0 - Q::R;
+  constexpr bool DERIVE = fwd_derive<LGN, P, ROW>();
 #pragma unroll
   for (int s = S0; s < Q::R; s++) {
     const int half = 1 << (Q::R - 1 - s);
@@ -188,7 +248,13 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, rf_gtw __restrict__
         // in pass 0 of a whole transform (H = 0) block 0 uses T[0] = 1: no
         // multiplication
         const bool one = !ROW && (P == 0) && (b == 0);
-        const double2 w = one ? make_double2(1.0, 0.0) : rf_ld(tw + ((H << s) + b));
+        double2 w;
+        if (one)
+          w = make_double2(1.0, 0.0);
+        else if constexpr (DERIVE)
+          w = b == 0 ? bs[u * Q::R + s] : derive_tw(bs[u * Q::R + s], rf_ld(tw + b), p, pinv);
+        else
+          w = rf_ld(tw + ((H << s) + b));
 #pragma unroll
         for (int jj = 0; jj < half; jj++) {
           const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
@@ -204,10 +270,33 @@ __device__ __forceinline__ void fwd_pass(double* v, int tid, rf_gtw __restrict__
 
 // R inverse (decimation-in-time) stages: at stage L = L0 + s the pair
 // (j, j + 2^s) uses V[2^L + (j mod 2^s) 2^L0 + lo] = omega_{2^(L+1)}^-(i mod 2^L).
+// Passes whose groups span at least 32 distinct lo per wave derive the
+// twiddles with jj > 0 from V[2^L + lo] and the uniform V[2^s + jj].
+template <int LGN, int P, int R0 = inv_r0<LGN>()>
+constexpr bool inv_derive() {
+  return RF_DERIVE_INV && LGN >= RF_BIG_LGN && P > 0 && Pass<LGN, P, R0>::L0 >= RF_DERIVE_INV_L0 &&
+         Pass<LGN, P, R0>::R >= 2;
+}
+// The R base twiddles V[2^(L0+s) + lo] of each of the thread's groups in
+// inverse pass P (loaded before the exchange into the pass).
+template <int LGN, int P, int R0 = inv_r0<LGN>()>
+__device__ __forceinline__ void inv_bases(double2* bs, int tid, rf_gtw __restrict__ tw) {
+  using Q = Pass<LGN, P, R0>;
+  if constexpr (inv_derive<LGN, P, R0>()) {
+#pragma unroll
+    for (int u = 0; u < Q::NG; u++) {
+      const int lo = (tid + Shape<LGN>::T * u) & ((1 << Q::L0) - 1);
+#pragma unroll
+      for (int s = 0; s < Q::R; s++) bs[u * Q::R + s] = rf_ld(tw + ((1 << (Q::L0 + s)) + lo));
+    }
+  }
+}
+
 template <int LGN, int P, int R0 = inv_r0<LGN>()>
 __device__ __forceinline__ void inv_pass(double* v, int tid, rf_gtw __restrict__ tw,
-                                         double p) {
+                                         double p, double pinv, const double2* bs) {
   using Q = Pass<LGN, P, R0>;
+  constexpr bool DERIVE = inv_derive<LGN, P, R0>();
 #pragma unroll
   for (int s = 0; s < Q::R; s++) {
     const int half = 1 << s;
@@ -220,7 +309,13 @@ __device__ __forceinline__ void inv_pass(double* v, int tid, rf_gtw __restrict__
       for (int jj = 0; jj < half; jj++) {
         // in pass 0 (lo = 0) the pair with jj = 0 uses V[len] = 1
         const bool one = (P == 0) && (jj == 0);
-        const double2 w = one ? make_double2(1.0, 0.0) : rf_ld(tw + (len + (jj << Q::L0) + lo));
+        double2 w;
+        if (one)
+          w = make_double2(1.0, 0.0);
+        else if constexpr (DERIVE)
+          w = jj == 0 ? bs[u * Q::R + s] : derive_tw(bs[u * Q::R + s], rf_ld(tw + (half + jj)), p, pinv);
+        else
+          w = rf_ld(tw + (len + (jj << Q::L0) + lo));
 #pragma unroll
         for (int b = 0; b < (Q::GS >> (s + 1)); b++) {
           const int j0 = u * Q::GS + b * 2 * half + jj, j1 = j0 + half;
@@ -240,12 +335,31 @@ __device__ __forceinline__ void reduce_all(double* v, double p, double pinv) {
   for (int e = 0; e < E; e++) v[e] = rf_reduce(v[e], p, pinv);
 }
 
+// Stores of a transform's results, non-temporal when the launch writes more
+// than the Infinity Cache holds (nt is uniform: the launcher sets it from
+// the launch's output bytes).  Smaller launches keep their results cached
+// for the next launch (the inverse, or a column pass that reads a row
+// pass's output straight back).
+#define RF_NT_BYTES ((size_t)256 << 20)
+__device__ __forceinline__ void st1(bool nt, double* p, double x) {
+  if (nt)
+    rf_st_nt(p, x);
+  else
+    *p = x;
+}
+__device__ __forceinline__ void st2(bool nt, double* p, double x, double y) {
+  if (nt)
+    rf_st_nt2(p, x, y);
+  else
+    *reinterpret_cast<double2*>(p) = make_double2(x, y);
+}
+
 // Reduced residues of the last forward pass to out (in place order).  When
 // its groups are consecutive residues (no index bits below its stages) they
 // go out as 16-byte pairs.
 template <int LGN>
 __device__ __forceinline__ void fwd_store(double* __restrict__ out, const double* v, int tid, double p,
-                                          double pinv) {
+                                          double pinv, bool nt) {
   using S = Shape<LGN>;
   using QL = Pass<LGN, S::NP - 1>;
   constexpr int NLO = LGN - QL::L0 - QL::R;
@@ -254,12 +368,12 @@ __device__ __forceinline__ void fwd_store(double* __restrict__ out, const double
     if constexpr (NLO == 0 && QL::GS >= 2) {
 #pragma unroll
       for (int j = 0; j < QL::GS; j += 2)
-        rf_st_nt2(out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j),
-                  rf_reduce(v[u * QL::GS + j], p, pinv), rf_reduce(v[u * QL::GS + j + 1], p, pinv));
+        st2(nt, out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j),
+                               rf_reduce(v[u * QL::GS + j], p, pinv), rf_reduce(v[u * QL::GS + j + 1], p, pinv));
     } else {
 #pragma unroll
       for (int j = 0; j < QL::GS; j++)
-        rf_st_nt(out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j), rf_reduce(v[u * QL::GS + j], p, pinv));
+        st1(nt, out + fwd_index<LGN, S::NP - 1>(fwd_group<LGN, S::NP - 1>(tid, u), j), rf_reduce(v[u * QL::GS + j], p, pinv));
     }
   }
 }
@@ -272,6 +386,8 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, rf_gtw 
   if constexpr (P < Shape<LGN>::NP) {
     using Q = Pass<LGN, P>;
     using Qp = Pass<LGN, P - 1>;
+    double2 bs[fwd_derive<LGN, P, ROW>() ? Q::NG * Q::R : 1];
+    fwd_bases<LGN, P, ROW>(bs, tid, tw, hb);
     if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
     // lanes run over consecutive residues on both sides when both passes
     // keep at least five low index bits below their stages
@@ -290,7 +406,7 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, rf_gtw 
       for (int j = 0; j < Q::GS; j++)
         v[u * Q::GS + j] = sh[lds_slot<SW>(fwd_index<LGN, P>(fwd_group<LGN, P>(tid, u), j))];
     xsync<fwd_local<LGN, P>()>();
-    fwd_pass<LGN, P, 0, ROW>(v, tid, tw, p, hb);
+    fwd_pass<LGN, P, 0, ROW>(v, tid, tw, p, pinv, bs, hb);
     fwd_rest<LGN, P + 1, ROW>(v, sh, tid, tw, p, pinv, hb);
   }
 }
@@ -302,6 +418,8 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, rf_gtw 
   if constexpr (P < npasses<LGN, R0>()) {
     using Q = Pass<LGN, P, R0>;
     using Qp = Pass<LGN, P - 1, R0>;
+    double2 bs[inv_derive<LGN, P>() ? Q::NG * Q::R : 1];
+    inv_bases<LGN, P>(bs, tid, tw);
     if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
     constexpr int SW = (Shape<LGN>::T >= 64 && Qp::L0 >= 5 && Q::L0 >= 5) ? 0 : 1;
 #pragma unroll
@@ -316,7 +434,7 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, rf_gtw 
       for (int j = 0; j < Q::GS; j++)
         v[u * Q::GS + j] = sh[lds_slot<SW>(inv_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
     __syncthreads();
-    inv_pass<LGN, P>(v, tid, tw, p);
+    inv_pass<LGN, P>(v, tid, tw, p, pinv, bs);
     inv_rest<LGN, P + 1>(v, sh, tid, tw, p, pinv);
   }
 }
@@ -478,7 +596,7 @@ __device__ __forceinline__ void pack_dispatch(rf_glimbs __restrict__ limbs, uint
 template <int LGN, int T>
 __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(RfDevTables tb,
                                                              const RfFwdTask* __restrict__ tasks,
-                                                             int ntasks, double* __restrict__ spec) {
+                                                             int ntasks, double* __restrict__ spec, int nt) {
   using S = Shape<LGN>;
   using Q0 = Pass<LGN, 0>;
   extern __shared__ double sh_all[];
@@ -522,7 +640,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
           v[u * Q0::GS + j0 + c + Q0::GS / 2] = y;
         }
       }
-    fwd_pass<LGN, 0, 1>(v, tid, tw, p);
+    fwd_pass<LGN, 0, 1>(v, tid, tw, p, pinv, nullptr);
   } else {
 #pragma unroll
     for (int u = 0; u < Q0::NG; u++)
@@ -541,10 +659,10 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 #pragma unroll
         for (int c = 0; c < CB; c++) v[u * Q0::GS + j0 + c] = ci[c] < nchunk ? sg * x[c] : 0.0;
       }
-    fwd_pass<LGN, 0, 0>(v, tid, tw, p);
+    fwd_pass<LGN, 0, 0>(v, tid, tw, p, pinv, nullptr);
   }
   fwd_rest<LGN, 1>(v, sh, tid, tw, p, pinv);
-  fwd_store<LGN>(spec + tk.spec_off, v, tid, p, pinv);
+  fwd_store<LGN>(spec + tk.spec_off, v, tid, p, pinv, nt != 0);
 }
 
 // Pointwise product sum_l a_l b_l of the thread's E frequencies
@@ -702,7 +820,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
                                                              const RfInvTask* __restrict__ tasks,
                                                              int ntasks, const double* __restrict__ aspec,
                                                              const double* __restrict__ bspec,
-                                                             double* __restrict__ coef) {
+                                                             double* __restrict__ coef, int nt) {
   using S = Shape<LGN>;
   extern __shared__ double sh_all[];
   const int sub = threadIdx.x / S::T, tid = threadIdx.x % S::T;
@@ -754,7 +872,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
       __syncthreads();
     }
   }
-  inv_pass<LGN, 0>(v, tid, tw, p);
+  inv_pass<LGN, 0>(v, tid, tw, p, pinv, nullptr);
   inv_rest<LGN, 1>(v, sh, tid, tw, p, pinv);
   constexpr int NPI = npasses<LGN, R0>();
   using QL = Pass<LGN, NPI - 1, R0>;
@@ -763,7 +881,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_inv(R
   for (int u = 0; u < QL::NG; u++)
 #pragma unroll
     for (int j = 0; j < QL::GS; j++)
-      rf_st_nt(out + inv_index<LGN, NPI - 1>(tid + S::T * u, j), rf_reduce(v[u * QL::GS + j], p, pinv));
+      st1(nt, out + inv_index<LGN, NPI - 1>(tid + S::T * u, j), rf_reduce(v[u * QL::GS + j], p, pinv));
 }
 
 // ---- twiddle tables, generated on the device at set-up ----
@@ -908,7 +1026,8 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_fwd_col(RfDevTables tb,
 // prefixed to every twiddle index.  Output in place (bit-reversed overall).
 __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTables tb,
                                                                          const RfRowTask* __restrict__ tasks,
-                                                                         int ntasks, double* __restrict__ spec) {
+                                                                         int ntasks, double* __restrict__ spec,
+                                                                         int nt) {
   constexpr int LGN = RF_LGN_ROW;
   using S = Shape<LGN>;
   using Q0 = Pass<LGN, 0>;
@@ -923,13 +1042,15 @@ __global__ __launch_bounds__(Shape<RF_LGN_ROW>::THREADS) void k_fwd_row(RfDevTab
   rf_gtw tw = fwd_table(tb, tk.q, tk.lg);
   double* a = spec + tk.off;
   double v[S::E];
+  double2 bs[fwd_derive<LGN, 0, true>() ? Q0::NG * Q0::R : 1];
+  fwd_bases<LGN, 0, true>(bs, tid, tw, (int)tk.row);
 #pragma unroll
   for (int u = 0; u < Q0::NG; u++)
 #pragma unroll
     for (int j = 0; j < Q0::GS; j++) v[u * Q0::GS + j] = a[fwd_index<LGN, 0>(tid + S::T * u, j)];
-  fwd_pass<LGN, 0, 0, true>(v, tid, tw, p, (int)tk.row);
+  fwd_pass<LGN, 0, 0, true>(v, tid, tw, p, pinv, bs, (int)tk.row);
   fwd_rest<LGN, 1, true>(v, sh_all, tid, tw, p, pinv, (int)tk.row);
-  fwd_store<LGN>(a, v, tid, p, pinv);
+  fwd_store<LGN>(a, v, tid, p, pinv, nt != 0);
 }
 
 // Forward middle pass of transforms above 2^(13+4) points: after the passes
@@ -1037,7 +1158,8 @@ hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntask
   const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_fwd<LGN, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_fwd<LGN, T>), dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec);
+  const int nt = (size_t)ntasks * S::N * sizeof(double) > RF_NT_BYTES;
+  hipLaunchKernelGGL((k_fwd<LGN, T>), dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec, nt);
   return hipGetLastError();
 }
 
@@ -1071,8 +1193,9 @@ hipError_t launch_inv_t(const RfDevTables& tb, const RfInvTask* tasks, int ntask
   const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_inv<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const int nt = (size_t)ntasks * S::N * sizeof(double) > RF_NT_BYTES;
   hipLaunchKernelGGL(k_inv<LGN>, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, aspec, bspec,
-                     coef);
+                     coef, nt);
   return hipGetLastError();
 }
 
@@ -1164,8 +1287,9 @@ hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int 
   if (ntasks <= 0) return hipSuccess;
   using S = Shape<RF_LGN_ROW>;
   const int grid = (ntasks + 7) & ~7;
+  const int nt = (size_t)ntasks * S::N * sizeof(double) > RF_NT_BYTES;
   hipLaunchKernelGGL(k_fwd_row, dim3(grid), dim3(S::THREADS), S::LDSN * sizeof(double), st, tb, tasks,
-                     ntasks, spec);
+                     ntasks, spec, nt);
   return hipGetLastError();
 }
 

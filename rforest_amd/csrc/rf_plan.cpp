@@ -270,11 +270,16 @@ void rf_consts_init(const RfPrimes* pr, RfConsts* c) {
     c->pu[j] = p;
     c->p[j] = (double)p;
     c->pinv[j] = 1.0 / (double)p;
+    uint64_t prod = 1;
     for (int l = 0; l < j; l++) {
-      const uint64_t inv = rf_h_invmod(pr->p[l] % p, p);
-      c->gi[j][l][0] = (double)inv;
-      c->gi[j][l][1] = (double)inv / (double)p;
+      uint64_t v = pr->p[l] % p;
+      c->pm[j][l][0] = (double)v;
+      c->pm[j][l][1] = (double)v / (double)p;
+      prod = rf_h_mulmod(prod, v, p);
     }
+    uint64_t inv = rf_h_invmod(prod, p);
+    c->ci[j][0] = (double)inv;
+    c->ci[j][1] = (double)inv / (double)p;
     for (int v = 0; v <= RF_LGN_MAX; v++) {
       uint64_t S = v == 0 ? 1 : rf_h_invmod(((uint64_t)1 << v) % p, p);
       for (int w = 0; w < RF_MAXWORDS; w++) {

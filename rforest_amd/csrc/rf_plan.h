@@ -102,8 +102,10 @@ int rf_plan_cyclic(const RfPrimes* pr, uint64_t min_limbs, uint64_t a_limbs, uin
 struct RfConsts {
   double p[RF_MAXP];
   double pinv[RF_MAXP];
-  // Garner: gi[m][j] = (p_j^-1 mod p_m, fl(that / p_m)) for j < m.
-  double gi[RF_MAXP][RF_MAXP][2];
+  // Garner: pm[j][l] = (p_l mod p_j, fl(that / p_j)) for l < j;
+  // ci[j] = ((p_0...p_{j-1})^-1 mod p_j, fl(. / p_j)).
+  double pm[RF_MAXP][RF_MAXP][2];
+  double ci[RF_MAXP][2];
   uint64_t pu[RF_MAXP];                  // the primes as integers
   uint64_t P[RF_MAXP + 1][RF_MAXS];      // P_k = p_0 ... p_{k-1}
   // pack: pk[q][v][w] = the three 16-bit pieces k0, k1, k2 of
