@@ -23,6 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# fp64 VALU: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz; v_fma_f64, v_mul_f64,
+# v_add_f64, v_rndne_f64 and v_mad_u64_u32 all issue at this full rate on
+# gfx950 (78.6 TFLOP/s counting an FMA as two flops).
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 WORKLOAD = "2x2 . 2x2 exact integer matrix product, 1,048,576-bit unsigned entries (top bit set), GMP seed 42"
 
 
@@ -45,10 +49,10 @@ def modcheck(products, results, q):
     return bad
 
 
-def traffic_from_profiles(kernel, batch, plan):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (profiles/pmc_traffic.json), if it was measured for this batch
-    and plan; otherwise None."""
+def traffic_from_profiles(kernel, batch, plan, field="hbm_bytes_per_launch"):
+    """HBM bytes (or another per-launch counter figure) of `kernel` from the
+    committed rocprofv3 --pmc summary (profiles/pmc_traffic.json), if it was
+    measured for this batch and plan; otherwise None."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
@@ -58,7 +62,7 @@ def traffic_from_profiles(kernel, batch, plan):
     if rec.get("batch") != batch or list(rec.get("plan", [])) != list(plan):
         return None
     k = rec.get("kernels", {}).get(kernel)
-    return None if k is None else k.get("hbm_bytes_per_launch")
+    return None if k is None else k.get(field)
 
 
 def cpu_baseline(products, gpu_results, budget_s):
@@ -698,11 +702,23 @@ def main():
     inv_bytes = args.batch * 4 * k * per_transform
     alg = {"fwd": fwd_bytes, "inv": inv_bytes}
     kern_names = {"fwd": "k_fwd (pack + forward NTT)", "inv": "k_inv (pointwise + inverse NTT)"}
+    # algorithmic fp64 VALU operations: 8 per radix-2 butterfly (a modular
+    # product is 6: two products and a rounding for the quotient, one FMA, the
+    # error term's add; then the sum and the difference), (N/2) lgN
+    # butterflies per transform and prime; the pack 8 per 64-bit limb and
+    # prime (two conversions, six FMAs); the pointwise product 7 per term
+    # (a modular product of two data operands)
+    bfly_ops = 8 * (N // 2) * lgN
+    alg_ops = {"fwd": args.batch * (4 + 4) * k * (bfly_ops + 8 * (1 << 20) // 64),
+               "inv": args.batch * 4 * k * (bfly_ops + 7 * 2 * N)}
     dom = max(("fwd", "inv"), key=lambda s: prof[s][0])
     ms, nl = prof[dom]
     avg_s = ms / nl / 1e3
     achieved = alg[dom] / avg_s / 1e9
+    valu_achieved = alg_ops[dom] / avg_s / 1e12
     traffic = traffic_from_profiles(dom, args.batch, plan)
+    valu_insts = traffic_from_profiles(dom, args.batch, plan, "valu_wave_insts_per_launch")
+    valu_busy = None if valu_insts is None else round(valu_insts * 64 / (VALU_PEAK_TOPS * 1e12) / avg_s, 4)
     ntt_ms = prof["fwd"][0] + prof["inv"][0]
     ntt_launch_s = ntt_ms / psteps / 1e3
     ntt_ach = (fwd_bytes + inv_bytes) / ntt_launch_s / 1e9
@@ -754,9 +770,16 @@ def main():
                    "entry_bits": 1 << 20, "chunk_bits": 64 * t, "ntt_length": N, "primes": k,
                    "prime_bits": 48, "parallelism": "single GPU",
                    "lanes": args.lanes},
-        "roofline": {"bound": "hbm", "kernel": kern_names[dom], "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "alg_bytes_per_launch": alg[dom],
+        # the dominant kernel issues fp64 VALU work at about half the VALU
+        # peak and moves about half the HBM peak; its counters show the VALU
+        # busy and the rest waiting on loads, so the binding roof named is
+        # the VALU's, with the HBM figures beside it
+        "roofline": {"bound": "valu", "kernel": kern_names[dom], "achieved": round(valu_achieved, 3),
+                     "peak": round(VALU_PEAK_TOPS, 3), "unit": "Tops/s",
+                     "frac": round(valu_achieved / VALU_PEAK_TOPS, 4), "traffic": traffic,
+                     "alg_ops_per_launch": alg_ops[dom], "valu_busy": valu_busy,
+                     "hbm": {"achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": alg[dom]},
                      "avg_launch_ms": round(avg_s * 1e3, 5)},
         "roofline_ntt_passes": {"achieved": round(ntt_ach, 2), "frac": round(ntt_ach / HBM_PEAK_GBS, 4),
                                 "alg_bytes_per_product": (fwd_bytes + inv_bytes) // args.batch,
