@@ -646,10 +646,14 @@ ScratchPtrs scratch_ptrs(char* base, const Layout& L) {
 }
 
 int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof) {
+  return launch_pipeline_tasks(pp, base, scratch_ptrs(base, pp.L).tasks, st, prof);
+}
+
+int launch_pipeline_tasks(const Pipeline& pp, char* base, const char* tasks, hipStream_t st, bool prof) {
   const ScratchPtrs S = scratch_ptrs(base, pp.L);
   for (const LaunchRec& l : pp.launches) {
     ProfScope ps(prof ? stage_of(l.kind) : -1, st);
-    const char* t = S.tasks + l.off;
+    const char* t = tasks + l.off;
     switch (l.kind) {
       case K_FWD:  // param: fwd_key
         HIPCHK(rf_launch_fwd(tables(), l.param >> 4, l.param & 15, (const RfFwdTask*)t, l.n, S.spec, st));

@@ -145,6 +145,10 @@ ScratchPtrs scratch_ptrs(char* base, const Layout& L);
 // Enqueues every kernel of a built pipeline whose task image already sits in
 // the scratch region at `base`.  No host synchronisation, no allocation.
 int launch_pipeline(const Pipeline& pp, char* base, hipStream_t st, bool prof);
+// The same with the task image at `tasks` (device) instead of inside the
+// scratch region, so that several built pipelines can share one scratch
+// region and be enqueued back to back.
+int launch_pipeline_tasks(const Pipeline& pp, char* base, const char* tasks, hipStream_t st, bool prof);
 uint64_t bitlen_limbs(const uint64_t* x, uint64_t n);
 // CRT blocks of one C entry, and its CRT tasks (coefficient residues at
 // coef_off doubles, k primes x N, into out): pre bytes and aggregate slots

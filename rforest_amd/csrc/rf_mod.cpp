@@ -201,7 +201,7 @@ struct ModPrep {
   uint64_t n = 0, m = 0, np = 0, il = 0;
   uint32_t s = 0, o = 0;
   std::vector<uint64_t> c;           // size chain: c[0] = n', c[j+1] = c[j] - floor((c[j] - 1) / 2)
-  std::vector<uint64_t> dnorm;       // D'' (n' limbs)
+  std::vector<uint64_t> dtop;        // the top c.back() limbs of D'' (D'' itself is formed on the device)
   std::vector<uint64_t> xbase;       // X of the base level (c.back() + 1 limbs)
   // arena offsets in limbs
   uint64_t dn_off = 0, t_off = 0, tm_off = 0, u_off = 0, i_off = 0, tn = 0, un = 0;
@@ -219,18 +219,22 @@ void mod_prep(uint64_t max_bits, const mpz_t d, ModPrep& P) {
   P.il = P.m - P.n + 2;
   P.c.assign(1, P.np);
   while (P.c.back() > kRecipBase) P.c.push_back(P.c.back() - (P.c.back() - 1) / 2);
+  // base: X = floor((B^(2 c) - 1) / A) for the top c limbs A of
   // D'' = (D << s) B^(n'-n)
-  P.dnorm.assign(P.np, 0);
-  const uint64_t* dl = (const uint64_t*)d->_mp_d;
-  for (uint64_t i = 0; i < P.n; i++)
-    P.dnorm[P.np - P.n + i] = P.s ? (dl[i] << P.s) | (i ? dl[i - 1] >> (64 - P.s) : 0) : dl[i];
-  // base: X = floor((B^(2 c) - 1) / A) for the top c limbs A of D''
   const uint64_t cb = P.c.back();
+  P.dtop.assign(cb, 0);
+  const uint64_t* dl = (const uint64_t*)d->_mp_d;
+  for (uint64_t q = 0; q < cb; q++) {
+    const uint64_t j = P.np - cb + q;  // limb of D''
+    if (j < P.np - P.n) continue;
+    const uint64_t i = j - (P.np - P.n);
+    P.dtop[q] = P.s ? (dl[i] << P.s) | (i ? dl[i - 1] >> (64 - P.s) : 0) : dl[i];
+  }
   mpz_t A, num, X;
   mpz_init(A);
   mpz_init(num);
   mpz_init(X);
-  mpz_import(A, cb, -1, 8, 0, 0, P.dnorm.data() + (P.np - cb));
+  mpz_import(A, cb, -1, 8, 0, 0, P.dtop.data());
   mpz_setbit(num, 128 * cb);
   mpz_sub_ui(num, num, 1);
   mpz_tdiv_q(X, num, A);
@@ -276,21 +280,23 @@ int mod_create_batch_locked(const mpz_t* ds, std::vector<ModPrep>& prep, rf_mod*
       who[j].push_back(i);
     }
   }
-  // arena layout (limbs): D'' and the base X's (uploaded), X levels, T, Tm,
-  // U, I per modulus; then task tables; then the pipelines' scratch
+  // arena layout (limbs): the base X's (uploaded), D'' (formed on the device
+  // from D), X levels, T, Tm, U, I per modulus; then the reciprocal and
+  // normalisation task tables, every level's pipeline task images, and the
+  // pipelines' shared scratch
   uint64_t lim = 0;
   auto take = [&](uint64_t n) {
     const uint64_t o = lim;
     lim += (n + 31) & ~(uint64_t)31;  // 256-byte steps
     return o;
   };
-  for (auto& P : prep) P.dn_off = take(P.np);
   for (auto& P : prep) {
     P.x_off.assign(P.c.size(), 0);
     P.x_off.back() = take(P.c.back() + 1);
   }
   const uint64_t up_limbs = lim;  // uploaded region
   for (auto& P : prep) {
+    P.dn_off = take(P.np);
     for (size_t j = 0; j + 1 < P.c.size(); j++) P.x_off[j] = take(P.c[j] + 1);
     P.t_off = take(std::max<uint64_t>(P.tn, 1));
     P.tm_off = take(P.c.size() > 1 ? P.c[1] + 1 : 1);
@@ -299,20 +305,33 @@ int mod_create_batch_locked(const mpz_t* ds, std::vector<ModPrep>& prep, rf_mod*
   }
   size_t nrt = 0;
   for (size_t j = 0; j < levels; j++) nrt += who[j].size();
-  const size_t rt_b = align256(nrt * sizeof(RfRecipTask)) + align256(cnt * sizeof(RfRecipFinal));
+  const size_t rt_b = align256(nrt * sizeof(RfRecipTask)) + align256(cnt * sizeof(RfRecipFinal)) +
+                      align256(cnt * sizeof(RfNormTask));
+  // each level's two pipelines get task images of their own, so that all
+  // levels are enqueued at once; their scratch (spectra, coefficients,
+  // carry codes) is shared, one level after the other on the stream
+  std::vector<size_t> img1(levels), img2(levels);
+  size_t img_b = 0;
   size_t pipe_b = operands_batch_task_bound(2 * (size_t)cnt);
   for (size_t j = 0; j < levels; j++) {
-    pipe_b = std::max(pipe_b, layout_of(p1[j]).total());
-    pipe_b = std::max(pipe_b, layout_of(p2[j]).total());
+    const Layout l1 = layout_of(p1[j]), l2 = layout_of(p2[j]);
+    img1[j] = img_b;
+    img_b += align256(l1.tasks_b);
+    img2[j] = img_b;
+    img_b += align256(l2.tasks_b);
+    pipe_b = std::max(pipe_b, std::max(l1.total(), l2.total()));
   }
   const size_t limbs_b = align256(lim * 8);
   char* A = nullptr;
-  int rc = engine_arena(limbs_b + rt_b + pipe_b, &A);
+  int rc = engine_arena(limbs_b + rt_b + img_b + pipe_b, &A);
   if (rc) return rc;
   uint64_t* L = (uint64_t*)A;
   RfRecipTask* d_rt = (RfRecipTask*)(A + limbs_b);
   RfRecipFinal* d_fin = (RfRecipFinal*)(A + limbs_b + align256(nrt * sizeof(RfRecipTask)));
-  const size_t pipe = limbs_b + rt_b;
+  RfNormTask* d_norm = (RfNormTask*)(A + limbs_b + align256(nrt * sizeof(RfRecipTask)) +
+                                     align256(cnt * sizeof(RfRecipFinal)));
+  char* d_img = A + limbs_b + rt_b;
+  const size_t pipe = limbs_b + rt_b + img_b;
   // D limbs (kept for the fix-up) in one allocation shared by the moduli
   uint64_t d_total = 0;
   std::vector<uint64_t> d_off(cnt);
@@ -320,14 +339,23 @@ int mod_create_batch_locked(const mpz_t* ds, std::vector<ModPrep>& prep, rf_mod*
   uint64_t* dslab_p = nullptr;
   std::shared_ptr<void> dslab = device_slab(d_total * 8, (void**)&dslab_p);
   if (!dslab) return fail(RF_ERR_NOMEM, "moduli of %llu limbs", (unsigned long long)d_total);
-  // host image: [uploaded limbs | recip tasks | final tasks | D limbs]
+  // the levels' products, their pipelines and reciprocal tasks
   std::vector<RfRecipTask> rts;
   std::vector<size_t> rt_first(levels + 1, 0);
+  std::vector<Pipeline> pp1(levels), pp2(levels);
   for (size_t j = levels; j-- > 0;) {  // bottom-up order, the order of the launches
     rt_first[j] = rts.size();
-    for (int i : who[j]) {
-      const ModPrep& P = prep[i];
+    for (size_t k = 0; k < who[j].size(); k++) {
+      const ModPrep& P = prep[who[j][k]];
       const uint64_t h = P.c[j + 1];
+      Product& a = p1[j][k];
+      a.A[0].dptr = L + P.dn_off + (P.np - P.c[j]);
+      a.B[0].dptr = L + P.x_off[j + 1];
+      a.out.assign(1, L + P.t_off);
+      Product& b = p2[j][k];
+      b.A[0].dptr = L + P.tm_off;
+      b.B[0].dptr = L + P.x_off[j + 1];
+      b.out.assign(1, L + P.u_off);
       RfRecipTask t{};
       t.a = L + P.dn_off + (P.np - P.c[j]);
       t.xh = L + P.x_off[j + 1];
@@ -338,57 +366,52 @@ int mod_create_batch_locked(const mpz_t* ds, std::vector<ModPrep>& prep, rf_mod*
       t.n = P.c[j];
       t.h = h;
       t.l = P.c[j] - h;
+      t.tn = a.plan.rn;
+      t.un = b.plan.rn;
       rts.push_back(t);
     }
+    rc = build_pipeline(p1[j], pp1[j]);
+    if (!rc) rc = build_pipeline(p2[j], pp2[j]);
+    if (rc) return rc;
   }
   std::vector<RfRecipFinal> fins(cnt);
-  uint64_t max_il = 0;
+  std::vector<RfNormTask> norms(cnt);
+  uint64_t max_il = 0, max_np = 0;
   for (int i = 0; i < cnt; i++) {
     const ModPrep& P = prep[i];
     fins[i] = RfRecipFinal{L + P.x_off[0], P.c[0] + 1, L + P.i_off, P.il, P.s, P.o};
+    norms[i] = RfNormTask{dslab_p + d_off[i], P.n, L + P.dn_off, P.np, P.s, 0};
     max_il = std::max(max_il, P.il);
+    max_np = std::max(max_np, P.np);
   }
-  const size_t img_b = limbs_b + rt_b + align256(d_total * 8);
+  // host image: [uploaded limbs | recip, final and normalisation tasks |
+  // level task images | D limbs], one staging copy and four transfers
+  const size_t img_off = limbs_b + rt_b, dl_off = img_off + img_b;
   char* H = nullptr;
-  rc = engine_stage(img_b, &H);
+  rc = engine_stage(dl_off + align256(d_total * 8), &H);
   if (rc) return rc;
   hipStream_t st = engine_stream();
   HIPCHK(hipStreamSynchronize(st));
-  for (auto& P : prep) {
-    memcpy(H + P.dn_off * 8, P.dnorm.data(), P.np * 8);
-    memcpy(H + P.x_off.back() * 8, P.xbase.data(), P.xbase.size() * 8);
-  }
+  for (auto& P : prep) memcpy(H + P.x_off.back() * 8, P.xbase.data(), P.xbase.size() * 8);
   if (!rts.empty()) memcpy(H + limbs_b, rts.data(), rts.size() * sizeof(RfRecipTask));
   memcpy(H + limbs_b + align256(nrt * sizeof(RfRecipTask)), fins.data(), fins.size() * sizeof(RfRecipFinal));
-  for (int i = 0; i < cnt; i++) memcpy(H + limbs_b + rt_b + d_off[i] * 8, ds[i]->_mp_d, prep[i].n * 8);
+  memcpy(H + limbs_b + align256(nrt * sizeof(RfRecipTask)) + align256(cnt * sizeof(RfRecipFinal)), norms.data(),
+         norms.size() * sizeof(RfNormTask));
+  for (size_t j = 0; j < levels; j++) {
+    if (!pp1[j].image.empty()) memcpy(H + img_off + img1[j], pp1[j].image.data(), pp1[j].image.size());
+    if (!pp2[j].image.empty()) memcpy(H + img_off + img2[j], pp2[j].image.data(), pp2[j].image.size());
+  }
+  for (int i = 0; i < cnt; i++) memcpy(H + dl_off + d_off[i] * 8, ds[i]->_mp_d, prep[i].n * 8);
   HIPCHK(hipMemcpyAsync(A, H, up_limbs * 8, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(A + limbs_b, H + limbs_b, rt_b, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dslab_p, H + limbs_b + rt_b, d_total * 8, hipMemcpyHostToDevice, st));
-  // the levels, bottom-up
+  HIPCHK(hipMemcpyAsync(A + limbs_b, H + limbs_b, rt_b + img_b, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dslab_p, H + dl_off, d_total * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(rf_launch_mod_norm(d_norm, cnt, max_np, st));
+  // the levels, bottom-up, enqueued without waiting
   for (size_t j = levels; j-- > 0;) {
-    for (size_t k = 0; k < who[j].size(); k++) {
-      const ModPrep& P = prep[who[j][k]];
-      Product& a = p1[j][k];
-      a.A[0].dptr = L + P.dn_off + (P.np - P.c[j]);
-      a.B[0].dptr = L + P.x_off[j + 1];
-      a.out.assign(1, L + P.t_off);
-      Product& b = p2[j][k];
-      b.A[0].dptr = L + P.tm_off;
-      b.B[0].dptr = L + P.x_off[j + 1];
-      b.out.assign(1, L + P.u_off);
-      rts[rt_first[j] + k].tn = a.plan.rn;
-      rts[rt_first[j] + k].un = b.plan.rn;
-    }
-    // the lengths tn / un come from the plans: patch them into the uploaded table
-    HIPCHK(hipStreamSynchronize(st));
-    rc = engine_stage(who[j].size() * sizeof(RfRecipTask), &H);
-    if (rc) return rc;
-    memcpy(H, rts.data() + rt_first[j], who[j].size() * sizeof(RfRecipTask));
-    HIPCHK(hipMemcpyAsync(d_rt + rt_first[j], H, who[j].size() * sizeof(RfRecipTask), hipMemcpyHostToDevice, st));
-    rc = pipeline_run(p1[j], layout_of(p1[j]), pipe);
+    rc = launch_pipeline_tasks(pp1[j], A + pipe, d_img + img1[j], st, false);
     if (rc) return rc;
     HIPCHK(rf_launch_recip_adjust(d_rt + rt_first[j], (int)who[j].size(), st));
-    rc = pipeline_run(p2[j], layout_of(p2[j]), pipe);
+    rc = launch_pipeline_tasks(pp2[j], A + pipe, d_img + img2[j], st, false);
     if (rc) return rc;
     HIPCHK(rf_launch_recip_combine(d_rt + rt_first[j], (int)who[j].size(), st));
   }

@@ -110,6 +110,16 @@ struct RfRecipFinal {
   uint32_t s, o;
 };
 hipError_t rf_launch_recip_final(const RfRecipFinal* tasks, int ntasks, uint64_t max_il, hipStream_t st);
+// D'' = (D 2^s) B^(np - n) into out (np limbs): D of n <= np limbs, 0 <= s < 64
+// with (D 2^s) < B^n.
+struct RfNormTask {
+  const uint64_t* d;
+  uint64_t n;
+  uint64_t* out;
+  uint64_t np;
+  uint32_t s, pad;
+};
+hipError_t rf_launch_mod_norm(const RfNormTask* tasks, int ntasks, uint64_t max_np, hipStream_t st);
 
 hipError_t rf_launch_gen(const RfSpan* spans, const RfBlock* blocks, int nblocks, hipStream_t st);
 hipError_t rf_launch_checksum(const RfSpan* spans, const RfBlock* blocks, int nblocks, uint64_t* sums,

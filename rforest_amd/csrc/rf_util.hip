@@ -340,6 +340,20 @@ __global__ __launch_bounds__(256) void k_recip_final(const RfRecipFinal* __restr
   tk.out[j] = tk.s ? (hi << tk.s) | (lo >> (64 - tk.s)) : hi;
 }
 
+__global__ __launch_bounds__(256) void k_mod_norm(const RfNormTask* __restrict__ tasks, int ntasks) {
+  const RfNormTask tk = tasks[blockIdx.y];
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= tk.np) return;
+  const uint64_t z = tk.np - tk.n;  // zero limbs below D 2^s
+  uint64_t v = 0;
+  if (j >= z) {
+    const uint64_t i = j - z;
+    v = tk.d[i] << tk.s;
+    if (tk.s && i) v |= tk.d[i - 1] >> (64 - tk.s);
+  }
+  tk.out[j] = v;
+}
+
 __global__ void k_fill(uint64_t* __restrict__ dst, uint64_t n, uint64_t v) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = v;
@@ -387,9 +401,23 @@ hipError_t rf_launch_recip_combine(const RfRecipTask* tasks, int ntasks, hipStre
 
 hipError_t rf_launch_recip_final(const RfRecipFinal* tasks, int ntasks, uint64_t max_il, hipStream_t st) {
   if (ntasks <= 0 || max_il == 0) return hipSuccess;
-  // the grid covers the longest output; shorter ones stop early
-  hipLaunchKernelGGL(k_recip_final, dim3((unsigned)((max_il + 255) / 256), (unsigned)ntasks), dim3(256), 0, st, tasks,
-                     ntasks);
+  // the grid covers the longest output; shorter ones stop early; the grid's
+  // y extent is at most 65535
+  for (int t0 = 0; t0 < ntasks; t0 += 65535) {
+    const int nt = ntasks - t0 < 65535 ? ntasks - t0 : 65535;
+    hipLaunchKernelGGL(k_recip_final, dim3((unsigned)((max_il + 255) / 256), (unsigned)nt), dim3(256), 0, st,
+                       tasks + t0, nt);
+  }
+  return hipGetLastError();
+}
+
+hipError_t rf_launch_mod_norm(const RfNormTask* tasks, int ntasks, uint64_t max_np, hipStream_t st) {
+  if (ntasks <= 0 || max_np == 0) return hipSuccess;
+  for (int t0 = 0; t0 < ntasks; t0 += 65535) {  // the grid's y extent is at most 65535
+    const int nt = ntasks - t0 < 65535 ? ntasks - t0 : 65535;
+    hipLaunchKernelGGL(k_mod_norm, dim3((unsigned)((max_np + 255) / 256), (unsigned)nt), dim3(256), 0, st,
+                       tasks + t0, nt);
+  }
   return hipGetLastError();
 }
 
