@@ -32,6 +32,13 @@ constexpr uint32_t FN_ID = (0u << 0) | (1u << 2) | (2u << 4);
 // Whether f maps every incoming carry to the same outgoing one.
 __device__ __forceinline__ bool fn_constant(uint32_t f) { return (f & 15u) == ((f >> 2) & 15u); }
 
+// The integer value of a double holding an integer 0 <= v < 2^52: 2^52 + v
+// is exact and its 52 mantissa bits are v (one add and one mask instead of
+// the general conversion's truncation, scaling and two converts).
+__device__ __forceinline__ uint64_t int_of_double(double v) {
+  return (uint64_t)__double_as_longlong(v + 0x1p52) & ((1ull << 52) - 1);
+}
+
 // Garner's algorithm on residues res[j * N] (|x| <= p_j/2 + 8) into the
 // S-limb two's complement of the representative in (-P/2, P/2)
 // (src/lib/crt.c:11-94 states the same CRT on 62-bit primes).
@@ -72,7 +79,7 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
 #pragma unroll
   for (int l = 0; l < W; l++) x[l] = 0;
   {
-    const uint64_t t = (uint64_t)v[K - 1];
+    const uint64_t t = int_of_double(v[K - 1]);
     x[0] = (uint32_t)t;
     x[1] = (uint32_t)(t >> 32);
   }
@@ -81,7 +88,7 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
     const int nl = (48 * (K - 1 - j) + 31) / 32;  // limbs of X before the step
     const uint64_t pj = C->pu[j];
     const uint32_t plo = (uint32_t)pj, phi = (uint32_t)(pj >> 32);
-    uint64_t c = (uint64_t)v[j];
+    uint64_t c = int_of_double(v[j]);
     uint32_t prev = 0;
 #pragma unroll
     for (int l = 0; l < W; l++) {
