@@ -196,6 +196,26 @@ def test_device_batch_mixed_level_matches_alone():
         assert rf.mpz_rmatrix_mult_fft(A, r, B, d) == C
 
 
+def test_tree_level_all_1024_products_through_the_prepared_lanes():
+    """The whole product-tree level workload (1024 products 2x2 . 2x2, entries
+    log-uniform in [2^10, 2^17] bits, seed 1003) as the benchmark runs it:
+    a prepared batch of three lanes replayed from its captured graph, twice;
+    every product equals GMP schoolbook (mpz_mul / mpz_addmul)."""
+    from rforest_amd.gmp import rmatrix_mult_schoolbook
+    prods = workloads.tree_level(1024, seed=1003)
+    want = [rmatrix_mult_schoolbook(A, r, B, d) for (r, d, A, B) in prods]
+    batch = rf.DeviceBatch(prods)
+    batch.prepare(lanes=3)
+    for _ in range(2):
+        batch.clear_outputs()
+        batch.launch(graph=True)
+        rf.device_sync()
+        got = batch.results()
+        bad = [i for i, (g, w) in enumerate(zip(got, want)) if g != w]
+        assert not bad, f"products {bad[:8]} differ from GMP"
+    batch.release()
+
+
 def test_device_batch_repeat_is_deterministic():
     prods = workloads.headline(3, seed=42)
     batch = rf.DeviceBatch(prods)

@@ -115,6 +115,24 @@ def test_resident_long_transform():
     op.release()
 
 
+def test_resident_case_all_sixteen_products():
+    """The resident-operand workload in full: one 3x3 left operand of
+    4,194,304-bit entries, transformed once, times all 16 right 3x3 operands
+    of 1,048,576-bit entries (seed 1004) in one batch; every product equals
+    GMP schoolbook."""
+    from rforest_amd.gmp import rmatrix_mult_schoolbook
+    A, Bs = workloads.resident_case(1004)
+    assert len(Bs) == 16
+    op = rf.ResidentOperand(A, 3, 3, 1 << 20)
+    batch = rf.DeviceBatch([(3, 3, op, B) for B in Bs])
+    batch.run()
+    got = batch.results()
+    for i, B in enumerate(Bs):
+        assert got[i] == rmatrix_mult_schoolbook(A, 3, B, 3), f"right operand {i}"
+    batch.release()
+    op.release()
+
+
 def test_resident_case_two_products():
     """The resident-operand workload's shapes (3x3, 4 Mbit left, 1 Mbit
     right) on two of its right operands."""
