@@ -282,11 +282,14 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
   // Scan by composition: inclusive f_tid o ... o f_0 within each wave by
   // shuffles, then each wave composes the earlier waves' aggregates.  A
   // constant function composed after anything is itself, so in a wave whose
-  // functions are all constant (a digit that absorbs any incoming carry: the
-  // usual case) every lane's inclusive prefix is its own function and the
-  // shuffle scan is skipped.
+  // digits' functions are all constant (a digit that absorbs any incoming
+  // carry: the usual case) every owning lane's inclusive prefix is its own
+  // function and the shuffle scan is skipped.  The other lanes hold the
+  // identity: the halo lanes below the block's first digit, whose prefix is
+  // the identity as well, and lanes past the entry's last digit (only in its
+  // last block), whose prefixes and aggregates nothing reads.
   const int lane = tid & 63, wave = tid >> 6;
-  if (__builtin_amdgcn_ballot_w64(!fn_constant(f)) != 0) {
+  if (__builtin_amdgcn_ballot_w64(own && !fn_constant(f)) != 0) {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t g = __shfl_up(f, off);
