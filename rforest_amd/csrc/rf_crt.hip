@@ -289,7 +289,8 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
   // the identity as well, and lanes past the entry's last digit (only in its
   // last block), whose prefixes and aggregates nothing reads.
   const int lane = tid & 63, wave = tid >> 6;
-  if (__builtin_amdgcn_ballot_w64(own && !fn_constant(f)) != 0) {
+  const bool slow = __builtin_amdgcn_ballot_w64(own && !fn_constant(f)) != 0;
+  if (slow) {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t g = __shfl_up(f, off);
@@ -298,9 +299,15 @@ __global__ __launch_bounds__(RF_CRT_THREADS) void k_crt(RfDevTables tb,
   }
   if (lane == 63) code[wave] = f;
   __syncthreads();
-  uint32_t before = FN_ID;  // composition of the earlier waves
-  for (int w = 0; w < wave; w++) before = fn_compose(code[w], before);
-  const uint32_t incl = fn_compose(f, before);
+  // composition of the earlier waves' aggregates: the same for every lane,
+  // formed in scalar registers
+  uint32_t before = FN_ID;
+  for (int w = 0; w < wave; w++) before = fn_compose(__builtin_amdgcn_readfirstlane(code[w]), before);
+  // after the shortcut every lane that matters holds a constant (or, below
+  // the first digit, the identity with the identity before it): its
+  // inclusive prefix is its own function
+  uint32_t incl = f;
+  if (slow) incl = fn_compose(f, before);
   const uint32_t prev = __shfl_up(incl, 1);
   pre[tk.pre_off + tid] = (uint8_t)(lane == 0 ? before : prev);
   if (tid == RF_CRT_THREADS - 1) fun[tk.fun_idx] = (uint8_t)incl;
