@@ -37,20 +37,78 @@ __device__ __forceinline__ int xcd_task(int bid, int grid, int ntasks) {
 }
 
 // LDS slot of residue i in an exchange.  Both sides of an exchange address
-// residue i; the slot permutes i within aligned blocks of 32 so that each
-// 32-lane half of a ds_read_b64 / ds_write_b64 covers 32 distinct slots
-// modulo 32 (all 64 banks).  SW = 0: identity, enough when both sides' lanes
-// run over 32 consecutive residues.  SW = 1: i ^ h(i >> 5) with
-// h(m) = 2 (m mod 16) + (m mod 2), which also spreads residues two apart
-// (i = 2 lane + c) and pairs 32 apart (i = 32 (lane / 2) + lane mod 2 + c).
+// residue i; the slot is chosen so that each 32-lane half of a
+// ds_read_b64 / ds_write_b64 covers 32 distinct slots modulo 32 (all 64
+// banks).
+//   SW = 0: identity, enough when both sides' lanes run over 32 consecutive
+//           residues.
+//   SW = 2: i + 2 (i >> 5), rows of 32 residues padded by two slots: one
+//           side's lanes run over 32 consecutive residues, the other's take
+//           2^B consecutive residues (B = 1 .. 4) from each of 32 / 2^B rows.
+//   SW = 3: i + (i >> 4), rows of 16 padded by one: one group of 16
+//           consecutive residues per lane on one side against lanes 16 rows
+//           apart on the other.
+//   SW = 4: (i + 2 (i >> 5)) xor ((i >> 5) mod 2), SW = 2 with bit 0 flipped
+//           in odd rows: lanes taking residue pairs from 16 rows (i =
+//           32 (lane / 2) + lane mod 2 + c) need the pads of 16 consecutive
+//           rows to share their parity, lanes two residues apart (i =
+//           2 lane + c) need adjacent rows of opposite parity; the flip
+//           gives both, which no plain padding does.
+//   SW = 1: i xor h(i >> 5), h(m) = 2 (m mod 16) + (m mod 2), no padding:
+//           it spreads both of the SW = 4 patterns as well, at one VALU
+//           operation per access.
+// Every map is injective with slots below N + N / 16 (below N for SW = 0, 1).
 template <int SW>
 __device__ __forceinline__ int lds_slot(int i) {
   if constexpr (SW == 0) {
     return i;
+  } else if constexpr (SW == 2) {
+    return i + ((i >> 5) << 1);
+  } else if constexpr (SW == 3) {
+    return i + (i >> 4);
+  } else if constexpr (SW == 4) {
+    return (i + ((i >> 5) << 1)) ^ ((i >> 5) & 1);
   } else {
     const int m = i >> 5;
     return i ^ (((m & 15) << 1) | (m & 1));
   }
+}
+
+// Address of residue t | o in an exchange, t holding the thread's index
+// bits and o, a compile-time constant, the register position's.  The two
+// never share a bit: every index below places the bits of the thread id, the
+// group number u and the register index j at distinct positions, and the
+// group of (tid, u) is tid + T u = tid | u T.  So
+//   SW = 0, 2, 3: slot(t | o) = slot(t) + slot(o) (t | o = t + o, and
+//                 (t | o) >> k = (t >> k) + (o >> k)), one base per thread
+//                 and immediate offsets;
+//   SW = 4:       the same sum with bit 0 set to (t0 | o0) xor row parity,
+//                 at most four bases per thread;
+//   SW = 1:       slot(t) xor slot(o) (h is linear over GF(2)).
+template <int SW>
+__device__ __forceinline__ double* lds_at(double* sh, int t, int o) {
+  if constexpr (SW == 1) {
+    return (double*)((char*)sh + ((lds_slot<1>(t) << 3) ^ (lds_slot<1>(o) << 3)));
+  } else if constexpr (SW == 4) {
+    const int pt = lds_slot<2>(t), po = lds_slot<2>(o);
+    return sh + ((pt & ~1) + (po & ~1) + (((pt | po) ^ ((t ^ o) >> 5)) & 1));
+  } else {
+    return sh + lds_slot<SW>(t) + lds_slot<SW>(o);
+  }
+}
+
+// Slot map of an exchange between a side whose lanes run over 2^A
+// consecutive residues and one whose lanes run over 2^B (A, B: index bits
+// below the stages of each pass), passes of R1 and R2 stages, in a
+// transform of at least one wave (wide) or of less, with padded LDS or not.
+constexpr int lds_map(bool pad, bool wide, int R1, int R2, int A, int B) {
+  if (!wide) return 1;
+  if (A > B) { const int t = A; A = B; B = t; }
+  if (A >= 5) return 0;
+  if (!pad) return 1;
+  if (R1 >= 3 && R2 >= 3 && A >= 1 && B >= 5) return 2;
+  if (R1 == 4 && R2 == 4 && A == 0 && B == 4) return 3;
+  return 4;
 }
 
 // Slot for the pointwise-to-first-pass transposition (coalesced writes,
@@ -65,7 +123,12 @@ struct Shape {
   static constexpr int T = N / E;                      // threads per transform
   static constexpr int G = T >= 64 ? 1 : 64 / T;       // transforms per workgroup
   static constexpr int NP = (LGN + 3) / 4;             // forward passes
-  static constexpr int LDSN = N;                       // doubles of LDS per transform
+  // LDS per transform padded for the exchanges' slot maps from 2^13 on,
+  // where two (2^13) or one (2^14) transforms per CU hold either way;
+  // shorter transforms can run more workgroups per CU than 6 % padding
+  // leaves room for, and keep N doubles
+  static constexpr bool PAD = LGN >= 13;
+  static constexpr int LDSN = PAD ? N + (N >> 4) : N;
   static constexpr int THREADS = T * G;
   // waves per SIMD the register allocation must allow (HIP's second
   // launch bound): four, as many as two 64 KiB transforms in LDS leave room
@@ -389,22 +452,25 @@ __device__ __forceinline__ void fwd_rest(double* v, double* sh, int tid, rf_gtw 
     double2 bs[fwd_derive<LGN, P, ROW>() ? Q::NG * Q::R : 1];
     fwd_bases<LGN, P, ROW>(bs, tid, tw, hb);
     if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
-    // lanes run over consecutive residues on both sides when both passes
-    // keep at least five low index bits below their stages
-    constexpr int SW = (Shape<LGN>::T >= 64 && LGN - Qp::L0 - Qp::R >= 5 && LGN - Q::L0 - Q::R >= 5) ? 0 : 1;
+    // index bits below the stages of each side: lanes run over that many
+    // consecutive residues
+    constexpr int SW = lds_map(Shape<LGN>::PAD, Shape<LGN>::T >= 64, Qp::R, Q::R, LGN - Qp::L0 - Qp::R,
+                               LGN - Q::L0 - Q::R);
+    const int tw0 = fwd_index<LGN, P - 1>(fwd_group<LGN, P - 1>(tid, 0), 0);
 #pragma unroll
     for (int u = 0; u < Qp::NG; u++)
 #pragma unroll
       for (int j = 0; j < Qp::GS; j++)
-        sh[lds_slot<SW>(fwd_index<LGN, P - 1>(fwd_group<LGN, P - 1>(tid, u), j))] = v[u * Qp::GS + j];
+        *lds_at<SW>(sh, tw0, fwd_index<LGN, P - 1>(fwd_group<LGN, P - 1>(0, u), j)) = v[u * Qp::GS + j];
     // the exchanges after a wave-local one are wave-local too, so a wave
     // barrier after the reads suffices there as well
     xsync<fwd_local<LGN, P>()>();
+    const int tr0 = fwd_index<LGN, P>(fwd_group<LGN, P>(tid, 0), 0);
 #pragma unroll
     for (int u = 0; u < Q::NG; u++)
 #pragma unroll
       for (int j = 0; j < Q::GS; j++)
-        v[u * Q::GS + j] = sh[lds_slot<SW>(fwd_index<LGN, P>(fwd_group<LGN, P>(tid, u), j))];
+        v[u * Q::GS + j] = *lds_at<SW>(sh, tr0, fwd_index<LGN, P>(fwd_group<LGN, P>(0, u), j));
     xsync<fwd_local<LGN, P>()>();
     fwd_pass<LGN, P, 0, ROW>(v, tid, tw, p, pinv, bs, hb);
     fwd_rest<LGN, P + 1, ROW>(v, sh, tid, tw, p, pinv, hb);
@@ -421,18 +487,20 @@ __device__ __forceinline__ void inv_rest(double* v, double* sh, int tid, rf_gtw 
     double2 bs[inv_derive<LGN, P>() ? Q::NG * Q::R : 1];
     inv_bases<LGN, P>(bs, tid, tw);
     if constexpr (Q::REDUCE_BEFORE) reduce_all<Shape<LGN>::E>(v, p, pinv);
-    constexpr int SW = (Shape<LGN>::T >= 64 && Qp::L0 >= 5 && Q::L0 >= 5) ? 0 : 1;
+    constexpr int SW = lds_map(Shape<LGN>::PAD, Shape<LGN>::T >= 64, Qp::R, Q::R, Qp::L0, Q::L0);
+    const int tw0 = inv_index<LGN, P - 1>(tid, 0);
 #pragma unroll
     for (int u = 0; u < Qp::NG; u++)
 #pragma unroll
       for (int j = 0; j < Qp::GS; j++)
-        sh[lds_slot<SW>(inv_index<LGN, P - 1>(tid + Shape<LGN>::T * u, j))] = v[u * Qp::GS + j];
+        *lds_at<SW>(sh, tw0, inv_index<LGN, P - 1>(Shape<LGN>::T * u, j)) = v[u * Qp::GS + j];
     __syncthreads();
+    const int tr0 = inv_index<LGN, P>(tid, 0);
 #pragma unroll
     for (int u = 0; u < Q::NG; u++)
 #pragma unroll
       for (int j = 0; j < Q::GS; j++)
-        v[u * Q::GS + j] = sh[lds_slot<SW>(inv_index<LGN, P>(tid + Shape<LGN>::T * u, j))];
+        v[u * Q::GS + j] = *lds_at<SW>(sh, tr0, inv_index<LGN, P>(Shape<LGN>::T * u, j));
     __syncthreads();
     inv_pass<LGN, P>(v, tid, tw, p, pinv, bs);
     inv_rest<LGN, P + 1>(v, sh, tid, tw, p, pinv);
@@ -1288,7 +1356,10 @@ hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int 
   using S = Shape<RF_LGN_ROW>;
   const int grid = (ntasks + 7) & ~7;
   const int nt = (size_t)ntasks * S::N * sizeof(double) > RF_NT_BYTES;
-  hipLaunchKernelGGL(k_fwd_row, dim3(grid), dim3(S::THREADS), S::LDSN * sizeof(double), st, tb, tasks,
+  const size_t lds = (size_t)S::LDSN * sizeof(double);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)k_fwd_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_fwd_row, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks,
                      ntasks, spec, nt);
   return hipGetLastError();
 }
