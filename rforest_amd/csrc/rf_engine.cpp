@@ -57,6 +57,7 @@ struct Ctx {
   char* d_arena = nullptr;
   size_t arena_cap = 0;
   char* h_stage = nullptr;
+  char* d_stage = nullptr;  // the staging buffer's device address
   size_t stage_cap = 0;
   // profiling
   bool prof = false;
@@ -275,13 +276,20 @@ int ensure_arena(size_t bytes) {
 int ensure_stage(size_t bytes) {
   if (bytes <= g.stage_cap) return RF_OK;
   HIPCHK(hipStreamSynchronize(g.stream));
+  const size_t cap = std::max(bytes, g.stage_cap * 3 / 2);
   if (g.h_stage) HIPCHK(hipHostFree(g.h_stage));
-  g.h_stage = nullptr;
-  size_t cap = std::max(bytes, g.stage_cap * 3 / 2);
+  g.h_stage = g.d_stage = nullptr;
+  g.stage_cap = 0;
   hipError_t e = hipHostMalloc((void**)&g.h_stage, cap, hipHostMallocDefault);
   if (e != hipSuccess) {
-    g.stage_cap = 0;
+    g.h_stage = nullptr;
     return fail(RF_ERR_NOMEM, "pinned staging of %zu bytes: %s", cap, hipGetErrorString(e));
+  }
+  e = hipHostGetDevicePointer((void**)&g.d_stage, g.h_stage, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(g.h_stage);
+    g.h_stage = g.d_stage = nullptr;
+    return fail(RF_ERR_HIP, "staging device address: %s", hipGetErrorString(e));
   }
   g.stage_cap = cap;
   return RF_OK;
@@ -684,26 +692,20 @@ int launch_pipeline_tasks(const Pipeline& pp, char* base, const char* tasks, hip
 
 // Build, upload the task image through the pinned staging buffer, launch.
 // The arena must hold base + L.total() bytes (scratch starts at base).
-// stage_free: the caller guarantees that no copy in flight touches the
-// staging buffer's first L.tasks_b bytes (so no wait is needed before the
-// task image is written there), and that the buffer already holds them.
-int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipStream_t st,
-                 bool stage_free = false) {
+int run_pipeline(std::vector<Product>& prods, const Layout& L, size_t base, hipStream_t st) {
   if (g.arena_cap < base + L.total()) return fail(RF_ERR_NOMEM, "arena too small for pipeline");
   Pipeline pp;
   int rc = build_pipeline(prods, pp);
   if (rc) return rc;
   char* sbase = g.d_arena + base;
   const ScratchPtrs S = scratch_ptrs(sbase, pp.L);
-  if (!stage_free) {
-    rc = ensure_stage(pp.image.size());
-    if (rc) return rc;
-    // Earlier copies out of the staging buffer, and kernels over the arena,
-    // must finish before either is reused: they may be on the library's
-    // stream or on st.
-    HIPCHK(hipStreamSynchronize(st));
-    if (st != g.stream) HIPCHK(hipStreamSynchronize(g.stream));
-  }
+  rc = ensure_stage(pp.image.size());
+  if (rc) return rc;
+  // Earlier copies out of the staging buffer, and kernels over the arena,
+  // must finish before either is reused: they may be on the library's
+  // stream or on st.
+  HIPCHK(hipStreamSynchronize(st));
+  if (st != g.stream) HIPCHK(hipStreamSynchronize(g.stream));
   if (!pp.image.empty()) {
     memcpy(g.h_stage, pp.image.data(), pp.image.size());
     HIPCHK(hipMemcpyAsync(S.tasks, g.h_stage, pp.image.size(), hipMemcpyHostToDevice, st));
@@ -722,10 +724,13 @@ uint64_t bitlen_limbs(const uint64_t* x, uint64_t n) {
 // buffer (valid until the engine's next staging use; rn = 0: a zero entry).
 using TcSink = int (*)(void* ctx, int e, const uint64_t* tc, size_t rn);
 
-// One product through the staging buffer: inputs go after the task-image
-// region, so the inputs' copy, the task image's copy, the launches and the
-// result copy are all stream-ordered and the call waits once; the results
-// are handed to `sink` straight from the staging buffer.
+// One product through the staging buffer, laid out [task image | inputs]
+// like the start of the device arena, so that one copy uploads both; the
+// results come back without a copy of their own: the CRT and carry kernels
+// store them straight into the staging buffer (pinned host memory, through
+// its device address), over the task image and the inputs, which the copy
+// has consumed by then.  The call waits once and hands the results to
+// `sink` from the staging buffer.
 int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, const int* as,
                  const uint64_t* const* bp, const uint64_t* bn, const int* bs, TcSink sink, void* ctx,
                  const rf_operand* left = nullptr) {
@@ -760,13 +765,14 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
   const Layout L = layout_of(prods);
   const size_t in_b = align256(total_limbs * 8);
   const size_t out_n = (size_t)r * d * pl.rn;
-  const size_t out_b = align256(out_n * 8);
-  const size_t in_off = align256(L.tasks_b);  // staging: [task image | inputs], results over both
-  rc = ensure_arena(in_b + out_b + L.total());
+  const size_t in_off = align256(L.tasks_b);
+  // device arena: [task image | inputs | pipeline scratch]
+  rc = ensure_arena(in_off + in_b + L.total());
   if (rc) return rc;
-  rc = ensure_stage(std::max(in_off + in_b, out_b));
+  rc = ensure_stage(std::max(in_off + in_b, out_n * 8));
   if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(g.stream));  // an earlier call's copies out of the staging buffer
+  // an earlier call's kernels may still write results into the staging buffer
+  HIPCHK(hipStreamSynchronize(g.stream));
   uint64_t* hs = (uint64_t*)(g.h_stage + in_off);
   Product& Q = prods[0];
   for (int e = left ? r * d : 0; e < r * d + d * d; e++) {
@@ -775,17 +781,20 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
     const Entry& E = isA ? Q.A[ei] : Q.B[ei];
     if (E.nlimbs) memcpy(hs + E.limb_off, isA ? ap[ei] : bp[ei], E.nlimbs * 8);
   }
-  uint64_t* d_in = (uint64_t*)g.d_arena;
-  uint64_t* d_out = (uint64_t*)(g.d_arena + in_b);
-  if (total_limbs) HIPCHK(hipMemcpyAsync(d_in, hs, total_limbs * 8, hipMemcpyHostToDevice, g.stream));
+  char* d_tasks = g.d_arena;
+  uint64_t* d_in = (uint64_t*)(g.d_arena + in_off);
   for (auto* v : {&Q.A, &Q.B})
     for (auto& E : *v) E.dptr = d_in + E.limb_off;
-  for (int e = 0; e < r * d; e++) Q.out.push_back(d_out + (size_t)e * pl.rn);
-  rc = run_pipeline(prods, L, in_b + out_b, g.stream, /*stage_free=*/true);
+  uint64_t* d_res = (uint64_t*)g.d_stage;
+  for (int e = 0; e < r * d; e++) Q.out.push_back(d_res + (size_t)e * pl.rn);
+  Pipeline pp;
+  rc = build_pipeline(prods, pp);
   if (rc) return rc;
-  // stream order puts the result copy after the input and task copies that
-  // read the same staging bytes
-  HIPCHK(hipMemcpyAsync(g.h_stage, d_out, out_n * 8, hipMemcpyDeviceToHost, g.stream));
+  if (pp.image.size() > in_off) return fail(RF_ERR_ARG, "internal: task image exceeds its staging room");
+  memcpy(g.h_stage, pp.image.data(), pp.image.size());
+  HIPCHK(hipMemcpyAsync(d_tasks, g.h_stage, in_off + total_limbs * 8, hipMemcpyHostToDevice, g.stream));
+  rc = launch_pipeline_tasks(pp, g.d_arena + in_off + in_b, d_tasks, g.stream, g.prof);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(g.stream));
   if (g.prof) prof_collect();
   for (int e = 0; e < r * d && !rc; e++) rc = sink(ctx, e, (const uint64_t*)g.h_stage + (size_t)e * pl.rn, pl.rn);

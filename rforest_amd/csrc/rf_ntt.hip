@@ -22,6 +22,8 @@
 // where w = 1 is not multiplied (the first pass's block 0), and every eighth
 // stage reduces, so all values stay below 12.2 p + 160 < 2^51.5 (primes below
 // 2^51 / 9; the derivation is at RF_PMAX).
+#include <atomic>
+
 #include "rf_field.h"
 #include "rf_kernels.h"
 
@@ -1217,6 +1219,18 @@ __global__ __launch_bounds__(RF_COL_THREADS, 4) void k_inv_col(RfDevTables tb,
   for (int j = 0; j < J; j++) a[(size_t)j * stride] = rf_reduce(v[j], p, pinv);
 }
 
+// Opts kernel fn into `lds` bytes of dynamic LDS when that exceeds 64 KiB,
+// once per device (`done` holds one bit per device id below 64).
+void lds_optin(const void* fn, size_t lds, std::atomic<uint64_t>& done) {
+  if (lds <= 65536) return;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const uint64_t bit = 1ull << (dev & 63);
+  if (done.load(std::memory_order_relaxed) & bit) return;
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  done.fetch_or(bit, std::memory_order_relaxed);
+}
+
 template <int LGN, int T>
 hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntasks, double* spec,
                         hipStream_t st) {
@@ -1224,8 +1238,8 @@ hipError_t launch_fwd_t(const RfDevTables& tb, const RfFwdTask* tasks, int ntask
   const int wgs = (ntasks + S::G - 1) / S::G;
   const int grid = (wgs + 7) & ~7;
   const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
-  if (lds > 65536)
-    (void)hipFuncSetAttribute((const void*)k_fwd<LGN, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static std::atomic<uint64_t> optin{0};
+  lds_optin((const void*)k_fwd<LGN, T>, lds, optin);
   const int nt = (size_t)ntasks * S::N * sizeof(double) > RF_NT_BYTES;
   hipLaunchKernelGGL((k_fwd<LGN, T>), dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, spec, nt);
   return hipGetLastError();
@@ -1259,8 +1273,8 @@ hipError_t launch_inv_t(const RfDevTables& tb, const RfInvTask* tasks, int ntask
   const int wgs = (ntasks + S::G - 1) / S::G;
   const int grid = (wgs + 7) & ~7;
   const size_t lds = (size_t)S::G * S::LDSN * sizeof(double);
-  if (lds > 65536)
-    (void)hipFuncSetAttribute((const void*)k_inv<LGN>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static std::atomic<uint64_t> optin{0};
+  lds_optin((const void*)k_inv<LGN>, lds, optin);
   const int nt = (size_t)ntasks * S::N * sizeof(double) > RF_NT_BYTES;
   hipLaunchKernelGGL(k_inv<LGN>, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks, ntasks, aspec, bspec,
                      coef, nt);
@@ -1357,8 +1371,8 @@ hipError_t rf_launch_fwd_row(const RfDevTables& tb, const RfRowTask* tasks, int 
   const int grid = (ntasks + 7) & ~7;
   const int nt = (size_t)ntasks * S::N * sizeof(double) > RF_NT_BYTES;
   const size_t lds = (size_t)S::LDSN * sizeof(double);
-  if (lds > 65536)
-    (void)hipFuncSetAttribute((const void*)k_fwd_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static std::atomic<uint64_t> optin{0};
+  lds_optin((const void*)k_fwd_row, lds, optin);
   hipLaunchKernelGGL(k_fwd_row, dim3(grid), dim3(S::THREADS), lds, st, tb, tasks,
                      ntasks, spec, nt);
   return hipGetLastError();
