@@ -126,14 +126,9 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
     }
   }
   if (gt) {
-    uint64_t br = 0;
+    unsigned long long br = 0;
 #pragma unroll
-    for (int l = 0; l < S; l++) {
-      const uint64_t y = C->P[K][l];
-      const uint64_t d = X[l] - y - br;
-      br = (X[l] < y) || (X[l] == y && br);
-      X[l] = d;
-    }
+    for (int l = 0; l < S; l++) X[l] = __builtin_subcll(X[l], C->P[K][l], br, &br);
   }
 }
 
@@ -142,37 +137,47 @@ __device__ __forceinline__ void garner(const double* __restrict__ res, uint32_t 
 // emit(l, value) and returns the signed carry out.  Coefficient j occupies
 // limbs [t j, t j + S) unsigned, minus 2^64 at limb t j + S if negative.
 // TT > 0 fixes t = TT (and M) at compile time so the loops unroll.
+// Every coefficient read, i - m (m < M) and the sign term i - S / t, lies at
+// or above j0 + 1 for the digits of a block (i >= digit0 - 1, halo M + 2),
+// and the rows of coefficients below 0 hold zeros, so the reads need no
+// tests and address one row with constant offsets.  A limb's sum is formed
+// in 32-bit words with add-with-carry chains, its carries counted in one
+// signed 32-bit word: at most M + 1 terms, so carries stay in [-1, M].
 template <int S, int TT, typename Emit>
 __device__ __forceinline__ int64_t digit_sum(const uint64_t* __restrict__ X, int j0, int i, int t_rt,
                                              int M_rt, int64_t cin, Emit emit) {
   const int t = TT > 0 ? TT : t_rt;
   const int M = TT > 0 ? (S + TT - 1) / TT : M_rt;
-  const int jneg = i - S / t, lneg = S % t;
-  bool neg = false;
-  if (jneg >= j0 && jneg >= 0) neg = (X[(size_t)(jneg - j0) * S + (S - 1)] >> 63) != 0;
-  uint64_t lo = (uint64_t)cin;
-  int64_t hi = cin >> 63;
+  const int lneg = S % t;
+  const uint64_t* row = X + (size_t)(i - j0) * S;  // coefficient i; coefficient i - m at row - m S
+  const uint32_t nb = (uint32_t)(X[(size_t)(i - S / t - j0) * S + (S - 1)] >> 63);
+  uint32_t w0 = (uint32_t)cin, w1 = (uint32_t)((uint64_t)cin >> 32);
+  int32_t h = (int32_t)(cin >> 63);
 #pragma unroll
   for (int l = 0; l < t; l++) {
 #pragma unroll
     for (int m = 0; m < M; m++) {
       const int idx = m * t + l;
-      const int j = i - m;
-      if (idx < S && j >= 0) {
-        const uint64_t x = X[(size_t)(j - j0) * S + idx];
-        lo += x;
-        hi += (lo < x);
+      if (idx < S) {
+        const uint64_t x = row[idx - m * S];
+        unsigned int cy;
+        w0 = __builtin_addc(w0, (uint32_t)x, 0u, &cy);
+        w1 = __builtin_addc(w1, (uint32_t)(x >> 32), cy, &cy);
+        h += (int32_t)cy;
       }
     }
-    if (neg && l == lneg) {
-      hi -= (lo == 0);
-      lo -= 1;
+    if (l == lneg) {
+      unsigned int br;
+      w0 = __builtin_subc(w0, nb, 0u, &br);
+      w1 = __builtin_subc(w1, 0u, br, &br);
+      h -= (int32_t)br;
     }
-    emit(l, lo);
-    lo = (uint64_t)hi;
-    hi = hi >> 63;
+    emit(l, ((uint64_t)w1 << 32) | w0);
+    w0 = (uint32_t)h;
+    w1 = (uint32_t)(h >> 31);
+    h >>= 31;
   }
-  return (int64_t)lo;
+  return (int64_t)(((uint64_t)w1 << 32) | w0);
 }
 
 template <int K>
