@@ -6,7 +6,8 @@ csrc/rf_fftops.hip transform-domain arithmetic, csrc/rf_noalloc.hip the
 fixed scratch of the no-allocation entry point), host orchestration and the
 C ABI (csrc/rf_engine.cpp one-shot calls, batches, operands, trees;
 csrc/rf_serve.cpp the request server; csrc/rf_fft.cpp the general transform
-API; csrc/rf_mod.cpp remainders), planner and tables (csrc/rf_plan.cpp).  GMP comes from /opt/conda (gmp.h is not installed
+API; csrc/rf_mod.cpp remainders; csrc/rf_hostpar.cpp the host copy pool),
+planner and tables (csrc/rf_plan.cpp).  GMP comes from /opt/conda (gmp.h is not installed
 system-wide on this image).
 """
 import os
@@ -18,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librforest_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", f)
            for f in ("rf_ntt.hip", "rf_crt.hip", "rf_util.hip", "rf_fftops.hip", "rf_noalloc.hip", "rf_engine.cpp",
-                     "rf_serve.cpp", "rf_plan.cpp", "rf_fft.cpp", "rf_mod.cpp")]
+                     "rf_serve.cpp", "rf_plan.cpp", "rf_fft.cpp", "rf_mod.cpp", "rf_hostpar.cpp")]
 OBJDIR = os.path.join(ROOT, "_build")
 GMP_PREFIX = os.environ.get("RF_GMP_PREFIX", "/opt/conda")
 

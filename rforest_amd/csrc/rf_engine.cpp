@@ -27,6 +27,7 @@
 
 #include "../../include/rforest_hip.h"
 #include "../../include/rforest_hip_stages.h"
+#include "rf_hostpar.h"
 #include "rf_internal.h"
 #include "rf_kernels.h"
 #include "rf_noalloc.h"
@@ -735,6 +736,7 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
                  const uint64_t* const* bp, const uint64_t* bn, const int* bs, TcSink sink, void* ctx,
                  const rf_operand* left = nullptr) {
   if (r <= 0 || d <= 0) return fail(RF_ERR_ARG, "bad dimensions r=%d d=%d", r, d);
+  par_wake();  // copy workers asleep after an idle spell start spinning now
   Product P;
   P.r = r;
   P.d = d;
@@ -775,12 +777,14 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
   HIPCHK(hipStreamSynchronize(g.stream));
   uint64_t* hs = (uint64_t*)(g.h_stage + in_off);
   Product& Q = prods[0];
+  std::vector<CopySeg> segs;
   for (int e = left ? r * d : 0; e < r * d + d * d; e++) {
     const bool isA = e < r * d;
     const int ei = isA ? e : e - r * d;
     const Entry& E = isA ? Q.A[ei] : Q.B[ei];
-    if (E.nlimbs) memcpy(hs + E.limb_off, isA ? ap[ei] : bp[ei], E.nlimbs * 8);
+    if (E.nlimbs) segs.push_back({hs + E.limb_off, isA ? ap[ei] : bp[ei], E.nlimbs * 8});
   }
+  par_copy(segs.data(), (int)segs.size());
   char* d_tasks = g.d_arena;
   uint64_t* d_in = (uint64_t*)(g.d_arena + in_off);
   for (auto* v : {&Q.A, &Q.B})
@@ -821,6 +825,53 @@ size_t tc_signmag(const uint64_t* tc, size_t rn, uint64_t* dst, bool& neg) {
   return n;
 }
 
+// tc_signmag for several results at once, in pieces over the copy pool.  A
+// negative tc's magnitude is 2^(64 rn) - tc: zero below its lowest nonzero
+// limb z, -tc[z] at z and ~tc[i] above, so every piece is independent.
+struct TcOut {
+  const uint64_t* tc;
+  size_t rn;
+  uint64_t* dst;  // rn limbs of room
+  bool neg = false;
+  size_t z = 0;   // lowest nonzero limb of a negative tc
+  size_t n = 0;   // the magnitude's length without high zero limbs
+};
+constexpr size_t RF_TC_PIECE = 8192;  // limbs per task
+
+void tc_signmag_many(std::vector<TcOut>& v) {
+  std::vector<size_t> first(v.size() + 1, 0);
+  for (size_t k = 0; k < v.size(); k++) {
+    TcOut& o = v[k];
+    o.neg = o.rn && (o.tc[o.rn - 1] >> 63);
+    o.z = 0;
+    if (o.neg)
+      while (o.tc[o.z] == 0) o.z++;  // stops: the top limb of a negative tc is nonzero
+    first[k + 1] = first[k] + (o.rn + RF_TC_PIECE - 1) / RF_TC_PIECE;
+  }
+  struct Job {
+    std::vector<TcOut>* v;
+    const size_t* first;
+  } job{&v, first.data()};
+  par_for((int)first[v.size()], [](void* vp, int i) {
+    const Job& j = *(const Job*)vp;
+    size_t k = 0;
+    while (j.first[k + 1] <= (size_t)i) k++;
+    const TcOut& o = (*j.v)[k];
+    const size_t lo = ((size_t)i - j.first[k]) * RF_TC_PIECE;
+    const size_t hi = std::min(o.rn, lo + RF_TC_PIECE);
+    if (!o.neg) {
+      memcpy(o.dst + lo, o.tc + lo, (hi - lo) * 8);
+      return;
+    }
+    for (size_t l = lo; l < hi; l++) o.dst[l] = l < o.z ? 0 : (l == o.z ? 0 - o.tc[l] : ~o.tc[l]);
+  }, &job);
+  for (TcOut& o : v) {
+    size_t n = o.rn;
+    while (n && o.dst[n - 1] == 0) n--;
+    o.n = n;
+  }
+}
+
 void tc_to_signmag(std::vector<uint64_t>& x, bool& neg) {
   neg = !x.empty() && (x.back() >> 63);
   if (neg) {
@@ -849,20 +900,37 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* le
     bs[e] = mpz_sgn(B[e]);
   }
   // results go straight from the staging buffer into C (no aliasing of C
-  // with A or B, src/include/hwmpz.h:128)
+  // with A or B, src/include/hwmpz.h:128): the sink makes room in each
+  // entry, then all entries convert at once over the copy pool while the
+  // staging buffer still holds them (the engine's lock is held throughout)
+  struct Out {
+    mpz_t* C;
+    std::vector<TcOut> outs;
+    std::vector<int> ents;
+  } out{C, {}, {}};
   auto sink = [](void* ctx, int e, const uint64_t* tc, size_t rn) -> int {
-    mpz_t* Cv = (mpz_t*)ctx;
+    Out& o = *(Out*)ctx;
     if (rn == 0) {
-      mpz_set_ui(Cv[e], 0);
+      mpz_set_ui(o.C[e], 0);
       return RF_OK;
     }
-    _mpz_realloc(Cv[e], (mp_size_t)rn);  // src/lib/hwmpz.c:114
-    bool neg = false;
-    const size_t n = tc_signmag(tc, rn, (uint64_t*)Cv[e]->_mp_d, neg);
-    Cv[e]->_mp_size = n == 0 ? 0 : (neg ? -(int)n : (int)n);  // src/lib/hwmpz.c:116-122
+    _mpz_realloc(o.C[e], (mp_size_t)rn);  // src/lib/hwmpz.c:114
+    TcOut t;
+    t.tc = tc;
+    t.rn = rn;
+    t.dst = (uint64_t*)o.C[e]->_mp_d;
+    o.outs.push_back(t);
+    o.ents.push_back(e);
     return RF_OK;
   };
-  return host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), sink, C, left);
+  int rc = host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), sink, &out, left);
+  if (rc) return rc;
+  tc_signmag_many(out.outs);
+  for (size_t k = 0; k < out.outs.size(); k++) {
+    const TcOut& t = out.outs[k];
+    out.C[out.ents[k]]->_mp_size = t.n == 0 ? 0 : (t.neg ? -(int)t.n : (int)t.n);  // src/lib/hwmpz.c:116-122
+  }
+  return RF_OK;
 }
 
 // ---- products beyond one device plan ----

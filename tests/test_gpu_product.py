@@ -165,6 +165,30 @@ def test_dropin_mpz_abi():
         assert rf.lib().rf_mpz_rmatrix_mult(mC2.ptr(), mA.ptr(), r, mB.ptr(), d) == rf.RF_OK
 
 
+def test_dropin_mpz_results_converted_in_pieces():
+    """The mpz_t drop-in converts two's complement results to sign and
+    magnitude in pieces of 8192 limbs: negative results whose lowest nonzero
+    limb lies in a later piece (or starts one), a power of two whose
+    magnitude is shorter than its two's complement, and mixed signs over
+    results of several pieces, against exact products."""
+    rng = random.Random(11)
+    odd = rng.getrandbits(3000) | 1
+    cases = [
+        ([-odd << (64 * 9000 + 5)], 1, [3], 1),                  # zero limbs 0..8999, negative
+        ([-(1 << (64 * 8192))], 1, [1], 1),                      # lowest nonzero limb = 8192, a piece start
+        ([1 << (64 * 8192)], 1, [-(1 << 64)], 1),                # the same, one limb higher
+        (_rand_product(rng, 600000, 2, 2)[0], 2, _rand_product(rng, 600000, 2, 2)[1], 2),  # ~18,750 limbs
+        ([-(rng.getrandbits(700000) << 640), rng.getrandbits(5000)], 1,
+         [rng.getrandbits(640000), -(1 << 100), 1, -rng.getrandbits(90000)], 2),
+    ]
+    w = MpzArray.zeros(1)
+    for (A, r, B, d) in cases:
+        mA, mB, mC = MpzArray(A), MpzArray(B), MpzArray.zeros(r * d)
+        mC.set([-(12345678901234567890 ** 5)] * (r * d))
+        rf.lib().mpz_rmatrix_mult_fft(mC.ptr(), mA.ptr(), r, mB.ptr(), d, w.ptr())
+        assert mC.values() == naive(A, r, B, d)
+
+
 def test_plain_array_path_refuses_sizes_without_a_plan():
     """rf_rmatrix_mult_limbs (plain arrays, one device plan per call) returns
     RF_ERR_UNSUPPORTED for a product of two 2^33-bit entries, which needs 2^25
