@@ -58,6 +58,14 @@ int rf_stage_dropin_piece_bits(uint64_t bits);
 int rf_stage_mod_reciprocal(const uint64_t* d, uint64_t dn, uint64_t max_bits, uint64_t* out, uint64_t cap,
                             uint64_t* nlimbs);
 
+/* The host epilogue of every host entry point: the rn-limb two's complement
+   tc as sign and magnitude, the magnitude's limbs (without high zeros) to dst
+   (cap limbs of room), its length times the sign to *nsigned; converted in
+   pieces over the library's copy threads.  RF_ERR_CAPACITY when the
+   magnitude needs more than cap limbs (nothing written then).  Host only:
+   no GPU work, usable before hw_mpz_setup. */
+int rf_stage_tc_to_signmag(const uint64_t* tc, uint64_t rn, uint64_t* dst, uint64_t cap, long* nsigned);
+
 /* Root of order 2^lgN used for prime index q. */
 uint64_t rf_root(int q, int lgN);
 

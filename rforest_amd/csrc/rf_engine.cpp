@@ -805,51 +805,40 @@ int host_product(int r, int d, const uint64_t* const* ap, const uint64_t* an, co
   return rc;
 }
 
-// two's complement -> (magnitude limbs, negative?)
-// Sign and magnitude of the rn-limb two's complement tc into dst (rn limbs
-// of room); returns the magnitude's length without high zero limbs.
-size_t tc_signmag(const uint64_t* tc, size_t rn, uint64_t* dst, bool& neg) {
-  neg = rn && (tc[rn - 1] >> 63);
-  if (neg) {
-    uint64_t br = 1;
-    for (size_t i = 0; i < rn; i++) {
-      const uint64_t y = ~tc[i] + br;
-      br = (br && y == 0);
-      dst[i] = y;
-    }
-  } else if (dst != tc) {
-    memcpy(dst, tc, rn * 8);
-  }
-  size_t n = rn;
-  while (n && dst[n - 1] == 0) n--;
-  return n;
-}
-
-// tc_signmag for several results at once, in pieces over the copy pool.  A
-// negative tc's magnitude is 2^(64 rn) - tc: zero below its lowest nonzero
-// limb z, -tc[z] at z and ~tc[i] above, so every piece is independent.
+// Sign and magnitude of rn-limb two's complement results, several at once
+// and in pieces over the copy pool.  A negative tc's magnitude is 2^(64 rn) - tc: zero below its lowest nonzero
+// limb z, -tc[z] at z and ~tc[i] above, so every piece is independent, and
+// its length is known before any limb is written (limb i > z of the
+// magnitude is zero exactly when tc[i] is all ones).
 struct TcOut {
   const uint64_t* tc;
   size_t rn;
-  uint64_t* dst;  // rn limbs of room
+  uint64_t* dst = nullptr;  // n limbs of room (set after tc_prepare)
   bool neg = false;
-  size_t z = 0;   // lowest nonzero limb of a negative tc
-  size_t n = 0;   // the magnitude's length without high zero limbs
+  size_t z = 0;  // lowest nonzero limb of a negative tc
+  size_t n = 0;  // the magnitude's length without high zero limbs
 };
 constexpr size_t RF_TC_PIECE = 8192;  // limbs per task
 
-void tc_signmag_many(std::vector<TcOut>& v) {
-  std::vector<size_t> first(v.size() + 1, 0);
-  for (size_t k = 0; k < v.size(); k++) {
-    TcOut& o = v[k];
-    o.neg = o.rn && (o.tc[o.rn - 1] >> 63);
-    o.z = 0;
-    if (o.neg)
-      while (o.tc[o.z] == 0) o.z++;  // stops: the top limb of a negative tc is nonzero
-    first[k + 1] = first[k] + (o.rn + RF_TC_PIECE - 1) / RF_TC_PIECE;
+void tc_prepare(TcOut& o) {
+  o.neg = o.rn && (o.tc[o.rn - 1] >> 63);
+  o.z = 0;
+  size_t n = o.rn;
+  if (o.neg) {
+    while (o.tc[o.z] == 0) o.z++;  // stops: the top limb of a negative tc is nonzero
+    while (n > o.z + 1 && o.tc[n - 1] == ~(uint64_t)0) n--;
+  } else {
+    while (n && o.tc[n - 1] == 0) n--;
   }
+  o.n = n;
+}
+
+// Writes every prepared result's n magnitude limbs to its dst.
+void tc_convert_many(const std::vector<TcOut>& v) {
+  std::vector<size_t> first(v.size() + 1, 0);
+  for (size_t k = 0; k < v.size(); k++) first[k + 1] = first[k] + (v[k].n + RF_TC_PIECE - 1) / RF_TC_PIECE;
   struct Job {
-    std::vector<TcOut>* v;
+    const std::vector<TcOut>* v;
     const size_t* first;
   } job{&v, first.data()};
   par_for((int)first[v.size()], [](void* vp, int i) {
@@ -858,18 +847,13 @@ void tc_signmag_many(std::vector<TcOut>& v) {
     while (j.first[k + 1] <= (size_t)i) k++;
     const TcOut& o = (*j.v)[k];
     const size_t lo = ((size_t)i - j.first[k]) * RF_TC_PIECE;
-    const size_t hi = std::min(o.rn, lo + RF_TC_PIECE);
+    const size_t hi = std::min(o.n, lo + RF_TC_PIECE);
     if (!o.neg) {
       memcpy(o.dst + lo, o.tc + lo, (hi - lo) * 8);
       return;
     }
     for (size_t l = lo; l < hi; l++) o.dst[l] = l < o.z ? 0 : (l == o.z ? 0 - o.tc[l] : ~o.tc[l]);
   }, &job);
-  for (TcOut& o : v) {
-    size_t n = o.rn;
-    while (n && o.dst[n - 1] == 0) n--;
-    o.n = n;
-  }
 }
 
 void tc_to_signmag(std::vector<uint64_t>& x, bool& neg) {
@@ -918,6 +902,7 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* le
     TcOut t;
     t.tc = tc;
     t.rn = rn;
+    tc_prepare(t);
     t.dst = (uint64_t*)o.C[e]->_mp_d;
     o.outs.push_back(t);
     o.ents.push_back(e);
@@ -925,7 +910,7 @@ int mpz_product(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d, const rf_operand* le
   };
   int rc = host_product(r, d, ap.data(), an.data(), as.data(), bp.data(), bn.data(), bs.data(), sink, &out, left);
   if (rc) return rc;
-  tc_signmag_many(out.outs);
+  tc_convert_many(out.outs);
   for (size_t k = 0; k < out.outs.size(); k++) {
     const TcOut& t = out.outs[k];
     out.C[out.ents[k]]->_mp_size = t.n == 0 ? 0 : (t.neg ? -(int)t.n : (int)t.n);  // src/lib/hwmpz.c:116-122
@@ -1070,6 +1055,19 @@ int rf_mpz_rmatrix_mult(mpz_t* C, mpz_t* A, int r, mpz_t* B, int d) {
   return mpz_product_any(C, A, r, B, d);
 }
 
+int rf_stage_tc_to_signmag(const uint64_t* tc, uint64_t rn, uint64_t* dst, uint64_t cap, long* nsigned) {
+  if ((rn && (!tc || !dst)) || !nsigned) return fail(RF_ERR_ARG, "rf_stage_tc_to_signmag: null argument");
+  std::vector<TcOut> v(1);
+  v[0].tc = tc;
+  v[0].rn = rn;
+  tc_prepare(v[0]);
+  if (v[0].n > cap) return fail(RF_ERR_CAPACITY, "magnitude needs %zu limbs", v[0].n);
+  v[0].dst = dst;
+  tc_convert_many(v);
+  *nsigned = v[0].neg ? -(long)v[0].n : (long)v[0].n;
+  return RF_OK;
+}
+
 int rf_stage_dropin_piece_bits(uint64_t bits) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (bits && bits < 64) return fail(RF_ERR_ARG, "piece size below 64 bits");
@@ -1113,26 +1111,38 @@ int rf_rmatrix_mult_limbs(uint64_t* c, const size_t* coff, const size_t* ccap, l
   std::vector<uint64_t> anv(r * d), bnv(d * d);
   for (int e = 0; e < r * d; e++) ap[e] = a + aoff[e], anv[e] = an[e];
   for (int e = 0; e < d * d; e++) bp[e] = b + boff[e], bnv[e] = bn[e];
+  // each result's length is known before it is written: the capacity check
+  // comes first, then all results convert at once over the copy pool
+  // straight into the caller's array (the staging buffer still holds them)
   struct Ctx {
     uint64_t* c;
     const size_t *coff, *ccap;
     long* csize;
-  } cx{c, coff, ccap, csize};
+    std::vector<TcOut> outs;
+    std::vector<int> ents;
+  } cx{c, coff, ccap, csize, {}, {}};
   auto sink = [](void* vp, int e, const uint64_t* tc, size_t rn) -> int {
     Ctx& x = *(Ctx*)vp;
-    if (rn == 0) {
-      x.csize[e] = 0;
-      return RF_OK;
-    }
-    std::vector<uint64_t> tmp(rn);
-    bool neg = false;
-    const size_t n = tc_signmag(tc, rn, tmp.data(), neg);
-    if (n > x.ccap[e]) return fail(RF_ERR_CAPACITY, "output %d needs %zu limbs", e, n);
-    memcpy(x.c + x.coff[e], tmp.data(), n * 8);
-    x.csize[e] = neg ? -(long)n : (long)n;
+    x.csize[e] = 0;
+    if (rn == 0) return RF_OK;
+    TcOut t;
+    t.tc = tc;
+    t.rn = rn;
+    tc_prepare(t);
+    if (t.n > x.ccap[e]) return fail(RF_ERR_CAPACITY, "output %d needs %zu limbs", e, t.n);
+    t.dst = x.c + x.coff[e];
+    x.outs.push_back(t);
+    x.ents.push_back(e);
     return RF_OK;
   };
-  return host_product(r, d, ap.data(), anv.data(), as, bp.data(), bnv.data(), bs, sink, &cx);
+  int rc = host_product(r, d, ap.data(), anv.data(), as, bp.data(), bnv.data(), bs, sink, &cx);
+  if (rc) return rc;
+  tc_convert_many(cx.outs);
+  for (size_t k = 0; k < cx.outs.size(); k++) {
+    const TcOut& t = cx.outs[k];
+    cx.csize[cx.ents[k]] = t.n == 0 ? 0 : (t.neg ? -(long)t.n : (long)t.n);
+  }
+  return RF_OK;
 }
 
 uint64_t rf_out_limbs(uint64_t la_limbs, uint64_t lb_limbs, int d) {

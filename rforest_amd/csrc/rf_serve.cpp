@@ -29,6 +29,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "rf_hostpar.h"
 #include "rf_internal.h"
 #include "rf_util.h"
 
@@ -85,79 +86,19 @@ class Pool {
 
 constexpr size_t NONE = ~(size_t)0;
 
-// Copy threads of a server: the host copies of RF_IN_HOST inputs into the
-// pinned staging buffer and of RF_OUT_HOST results out of it run on these
-// and the worker thread together (one thread copies far below the host's
-// memory bandwidth).  Small copies stay on the worker.
-class CopyPool {
- public:
-  explicit CopyPool(int n) {
-    for (int i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+// Host copies of RF_IN_HOST inputs into a slot's staging buffer and of
+// RF_OUT_HOST results out of it: on the copy pool from 8 MiB on, below that
+// on the worker thread (a batch's small copies do not repay the hand-off).
+void serve_copy(const std::vector<CopySeg>& cps) {
+  size_t bytes = 0;
+  for (const CopySeg& c : cps) bytes += c.n;
+  if (bytes >= ((size_t)8 << 20)) {
+    par_copy(cps.data(), (int)cps.size());
+    return;
   }
-  ~CopyPool() {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (auto& t : th_) t.join();
-  }
-  // f(i) for i in [0, n) on the pool and the calling thread; returns when done.
-  void run(size_t n, const std::function<void(size_t)>& f, size_t bytes) {
-    if (n <= 1 || th_.empty() || bytes < ((size_t)8 << 20)) {
-      for (size_t i = 0; i < n; i++) f(i);
-      return;
-    }
-    std::unique_lock<std::mutex> lk(mu_);
-    job_ = &f;
-    n_ = n;
-    next_ = 0;
-    left_ = n;
-    gen_++;
-    cv_.notify_all();
-    lk.unlock();
-    work();
-    lk.lock();
-    done_.wait(lk, [&] { return left_ == 0; });
-    job_ = nullptr;
-  }
+  for (const CopySeg& c : cps) memcpy(c.dst, c.src, c.n);
+}
 
- private:
-  void loop() {
-    uint64_t seen = 0;
-    for (;;) {
-      {
-        std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-        if (stop_) return;
-        seen = gen_;
-      }
-      work();
-    }
-  }
-  void work() {
-    for (;;) {
-      const std::function<void(size_t)>* f;
-      size_t i;
-      {
-        std::lock_guard<std::mutex> lk(mu_);
-        if (!job_ || next_ >= n_) return;
-        f = job_;
-        i = next_++;
-      }
-      (*f)(i);
-      std::lock_guard<std::mutex> lk(mu_);
-      if (--left_ == 0) done_.notify_all();
-    }
-  }
-  std::vector<std::thread> th_;
-  std::mutex mu_;
-  std::condition_variable cv_, done_;
-  const std::function<void(size_t)>* job_ = nullptr;
-  size_t n_ = 0, next_ = 0, left_ = 0;
-  uint64_t gen_ = 0;
-  bool stop_ = false;
-};
 
 struct Req {
   uint64_t id = 0, tag = 0, seed = 0;
@@ -215,7 +156,6 @@ struct rf_server {
   std::unordered_map<uint64_t, Req*> kept;
   std::vector<rf_completion> done;
   std::vector<Slot> slots;
-  std::unique_ptr<CopyPool> copiers;
   std::thread worker;
   bool stop = false;
   uint64_t next_id = 1;
@@ -251,16 +191,13 @@ void retire(rf_server* S, Slot& sl, int status) {
   // host results: from the slot's staging buffer to the callers' arrays
   // (the batch has completed, and the slot is not reused before this returns)
   if (status == RF_OK) {
-    std::vector<std::pair<Req*, int>> jobs;  // (request, C entry)
-    size_t bytes = 0;
+    std::vector<CopySeg> cps;
     for (Req* q : sl.reqs)
       if (q->status == RF_OK && q->st_out != NONE)
-        for (int e = 0; e < q->r * q->d; e++) jobs.push_back({q, e}), bytes += q->rn * 8;
-    S->copiers->run(jobs.size(), [&](size_t i) {
-      Req* q = jobs[i].first;
-      const int e = jobs[i].second;
-      memcpy(q->out_host + (size_t)e * q->out_cap, sl.h_stage + q->st_out + (size_t)e * q->rn * 8, q->rn * 8);
-    }, bytes);
+        for (int e = 0; e < q->r * q->d; e++)
+          cps.push_back(CopySeg{q->out_host + (size_t)e * q->out_cap, sl.h_stage + q->st_out + (size_t)e * q->rn * 8,
+                                q->rn * 8});
+    serve_copy(cps);
   }
   const double t = S->now();
   std::lock_guard<std::mutex> lk(S->mu);
@@ -420,23 +357,18 @@ bool admit(rf_server* S, Slot& sl) {
     if (!blocks.empty()) memcpy(h + o_gb, blocks.data(), blocks.size() * sizeof(RfBlock));
     if (!sspans.empty()) memcpy(h + o_ss, sspans.data(), sspans.size() * sizeof(RfSpan));
     if (!sblocks.empty()) memcpy(h + o_sb, sblocks.data(), sblocks.size() * sizeof(RfBlock));
-    struct Cp {
-      char* dst;
-      const void* src;
-      size_t n;
-    };
-    std::vector<Cp> cps;
-    size_t bytes = 0;
+    // RF_IN_HOST inputs into the staging buffer, split over the copy pool
+    std::vector<CopySeg> cps;
     for (Req* q : take) {
       if (q->st_in == NONE) continue;
       uint64_t pos = 0;
       for (size_t e = 0; e < q->ents.size(); e++) {
         const uint64_t n = q->ents[e].sign ? (q->ents[e].bits + 63) / 64 : 0;
-        if (n) cps.push_back(Cp{h + q->st_in + pos * 8, q->ents[e].limbs, n * 8}), bytes += n * 8;
+        if (n) cps.push_back(CopySeg{h + q->st_in + pos * 8, q->ents[e].limbs, n * 8});
         pos += n;
       }
     }
-    S->copiers->run(cps.size(), [&](size_t i) { memcpy(cps[i].dst, cps[i].src, cps[i].n); }, bytes);
+    serve_copy(cps);
   }
   // device side: the task image sits in the layout's task area, the util
   // tables after the layout
@@ -570,7 +502,6 @@ int rf_server_create(const rf_server_config* cfg, rf_server** out) {
       return rc;
     }
   }
-  S->copiers.reset(new CopyPool(3));
   S->worker = std::thread(worker_main, S);
   *out = S;
   return RF_OK;
