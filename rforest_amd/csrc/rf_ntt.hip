@@ -568,7 +568,8 @@ __device__ __forceinline__ void pack_chunks(rf_glimbs __restrict__ limbs, uint32
 }
 
 // Packs with a compile-time chunk width T: every limb of a sub-batch of SB
-// chunks (at most 16 limbs) is loaded before any is used.  pack_full16: every
+// chunks (at most 16 limbs, or LIMBS on the aligned path) is loaded before
+// any is used.  pack_full16: every
 // chunk inside the entry, T even and the entry 16-byte aligned, so each chunk
 // is T/2 aligned 16-byte loads.  pack_any8: 8-byte loads with the bounds
 // checks of pack_chunks.  Same arithmetic and bounds as pack_chunks.
@@ -588,21 +589,21 @@ __device__ __forceinline__ double pack_combine(const uint64_t (&x)[T], const dou
   return rf_reduce(r, p, pinv);
 }
 
-// Chunks per sub-batch: the largest power of two with at most 16 limbs
+// Chunks per sub-batch: the largest power of two with at most LIMBS limbs
 // (at least one chunk), never more than NCH; NCH is a power of two, so
 // sub-batches tile it exactly.
-template <int NCH, int T>
+template <int NCH, int T, int LIMBS>
 constexpr int pack_sb() {
   int sb = 1;
-  while (sb * 2 <= NCH && sb * 2 * T <= 16) sb *= 2;
+  while (sb * 2 <= NCH && sb * 2 * T <= LIMBS) sb *= 2;
   return sb;
 }
 
-template <int NCH, int T>
+template <int NCH, int T, int LIMBS>
 __device__ __forceinline__ void pack_full16(rf_glimbs __restrict__ limbs, const uint32_t (&ci)[NCH],
                                             const double (*pk)[3], double p, double pinv,
                                             double (&out)[NCH]) {
-  constexpr int SB = pack_sb<NCH, T>();
+  constexpr int SB = pack_sb<NCH, T, LIMBS>();
   static_assert(NCH % SB == 0, "sub-batches must tile the chunks");
 #pragma unroll
   for (int c0 = 0; c0 < NCH; c0 += SB) {
@@ -626,7 +627,7 @@ template <int NCH, int T>
 __device__ __forceinline__ void pack_any8(rf_glimbs __restrict__ limbs, uint32_t nlimbs,
                                           const uint32_t (&ci)[NCH], const double (*pk)[3],
                                           double p, double pinv, double (&out)[NCH]) {
-  constexpr int SB = pack_sb<NCH, T>();
+  constexpr int SB = pack_sb<NCH, T, 16>();
   static_assert(NCH % SB == 0, "sub-batches must tile the chunks");
 #pragma unroll
   for (int c0 = 0; c0 < NCH; c0 += SB) {
@@ -645,7 +646,8 @@ __device__ __forceinline__ void pack_any8(rf_glimbs __restrict__ limbs, uint32_t
 }
 
 // T > 0: chunk width fixed at compile time (t == T); T == 0: run-time t.
-template <int NCH, int T>
+// FULL_LIMBS: the limbs per sub-batch of the aligned, bounds-free path.
+template <int NCH, int T, int FULL_LIMBS = 16>
 __device__ __forceinline__ void pack_dispatch(rf_glimbs __restrict__ limbs, uint32_t nlimbs,
                                               const uint32_t (&ci)[NCH], int t, const double (*pk)[3],
                                               double p, double pinv, double (&out)[NCH]) {
@@ -657,7 +659,7 @@ __device__ __forceinline__ void pack_dispatch(rf_glimbs __restrict__ limbs, uint
     for (int c = 0; c < NCH; c++) cmax = cmax > ci[c] ? cmax : ci[c];
     const bool full = __builtin_amdgcn_ballot_w64((uint64_t)(cmax + 1) * T > nlimbs) == 0;
     if (T % 2 == 0 && full && ((uintptr_t)limbs & 15) == 0)
-      pack_full16<NCH, T>(limbs, ci, pk, p, pinv, out);
+      pack_full16<NCH, T, FULL_LIMBS>(limbs, ci, pk, p, pinv, out);
     else
       pack_any8<NCH, T>(limbs, nlimbs, ci, pk, p, pinv, out);
   }
@@ -687,7 +689,13 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
   // Pass 0 holds, per group, residues i = j 2^(LGN-R) + g: the upper half of
   // j covers positions >= N/2.  When the entry has at most N/2 chunks those
   // are zero and the first stage (twiddle 1) just copies (x, 0) -> (x, x).
-  constexpr int CB = S::E / 2 < 8 ? (S::E / 2 > 0 ? S::E / 2 : 1) : 8;  // chunks per load batch
+  // chunks per load batch (four with a run-time chunk width, whose pack
+  // keeps three accumulators per chunk across all t limbs)
+  constexpr int CB = S::E / 2 < 8 ? (S::E / 2 > 0 ? S::E / 2 : 1) : (T == 0 ? 4 : 8);
+  // chunks of at most four limbs load a whole batch (up to 32 limbs) before
+  // any is used: one memory round trip instead of two, still without spills
+  // (k_fwd<13, 4>: 121 VGPRs; 2^13 points, 1-Mbit entries: k_fwd -1.6 %)
+  constexpr int PACK_LIMBS = (T >= 1 && T <= 4) ? 32 : 16;
   if (nchunk <= (uint32_t)(S::N / 2) && Q0::GS > 1) {
 #pragma unroll
     for (int u = 0; u < Q0::NG; u++)
@@ -698,7 +706,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {  // a zero entry reads no limbs at all
-          pack_dispatch<CB, T>((rf_glimbs)tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
+          pack_dispatch<CB, T, PACK_LIMBS>((rf_glimbs)tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
@@ -721,7 +729,7 @@ __global__ __launch_bounds__(Shape<LGN>::THREADS, Shape<LGN>::MINB) void k_fwd(R
 #pragma unroll
         for (int c = 0; c < CB; c++) ci[c] = (uint32_t)fwd_index<LGN, 0>(tid + S::T * u, j0 + c);
         if (nchunk) {
-          pack_dispatch<CB, T>((rf_glimbs)tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
+          pack_dispatch<CB, T, PACK_LIMBS>((rf_glimbs)tk.limbs, tk.nlimbs, ci, tk.t, pk, p, pinv, x);
         } else {
 #pragma unroll
           for (int c = 0; c < CB; c++) x[c] = 0.0;
