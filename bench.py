@@ -190,12 +190,13 @@ def dropin_section(rf, products, seconds):
 def tree_level_section(rf, workloads, reps):
     """1024 independent 2x2 . 2x2 products of uneven sizes (entries
     log-uniform in [2^10, 2^18) bits, each matrix within 2x, random signs):
-    one prepared batch replayed from its graph, against the plain path (each
-    product alone through the one-product pipeline, and the host mpz
-    drop-in on a sample)."""
+    one prepared batch of two lanes (its length groups' kernels, each a few
+    tens of microseconds, overlap pairwise) replayed from its graph, against
+    the plain path (each product alone through the one-product pipeline, and
+    the host mpz drop-in on a sample)."""
     prods = workloads.tree_level(1024, seed=1003)
     batch = rf.DeviceBatch(prods)
-    batch.prepare()
+    batch.prepare(lanes=2)
     batch.launch(graph=True)
     rf.device_sync()
     res = batch.results()
@@ -223,7 +224,7 @@ def tree_level_section(rf, workloads, reps):
     nrules = tune.reset()
     tune.clear()
     mb = rf.DeviceBatch(prods)
-    mb.prepare()
+    mb.prepare(lanes=2)
     mb.launch(graph=True)
     rf.device_sync()
     same_model = mb.results() == res
@@ -235,7 +236,7 @@ def tree_level_section(rf, workloads, reps):
     mb.release()
     tune.reset()
     return {"workload": "1024 products 2x2 . 2x2, entries log-uniform in [2^10, 2^18) bits (matrix within 2x), "
-                        "random signs, GMP seed 1003",
+                        "random signs, GMP seed 1003; two lanes",
             "autotuned_plans": nrules,
             "cost_model_products_per_s": round(model, 1),
             "speedup_vs_cost_model": round(batched / model, 3),
