@@ -632,13 +632,23 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
     pp.launches.push_back({K_INV, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
   for (auto& kv : icol)
     pp.launches.push_back({K_INV_COL, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
-  std::vector<LaunchRec> crt_l;
+  // the CRT tasks of every prime count in one array: one CRT launch per
+  // prime count over its range, then one carry application over all of them
+  // (k_apply does not depend on the prime count; a level of mixed sizes has
+  // several counts, and each launch of a few microseconds would wait on the
+  // one before)
+  std::vector<RfCrtTask> crt_all;
+  std::vector<std::pair<int, std::pair<size_t, size_t>>> crt_groups;  // (k, (first task, tasks))
   for (auto& kv : crt) {
-    size_t off = append_tasks(pp.image, kv.second);
-    pp.launches.push_back({K_CRT, kv.first, off, (int)kv.second.size()});
-    crt_l.push_back({K_APPLY, kv.first, off, (int)kv.second.size()});
+    crt_groups.push_back({kv.first, {crt_all.size(), kv.second.size()}});
+    crt_all.insert(crt_all.end(), kv.second.begin(), kv.second.end());
   }
-  for (auto& l : crt_l) pp.launches.push_back(l);
+  if (!crt_all.empty()) {
+    const size_t off = append_tasks(pp.image, crt_all);
+    for (auto& cg : crt_groups)
+      pp.launches.push_back({K_CRT, cg.first, off + cg.second.first * sizeof(RfCrtTask), (int)cg.second.second});
+    pp.launches.push_back({K_APPLY, 0, off, (int)crt_all.size()});
+  }
   if (pp.image.size() > pp.L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
   return RF_OK;
 }
