@@ -46,7 +46,15 @@ struct Pool {
   std::vector<std::thread> th;
 
   Pool() {
-    for (int w = 0; w < kWorkers; w++) th.emplace_back([this] { loop(); });
+    // fewer workers (none: the caller does every task) when the process
+    // may not start more threads
+    for (int w = 0; w < kWorkers; w++) {
+      try {
+        th.emplace_back([this] { loop(); });
+      } catch (...) {
+        break;
+      }
+    }
   }
   ~Pool() {
     {
