@@ -87,6 +87,10 @@ __device__ __forceinline__ int lds_slot(int i) {
 //   SW = 4:       the same sum with bit 0 set to (t0 | o0) xor row parity,
 //                 at most four bases per thread;
 //   SW = 1:       slot(t) xor slot(o) (h is linear over GF(2)).
+// With constant offsets the compiler merges accesses into ds_read2_b64 /
+// ds_write2_b64 pairs; pairs whose two slots lie in different banks conflict
+// (64 conflict cycles per wave in the 2^13-point kernels, against 80 LDS
+// instructions and no conflicts unmerged, which measured no faster).
 template <int SW>
 __device__ __forceinline__ double* lds_at(double* sh, int t, int o) {
   if constexpr (SW == 1) {
