@@ -66,6 +66,12 @@ int rf_stage_mod_reciprocal(const uint64_t* d, uint64_t dn, uint64_t max_bits, u
    no GPU work, usable before hw_mpz_setup. */
 int rf_stage_tc_to_signmag(const uint64_t* tc, uint64_t rn, uint64_t* dst, uint64_t cap, long* nsigned);
 
+/* The library's segmented host copy (operands into the staging buffer,
+   product-tree leaves and roots): dst[s][0, n[s]) = src[s][0, n[s]) for
+   every segment, in 64 KiB pieces of the segments' concatenation over the
+   copy threads.  Host only: no GPU work, usable before hw_mpz_setup. */
+int rf_stage_copy_segments(void* const* dst, const void* const* src, const uint64_t* n, int nseg);
+
 /* Root of order 2^lgN used for prime index q. */
 uint64_t rf_root(int q, int lgN);
 

@@ -60,6 +60,11 @@ struct Ctx {
   char* h_stage = nullptr;
   char* d_stage = nullptr;  // the staging buffer's device address
   size_t stage_cap = 0;
+  // Device blocks of a product tree (its inputs, each level's outputs, its
+  // root), kept from one tree to the next so that a tree of the same shape
+  // allocates nothing: (pointer, bytes), handed out in order from tree_next.
+  std::vector<std::pair<uint64_t*, size_t>> tree_blk;
+  size_t tree_next = 0;
   // profiling
   bool prof = false;
   struct Ev {
@@ -200,6 +205,7 @@ void clear_locked() {
     hipFree(g_twi_long.exchange(nullptr));
   }
   if (g.d_arena) hipFree(g.d_arena);
+  for (auto& b : g.tree_blk) hipFree(b.first);
   if (g.h_stage) hipHostFree(g.h_stage);
   if (g.foreign_ev) hipEventDestroy(g.foreign_ev);
   hipStreamDestroy(g.stream);
@@ -486,7 +492,7 @@ void crt_tasks(const RfPlan& pl, uint64_t coef_off, uint64_t* out, uint32_t& npr
   nblk_total += nblk;
 }
 
-Layout layout_of(std::vector<Product>& prods) {
+Layout layout_of(std::vector<Product>& prods, TaskGroups* groups) {
   Layout L;
   // task counts per launch group, as build_pipeline groups them
   std::map<int, size_t> fwd_n, inv_n, col_n, crt_n;
@@ -525,6 +531,13 @@ Layout layout_of(std::vector<Product>& prods) {
   L.tasks_b = tb;
   L.pre_b = align256((size_t)L.nblk * RF_CRT_THREADS);
   L.fun_b = align256(L.nblk);
+  if (groups) {
+    groups->fwd.swap(fwd_n);
+    groups->inv.swap(inv_n);
+    groups->col.swap(col_n);
+    groups->crt.swap(crt_n);
+    groups->row = row_n;
+  }
   return L;
 }
 
@@ -550,16 +563,57 @@ size_t append_tasks(std::vector<char>& img, const std::vector<T>& v) {
   return off;
 }
 
+// Every task is written once, straight into the image: the groups' sizes
+// (from the layout's counts) fix each group's place, in launch order (every
+// forward pass, column passes before the row pass, then every inverse pass,
+// row passes before the column passes, CRT, carry application), and a
+// cursor per group takes the products' tasks in product order.
 int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
-  pp.L = layout_of(prods);
-  pp.image.clear();
+  TaskGroups tg;
+  pp.L = layout_of(prods, &tg);
   pp.launches.clear();
-  std::map<int, std::vector<RfFwdTask>> fwd;
-  std::map<int, std::vector<RfInvTask>> inv;
-  std::map<int, std::vector<RfRowTask>> icol;
-  std::vector<RfRowTask> frow;
-  std::map<int, std::vector<RfCrtTask>> crt;
+  size_t off = 0;
+  std::map<int, size_t> fwd_at, inv_at, col_at;  // next task's byte offset, per group
+  for (auto& kv : tg.fwd) {
+    pp.launches.push_back({(kv.first >> 12) ? K_FWD_COL : K_FWD, kv.first & 4095, off, (int)kv.second});
+    fwd_at[kv.first] = off;
+    off = align256(off + kv.second * sizeof(RfFwdTask));
+  }
+  size_t row_at = off;
+  if (tg.row) {
+    pp.launches.push_back({K_FWD_ROW, RF_LGN_ROW, off, (int)tg.row});
+    off = align256(off + tg.row * sizeof(RfRowTask));
+  }
+  for (auto& kv : tg.inv) {
+    pp.launches.push_back({K_INV, kv.first, off, (int)kv.second});
+    inv_at[kv.first] = off;
+    off = align256(off + kv.second * sizeof(RfInvTask));
+  }
+  for (auto& kv : tg.col) {
+    pp.launches.push_back({K_INV_COL, kv.first, off, (int)kv.second});
+    col_at[kv.first] = off;
+    off = align256(off + kv.second * sizeof(RfRowTask));
+  }
+  // the CRT tasks of every prime count in one array: one CRT launch per
+  // prime count over its range, then one carry application over all of them
+  // (k_apply does not depend on the prime count; a level of mixed sizes has
+  // several counts, and each launch of a few microseconds would wait on the
+  // one before)
+  std::map<int, size_t> crt_at;
+  const size_t crt0 = off;
+  size_t ncrt = 0;
+  for (auto& kv : tg.crt) {
+    pp.launches.push_back({K_CRT, kv.first, off, (int)kv.second});
+    crt_at[kv.first] = off;
+    off += kv.second * sizeof(RfCrtTask);
+    ncrt += kv.second;
+  }
+  if (ncrt) pp.launches.push_back({K_APPLY, 0, crt0, (int)ncrt});
+  if (off > pp.L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
+  pp.image.resize(off);
+  char* img = pp.image.data();
   uint32_t nblk_total = 0, npre = 0;
+  std::vector<RfCrtTask> tv;
   for (auto& P : prods) {
     const RfPlan& pl = P.plan;
     if (pl.k == 0) continue;
@@ -576,10 +630,11 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
       return P.left ? ((uint64_t)poly * k + q) * N : spec_of(poly, q);
     };
     const double* abase = P.left ? P.left->spec : nullptr;
+    size_t& fa = fwd_at[fwd_key(pl)];
     for (int e = P.left ? rd : 0; e < r * d + d * d; e++) {
       const Entry& E = e < r * d ? P.A[e] : P.B[e - r * d];
       const bool isB = e >= r * d;
-      for (int q = 0; q < k; q++) {
+      for (int q = 0; q < k; q++, fa += sizeof(RfFwdTask)) {
         RfFwdTask t{};
         t.limbs = E.dptr;
         t.spec_off = spec_of(e, q);
@@ -589,7 +644,7 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
         t.q = (uint16_t)q;
         t.t = (uint8_t)pl.t;
         t.scale = isB ? (uint8_t)pl.lgN : 0;
-        fwd[fwd_key(pl)].push_back(t);
+        memcpy(img + fa, &t, sizeof t);
       }
     }
     // Longer transforms: 2^(lgN-13) rows of 2^13 residues each.
@@ -599,12 +654,17 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
     if (split)
       for (int e = P.left ? rd : 0; e < r * d + d * d; e++)
         for (int q = 0; q < k; q++)
-          for (uint32_t h = 0; h < rows; h++) frow.push_back(RfRowTask{spec_of(e, q) + h * rowlen, h, (uint16_t)q, (uint16_t)pl.lgN});
+          for (uint32_t h = 0; h < rows; h++, row_at += sizeof(RfRowTask)) {
+            const RfRowTask t{spec_of(e, q) + h * rowlen, h, (uint16_t)q, (uint16_t)pl.lgN};
+            memcpy(img + row_at, &t, sizeof t);
+          }
+    size_t& ia = inv_at[split ? RF_LGN_ROW : pl.lgN];
+    size_t* ca = split ? &col_at[pl.lgN] : nullptr;
     for (int q = 0; q < k; q++)
       for (int i = 0; i < r; i++)
         for (int j = 0; j < d; j++) {
           const uint64_t out_off = P.coef_off + ((uint64_t)(i * d + j) * k + q) * N;
-          for (uint32_t h = 0; h < rows; h++) {
+          for (uint32_t h = 0; h < rows; h++, ia += sizeof(RfInvTask)) {
             RfInvTask t{};
             t.a_off = aspec_of(i * d, q) + h * rowlen;
             t.a_stride = (uint32_t)(k * N);
@@ -614,42 +674,22 @@ int build_pipeline(std::vector<Product>& prods, Pipeline& pp) {
             t.q = (uint16_t)q;
             t.d = (uint16_t)d;
             t.a_base = abase;  // a resident operand's spectra, in the same launch as the rest
-            inv[split ? RF_LGN_ROW : pl.lgN].push_back(t);
+            memcpy(img + ia, &t, sizeof t);
           }
-          if (split) icol[pl.lgN].push_back(RfRowTask{out_off, 0, (uint16_t)q, (uint16_t)pl.lgN});
+          if (ca) {
+            const RfRowTask t{out_off, 0, (uint16_t)q, (uint16_t)pl.lgN};
+            memcpy(img + *ca, &t, sizeof t);
+            *ca += sizeof t;
+          }
         }
-    for (int e = 0; e < r * d; e++)
-      crt_tasks(pl, P.coef_off + (uint64_t)e * k * N, P.out[e], npre, nblk_total, crt[k]);
+    size_t& ta = crt_at[k];
+    for (int e = 0; e < r * d; e++) {
+      tv.clear();
+      crt_tasks(pl, P.coef_off + (uint64_t)e * k * N, P.out[e], npre, nblk_total, tv);
+      memcpy(img + ta, tv.data(), tv.size() * sizeof(RfCrtTask));
+      ta += tv.size() * sizeof(RfCrtTask);
+    }
   }
-  // Stream order: every forward pass (column passes before the row pass),
-  // then every inverse pass (row passes before the column passes), CRT,
-  // carry application.
-  for (auto& kv : fwd)
-    pp.launches.push_back({(kv.first >> 12) ? K_FWD_COL : K_FWD, kv.first & 4095,
-                           append_tasks(pp.image, kv.second), (int)kv.second.size()});
-  if (!frow.empty()) pp.launches.push_back({K_FWD_ROW, RF_LGN_ROW, append_tasks(pp.image, frow), (int)frow.size()});
-  for (auto& kv : inv)
-    pp.launches.push_back({K_INV, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
-  for (auto& kv : icol)
-    pp.launches.push_back({K_INV_COL, kv.first, append_tasks(pp.image, kv.second), (int)kv.second.size()});
-  // the CRT tasks of every prime count in one array: one CRT launch per
-  // prime count over its range, then one carry application over all of them
-  // (k_apply does not depend on the prime count; a level of mixed sizes has
-  // several counts, and each launch of a few microseconds would wait on the
-  // one before)
-  std::vector<RfCrtTask> crt_all;
-  std::vector<std::pair<int, std::pair<size_t, size_t>>> crt_groups;  // (k, (first task, tasks))
-  for (auto& kv : crt) {
-    crt_groups.push_back({kv.first, {crt_all.size(), kv.second.size()}});
-    crt_all.insert(crt_all.end(), kv.second.begin(), kv.second.end());
-  }
-  if (!crt_all.empty()) {
-    const size_t off = append_tasks(pp.image, crt_all);
-    for (auto& cg : crt_groups)
-      pp.launches.push_back({K_CRT, cg.first, off + cg.second.first * sizeof(RfCrtTask), (int)cg.second.second});
-    pp.launches.push_back({K_APPLY, 0, off, (int)crt_all.size()});
-  }
-  if (pp.image.size() > pp.L.tasks_b) return fail(RF_ERR_ARG, "internal: task tables exceed their layout");
   return RF_OK;
 }
 
@@ -1075,6 +1115,14 @@ int rf_stage_tc_to_signmag(const uint64_t* tc, uint64_t rn, uint64_t* dst, uint6
   v[0].dst = dst;
   tc_convert_many(v);
   *nsigned = v[0].neg ? -(long)v[0].n : (long)v[0].n;
+  return RF_OK;
+}
+
+int rf_stage_copy_segments(void* const* dst, const void* const* src, const uint64_t* n, int nseg) {
+  if (nseg < 0 || (nseg && (!dst || !src || !n))) return fail(RF_ERR_ARG, "rf_stage_copy_segments: bad arguments");
+  std::vector<CopySeg> segs(nseg);
+  for (int s = 0; s < nseg; s++) segs[s] = CopySeg{dst[s], src[s], (size_t)n[s]};
+  par_copy(segs.data(), nseg);
   return RF_OK;
 }
 
@@ -1575,13 +1623,47 @@ namespace {
 // A d x d matrix of sign/magnitude entries in device memory.
 using Mat = std::vector<Entry>;
 
+// The next device block of the current tree, of at least `bytes`: the block
+// an earlier tree took at the same point when it is large enough, else a new
+// one in its place (nothing of the current tree lives there yet).
+int tree_block(size_t bytes, uint64_t** out) {
+  *out = nullptr;
+  if (bytes == 0) return RF_OK;
+  if (g.tree_next < g.tree_blk.size() && g.tree_blk[g.tree_next].second >= bytes) {
+    *out = g.tree_blk[g.tree_next++].first;
+    return RF_OK;
+  }
+  const size_t cap = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+  if (g.tree_next < g.tree_blk.size()) {
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipFree(g.tree_blk[g.tree_next].first));
+    g.tree_blk[g.tree_next] = {nullptr, 0};
+  } else {
+    g.tree_blk.push_back({nullptr, 0});
+  }
+  uint64_t* p = nullptr;
+  hipError_t e = hipMalloc((void**)&p, cap);
+  if (e != hipSuccess) return fail(RF_ERR_NOMEM, "product tree block of %zu bytes: %s", cap, hipGetErrorString(e));
+  g.tree_blk[g.tree_next++] = {p, cap};
+  *out = p;
+  return RF_OK;
+}
+
+// A d x d matrix of sign/magnitude entries in device memory.
+using Mat = std::vector<Entry>;
+
 // One level of a product tree: pairs (M_2i, M_2i+1) multiplied in one
 // pipeline, outputs converted in place to sign and magnitude; an odd last
-// matrix is carried up.  The level's outputs live in *buf (allocated here).
-int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next, uint64_t** buf) {
+// matrix is carried up.  The level's outputs go to the tree's next block.
+// One upload (the task image and the conversion tasks, through the staging
+// buffer) and one wait: the conversion kernels follow the pipeline on the
+// stream, and the signs and lengths come back with them.
+// Device arena: [task image | conversion tasks | metadata | pipeline scratch].
+int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next) {
   const int np = (int)cur.size() / 2;
   std::vector<Product> ps(np);
   size_t total = 0;
+  uint64_t max_rn = 0;
   std::vector<size_t> base(np);
   for (int i = 0; i < np; i++) {
     Product& P = ps[i];
@@ -1590,39 +1672,48 @@ int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next, uint6
     P.B = cur[2 * i + 1];
     int rc = plan_product(P);
     if (rc) return rc;
+    const uint64_t rn = P.plan.k ? P.plan.rn : 0;
     base[i] = total;
-    total += (size_t)d * d * (P.plan.k ? P.plan.rn : 0);
+    total += (size_t)d * d * rn;
+    max_rn = std::max(max_rn, rn);
   }
-  *buf = nullptr;
-  if (total) HIPCHK(hipMalloc((void**)buf, total * 8));
-  for (int i = 0; i < np; i++)
-    for (int e = 0; e < d * d; e++)
-      ps[i].out.push_back(ps[i].plan.k ? *buf + base[i] + (size_t)e * ps[i].plan.rn : nullptr);
-  const Layout L = layout_of(ps);
-  const size_t conv_b = align256((size_t)np * d * d * sizeof(RfConvTask));
-  const size_t meta_b = align256((size_t)np * d * d * sizeof(RfEntryMeta));
-  int rc = ensure_arena(L.total() + conv_b + meta_b);
+  uint64_t* buf = nullptr;
+  int rc = tree_block(total * 8, &buf);
   if (rc) return rc;
-  rc = run_pipeline(ps, L, conv_b + meta_b, g.stream);
-  if (rc) return rc;
-  // run_pipeline synchronised the stream before staging its tables, and the
-  // staging buffer is free again once the stream drains
+  const size_t nconv = (size_t)np * d * d;
   std::vector<RfConvTask> conv;
+  conv.reserve(nconv);
   for (int i = 0; i < np; i++)
-    for (int e = 0; e < d * d; e++) conv.push_back(RfConvTask{ps[i].out[e], ps[i].plan.k ? ps[i].plan.rn : 0});
-  HIPCHK(hipStreamSynchronize(g.stream));
-  rc = ensure_stage(conv_b + meta_b);
+    for (int e = 0; e < d * d; e++) {
+      const uint64_t rn = ps[i].plan.k ? ps[i].plan.rn : 0;
+      ps[i].out.push_back(rn ? buf + base[i] + (size_t)e * rn : nullptr);
+      conv.push_back(RfConvTask{ps[i].out.back(), rn});
+    }
+  Pipeline pp;
+  rc = build_pipeline(ps, pp);
   if (rc) return rc;
-  RfConvTask* d_conv = (RfConvTask*)g.d_arena;
-  RfEntryMeta* d_meta = (RfEntryMeta*)(g.d_arena + conv_b);
-  memcpy(g.h_stage, conv.data(), conv.size() * sizeof(RfConvTask));
-  HIPCHK(hipMemcpyAsync(d_conv, g.h_stage, conv.size() * sizeof(RfConvTask), hipMemcpyHostToDevice, g.stream));
-  HIPCHK(rf_launch_signmag(d_conv, (int)conv.size(), d_meta, g.stream));
-  HIPCHK(hipMemcpyAsync(g.h_stage + conv_b, d_meta, conv.size() * sizeof(RfEntryMeta), hipMemcpyDeviceToHost,
+  const size_t in_off = align256(pp.L.tasks_b);
+  const size_t conv_b = align256(nconv * sizeof(RfConvTask));
+  const size_t meta_b = align256(nconv * sizeof(RfEntryMeta));
+  rc = ensure_arena(in_off + conv_b + meta_b + pp.L.total());
+  if (rc) return rc;
+  rc = ensure_stage(in_off + conv_b + meta_b);
+  if (rc) return rc;
+  // earlier copies out of the staging buffer (the tree's leaves) must be done
+  HIPCHK(hipStreamSynchronize(g.stream));
+  par_memcpy(g.h_stage, pp.image.data(), pp.image.size());
+  memcpy(g.h_stage + in_off, conv.data(), nconv * sizeof(RfConvTask));
+  HIPCHK(hipMemcpyAsync(g.d_arena, g.h_stage, in_off + nconv * sizeof(RfConvTask), hipMemcpyHostToDevice,
                         g.stream));
+  rc = launch_pipeline_tasks(pp, g.d_arena + in_off + conv_b + meta_b, g.d_arena, g.stream, g.prof);
+  if (rc) return rc;
+  RfConvTask* d_conv = (RfConvTask*)(g.d_arena + in_off);
+  RfEntryMeta* d_meta = (RfEntryMeta*)(g.d_arena + in_off + conv_b);
+  HIPCHK(rf_launch_signmag(d_conv, (int)nconv, d_meta, max_rn, g.stream));
+  const RfEntryMeta* meta = (const RfEntryMeta*)(g.h_stage + in_off + conv_b);
+  HIPCHK(hipMemcpyAsync((void*)meta, d_meta, nconv * sizeof(RfEntryMeta), hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
   if (g.prof) prof_collect();
-  const RfEntryMeta* meta = (const RfEntryMeta*)(g.h_stage + conv_b);
   next.clear();
   for (int i = 0; i < np; i++) {
     Mat M(d * d);
@@ -1640,15 +1731,14 @@ int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next, uint6
 }
 
 // The whole tree; the root's entries (sign/magnitude, device) are copied to
-// out + e * out_cap, their signs and lengths to sign[e], nlimbs[e].
+// out + e * out_cap, their signs and lengths to sign[e], nlimbs[e].  Every
+// level's block stays the tree's until the end: a carried matrix may point
+// into any earlier level.
 int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, int* sign, uint64_t* nlimbs) {
-  std::vector<uint64_t*> bufs;
   int rc = RF_OK;
   while (cur.size() > 1 && !rc) {
     std::vector<Mat> next;
-    uint64_t* buf = nullptr;
-    rc = tree_level(cur, d, next, &buf);
-    if (buf) bufs.push_back(buf);  // a carried matrix may point into any earlier level
+    rc = tree_level(cur, d, next);
     if (!rc) cur.swap(next);
   }
   for (int e = 0; e < d * d && !rc; e++) {
@@ -1666,7 +1756,6 @@ int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, i
     hipError_t x = hipStreamSynchronize(g.stream);
     if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree: %s", hipGetErrorString(x));
   }
-  for (uint64_t* b : bufs) (void)hipFree(b);
   return rc;
 }
 }  // namespace
@@ -1688,6 +1777,7 @@ int rf_tree_product_device(const rf_dev_entry* mats, int n, int d, uint64_t* out
       E.sign = x.nlimbs ? (x.sign > 0 ? 1 : (x.sign < 0 ? -1 : 0)) : 0;
       E.bits = E.sign ? 64 * x.nlimbs : 0;
     }
+  g.tree_next = 0;
   return tree_product(cur, d, out, out_cap, sign, nlimbs);
 }
 
@@ -1711,49 +1801,44 @@ int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d) {
   while ((1 << cl) < d) cl++;
   for (int m = 0; m < n; m++) bound += mx[m];
   bound += ((uint64_t)(n - 1) * cl + 63) / 64 + 1;
+  // the tree's blocks: its leaves, its root, then a block per level
+  g.tree_next = 0;
   uint64_t *d_in = nullptr, *d_out = nullptr;
-  HIPCHK(hipMalloc((void**)&d_in, std::max<uint64_t>(total, 1) * 8));
-  hipError_t x = hipMalloc((void**)&d_out, (size_t)d * d * bound * 8);
-  if (x != hipSuccess) {
-    (void)hipFree(d_in);
-    return fail(RF_ERR_NOMEM, "tree output: %s", hipGetErrorString(x));
-  }
-  int rc = ensure_stage(std::max<uint64_t>(total * 8, (size_t)d * d * bound * 8));
+  int rc = tree_block(total * 8, &d_in);
+  if (!rc) rc = tree_block((size_t)d * d * bound * 8, &d_out);
+  if (!rc) rc = ensure_stage(std::max<uint64_t>(total * 8, (size_t)d * d * bound * 8));
+  if (rc) return rc;
   std::vector<Mat> cur(n, Mat(d * d));
-  if (!rc) {
-    HIPCHK(hipStreamSynchronize(g.stream));
-    for (size_t i = 0; i < ne; i++) {
-      const uint64_t nl = mpz_size(mats[i]);
-      if (nl) memcpy((uint64_t*)g.h_stage + off[i], mats[i]->_mp_d, nl * 8);
-      Entry& E = cur[i / (d * d)][i % (d * d)];
-      E.dptr = d_in + off[i];
-      E.nlimbs = nl;
-      E.sign = mpz_sgn(mats[i]);
-      E.bits = E.sign ? mpz_sizeinbase(mats[i], 2) : 0;
-    }
-    x = hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream);
-    if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree upload: %s", hipGetErrorString(x));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  std::vector<CopySeg> segs;
+  for (size_t i = 0; i < ne; i++) {
+    const uint64_t nl = mpz_size(mats[i]);
+    if (nl) segs.push_back({(uint64_t*)g.h_stage + off[i], mats[i]->_mp_d, nl * 8});
+    Entry& E = cur[i / (d * d)][i % (d * d)];
+    E.dptr = d_in + off[i];
+    E.nlimbs = nl;
+    E.sign = mpz_sgn(mats[i]);
+    E.bits = E.sign ? mpz_sizeinbase(mats[i], 2) : 0;
   }
+  par_copy(segs.data(), (int)segs.size());
+  if (total) HIPCHK(hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream));
   std::vector<int> sg(d * d);
   std::vector<uint64_t> nls(d * d);
-  if (!rc) rc = tree_product(cur, d, d_out, bound, sg.data(), nls.data());
-  if (!rc) {
-    x = hipMemcpy(g.h_stage, d_out, (size_t)d * d * bound * 8, hipMemcpyDeviceToHost);
-    if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree download: %s", hipGetErrorString(x));
-  }
-  if (!rc)
-    for (int e = 0; e < d * d; e++) {
-      if (!sg[e]) {
-        mpz_set_ui(C[e], 0);
-        continue;
-      }
-      _mpz_realloc(C[e], (mp_size_t)nls[e]);
-      memcpy(C[e]->_mp_d, (uint64_t*)g.h_stage + (size_t)e * bound, nls[e] * 8);
-      C[e]->_mp_size = sg[e] < 0 ? -(int)nls[e] : (int)nls[e];
+  rc = tree_product(cur, d, d_out, bound, sg.data(), nls.data());
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(g.h_stage, d_out, (size_t)d * d * bound * 8, hipMemcpyDeviceToHost));
+  segs.clear();
+  for (int e = 0; e < d * d; e++) {
+    if (!sg[e]) {
+      mpz_set_ui(C[e], 0);
+      continue;
     }
-  (void)hipFree(d_in);
-  (void)hipFree(d_out);
-  return rc;
+    _mpz_realloc(C[e], (mp_size_t)nls[e]);
+    segs.push_back({C[e]->_mp_d, (uint64_t*)g.h_stage + (size_t)e * bound, nls[e] * 8});
+    C[e]->_mp_size = sg[e] < 0 ? -(int)nls[e] : (int)nls[e];
+  }
+  par_copy(segs.data(), (int)segs.size());
+  return RF_OK;
 }
 
 int rf_plan_query(uint64_t la_bits, uint64_t lb_bits, int r, int d, int* t, int* lgN, int* k) {

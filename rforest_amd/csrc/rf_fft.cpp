@@ -329,7 +329,7 @@ int rf_fft_inverse_device(const rf_fft* op, uint64_t* out, uint64_t cap, int* si
   memcpy(h_base, &ct, sizeof ct);
   HIPCHK(hipMemcpyAsync(d_base, h_base, sizeof ct, hipMemcpyHostToDevice, st));
   RfEntryMeta* d_meta = (RfEntryMeta*)(d_base + conv_b);
-  HIPCHK(rf_launch_signmag((const RfConvTask*)d_base, 1, d_meta, st));
+  HIPCHK(rf_launch_signmag((const RfConvTask*)d_base, 1, d_meta, rn, st));
   HIPCHK(hipMemcpyAsync(h_base + conv_b, d_meta, sizeof(RfEntryMeta), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   RfEntryMeta m;

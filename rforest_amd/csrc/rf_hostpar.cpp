@@ -14,6 +14,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -161,29 +162,35 @@ void copy_piece(void* vp, int i) {
   memcpy(j.dst + off, j.src + off, len);
 }
 
+// The segments laid end to end: piece i copies bytes [i, i + 1) * kPiece of
+// the concatenation, whatever segments (many small ones, or part of a large
+// one) that range covers.
 struct SegJob {
   const CopySeg* segs;
-  const size_t* first;  // first piece of each segment, and the total at [nseg]
+  const size_t* start;  // byte offset of each segment, and the total at [nseg]
   int nseg;
 };
 
 void seg_piece(void* vp, int i) {
   const SegJob& j = *(const SegJob*)vp;
-  int s = 0;
-  while (j.first[s + 1] <= (size_t)i) s++;
-  const CopySeg& g = j.segs[s];
-  const size_t off = ((size_t)i - j.first[s]) * kPiece;
-  const size_t len = g.n - off < kPiece ? g.n - off : kPiece;
-  memcpy((char*)g.dst + off, (const char*)g.src + off, len);
+  const size_t lo = (size_t)i * kPiece;
+  const size_t hi = lo + kPiece < j.start[j.nseg] ? lo + kPiece : j.start[j.nseg];
+  int s = (int)(std::upper_bound(j.start, j.start + j.nseg + 1, lo) - j.start) - 1;
+  for (; s < j.nseg && j.start[s] < hi; s++) {
+    const size_t a = (lo > j.start[s] ? lo : j.start[s]) - j.start[s];
+    const size_t b = (hi < j.start[s + 1] ? hi : j.start[s + 1]) - j.start[s];
+    if (b > a) memcpy((char*)j.segs[s].dst + a, (const char*)j.segs[s].src + a, b - a);
+  }
 }
 
 }  // namespace
 
 void par_copy(const CopySeg* segs, int nseg) {
-  std::vector<size_t> first(nseg + 1, 0);
-  for (int s = 0; s < nseg; s++) first[s + 1] = first[s] + (segs[s].n + kPiece - 1) / kPiece;
-  SegJob j{segs, first.data(), nseg};
-  par_for((int)first[nseg], seg_piece, &j);
+  if (nseg <= 0) return;
+  std::vector<size_t> start(nseg + 1, 0);
+  for (int s = 0; s < nseg; s++) start[s + 1] = start[s] + segs[s].n;
+  SegJob j{segs, start.data(), nseg};
+  par_for((int)((start[nseg] + kPiece - 1) / kPiece), seg_piece, &j);
 }
 
 void par_for(int n, void (*fn)(void* ctx, int i), void* ctx) {

@@ -18,7 +18,8 @@ void par_wake();
 // dst[0, n) = src[0, n) in pieces on the pool (n in bytes).
 void par_memcpy(void* dst, const void* src, size_t n);
 
-// Every segment's copy, split into pieces over the pool.
+// Every segment's copy, in pieces of the segments' concatenation over the
+// pool (small segments share a piece, large ones span several).
 struct CopySeg {
   void* dst;
   const void* src;

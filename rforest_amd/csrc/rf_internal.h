@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <memory>
 #include <mutex>
 #include <shared_mutex>
@@ -139,7 +140,13 @@ int engine_stage(size_t bytes, char** host);
 // when the plan needs them (false: a plan that needs missing tables fails
 // with RF_ERR_CAPACITY instead, for callers that may not allocate).
 int plan_product(Product& P, bool make_tables = true);
-Layout layout_of(std::vector<Product>& prods);
+// Task counts per launch group, keyed as build_pipeline keys its launches
+// (forward: fwd_key; inverse and inverse columns: lgN; CRT: primes).
+struct TaskGroups {
+  std::map<int, size_t> fwd, inv, col, crt;
+  size_t row = 0;
+};
+Layout layout_of(std::vector<Product>& prods, TaskGroups* groups = nullptr);
 int build_pipeline(std::vector<Product>& prods, Pipeline& pp);
 ScratchPtrs scratch_ptrs(char* base, const Layout& L);
 // Enqueues every kernel of a built pipeline whose task image already sits in

@@ -50,6 +50,9 @@ struct RfBlock {
 
 // Two's complement -> sign and magnitude, in place, one entry per task; the
 // entry's sign and magnitude length (leading zero limbs dropped) go to meta.
+// Two launches: a scan per entry (sign, length, lowest nonzero limb), then
+// the negation of the negative entries in tiles of RF_FLIP_LIMBS limbs over
+// as many workgroups as the longest entry (max_n limbs) needs.
 struct RfConvTask {
   uint64_t* limbs;
   uint64_t n;  // limbs
@@ -57,8 +60,11 @@ struct RfConvTask {
 struct RfEntryMeta {
   int32_t sign;
   uint32_t nlimbs;
+  uint32_t l0;  // lowest nonzero limb of a negative entry
+  uint32_t pad;
 };
-hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, hipStream_t st);
+#define RF_FLIP_LIMBS 2048
+hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, uint64_t max_n, hipStream_t st);
 
 // Remainder fix-up, one task per remainder X mod D (D of n limbs, top limb
 // nonzero): given QD = Q D for a quotient estimate with X - Q D in [0, 3D),

@@ -5,8 +5,8 @@
 //   k_checksum  64-bit checksums of result limbs, summed per request
 //   k_fill      writes a constant over a u64 range (clears checksum slots
 //               without a memset node, safe under graph capture)
-//   k_signmag   two's complement results -> sign and magnitude in place (a
-//               product tree's level outputs become the next level's inputs)
+//   k_sm_scan,  two's complement results -> sign and magnitude in place (a
+//   k_sm_flip   product tree's level outputs become the next level's inputs)
 //   k_mod_fix   the last step of a remainder X mod D: X - Q D and its
 //               corrections by multi-limb subtraction
 //
@@ -65,50 +65,76 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_checksum(const RfSpan* __re
   }
 }
 
-// One workgroup per entry: sign from the top bit; a negative entry becomes
-// its magnitude ~x + 1, which is 0 below the lowest nonzero limb l0, -x[l0]
-// at l0 and ~x above, so every limb is independent once l0 is known; then the
-// highest nonzero limb gives the length.
-__global__ __launch_bounds__(RF_UTIL_THREADS) void k_signmag(const RfConvTask* __restrict__ tasks, int ntasks,
+// The scan, one workgroup per entry: the sign is the top bit; a negative
+// entry's magnitude ~x + 1 is 0 below its lowest nonzero limb l0, -x[l0] at
+// l0 and ~x above, so its length is found without writing: limb i > l0 of
+// the magnitude is zero exactly when x[i] is all ones.  Both ends are found
+// by early-exit sweeps of RF_UTIL_THREADS limbs (one each, mostly), from the
+// bottom for l0 and from the top for the length.
+__global__ __launch_bounds__(RF_UTIL_THREADS) void k_sm_scan(const RfConvTask* __restrict__ tasks,
                                                             RfEntryMeta* __restrict__ meta) {
-  __shared__ unsigned long long lo_sh;
-  __shared__ long long hi_sh;
-  if ((int)blockIdx.x >= ntasks) return;
+  __shared__ unsigned long long lo_sh, hi_sh;
   const RfConvTask tk = tasks[blockIdx.x];
   const uint64_t n = tk.n;
+  const uint64_t* __restrict__ x = tk.limbs;
+  const bool neg = n > 0 && (x[n - 1] >> 63) != 0;
   if (threadIdx.x == 0) {
     lo_sh = n;
-    hi_sh = -1;
+    hi_sh = 0;
   }
   __syncthreads();
-  const bool neg = n > 0 && (tk.limbs[n - 1] >> 63) != 0;
-  if (neg) {
-    unsigned long long lo = n;
-    for (uint64_t i = threadIdx.x; i < n; i += RF_UTIL_THREADS)
-      if (tk.limbs[i] != 0) {
-        lo = i;
+  uint64_t l0 = 0;
+  if (neg) {  // a nonzero limb exists: the top one
+    for (uint64_t b = 0; b < n; b += RF_UTIL_THREADS) {
+      const uint64_t i = b + threadIdx.x;
+      const bool nz = i < n && x[i] != 0;
+      if (__syncthreads_or(nz)) {
+        if (nz) atomicMin(&lo_sh, (unsigned long long)i);
+        __syncthreads();
         break;
       }
-    atomicMin(&lo_sh, lo);
-    __syncthreads();
-    const uint64_t l0 = lo_sh;
-    for (uint64_t i = threadIdx.x; i < n; i += RF_UTIL_THREADS) {
-      const uint64_t x = tk.limbs[i];
-      if (i == l0)
-        tk.limbs[i] = ~x + 1;
-      else if (i > l0)
-        tk.limbs[i] = ~x;
     }
-    __syncthreads();
+    l0 = lo_sh;
   }
-  long long hi = -1;
-  for (uint64_t i = threadIdx.x; i < n; i += RF_UTIL_THREADS)
-    if (tk.limbs[i] != 0) hi = (long long)i;
-  atomicMax(&hi_sh, hi);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t len = (uint32_t)(hi_sh + 1);
-    meta[blockIdx.x] = RfEntryMeta{len == 0 ? 0 : (neg ? -1 : 1), len};
+  const uint64_t fill = neg ? ~0ull : 0ull;
+  uint64_t len = 0;
+  // sweeps of [top - RF_UTIL_THREADS, top) down to l0; a negative entry's
+  // sweep ends at l0 at the latest
+  for (uint64_t top = n; top > l0; top = top - l0 > RF_UTIL_THREADS ? top - RF_UTIL_THREADS : l0) {
+    const uint64_t i = top - 1 - threadIdx.x;
+    const bool sig = threadIdx.x < top - l0 && (x[i] != fill || (neg && i == l0));
+    if (__syncthreads_or(sig)) {
+      if (sig) atomicMax(&hi_sh, (unsigned long long)(i + 1));
+      __syncthreads();
+      len = hi_sh;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) meta[blockIdx.x] = RfEntryMeta{len == 0 ? 0 : (neg ? -1 : 1), (uint32_t)len, (uint32_t)l0, 0};
+}
+
+// The negation of the negative entries: workgroup (x, y) turns limbs
+// [x, x + 1) * RF_FLIP_LIMBS of entry y into the magnitude's, from l0 up
+// (the limbs below l0 are zero in both), coalesced; loads first, then stores.
+__global__ __launch_bounds__(RF_UTIL_THREADS) void k_sm_flip(const RfConvTask* __restrict__ tasks,
+                                                            const RfEntryMeta* __restrict__ meta) {
+  const RfEntryMeta m = meta[blockIdx.y];
+  if (m.sign >= 0) return;
+  const RfConvTask tk = tasks[blockIdx.y];
+  const uint64_t t0 = (uint64_t)blockIdx.x * RF_FLIP_LIMBS;
+  if (t0 >= tk.n) return;
+  const uint64_t l0 = m.l0;
+  constexpr int U = RF_FLIP_LIMBS / RF_UTIL_THREADS;
+  uint64_t v[U];
+#pragma unroll
+  for (int k = 0; k < U; k++) {
+    const uint64_t i = t0 + (uint64_t)k * RF_UTIL_THREADS + threadIdx.x;
+    v[k] = i >= l0 && i < tk.n ? tk.limbs[i] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < U; k++) {
+    const uint64_t i = t0 + (uint64_t)k * RF_UTIL_THREADS + threadIdx.x;
+    if (i >= l0 && i < tk.n) tk.limbs[i] = i == l0 ? 0 - v[k] : ~v[k];
   }
 }
 
@@ -421,8 +447,17 @@ hipError_t rf_launch_mod_norm(const RfNormTask* tasks, int ntasks, uint64_t max_
   return hipGetLastError();
 }
 
-hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, hipStream_t st) {
+hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, uint64_t max_n,
+                             hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_signmag, dim3(ntasks), dim3(RF_UTIL_THREADS), 0, st, tasks, ntasks, meta);
+  hipLaunchKernelGGL(k_sm_scan, dim3(ntasks), dim3(RF_UTIL_THREADS), 0, st, tasks, meta);
+  if (max_n == 0) return hipGetLastError();
+  // every entry's tiles fit the grid's x extent (max_n bounds every n); the
+  // y extent is at most 65535
+  const unsigned gx = (unsigned)((max_n + RF_FLIP_LIMBS - 1) / RF_FLIP_LIMBS);
+  for (int t0 = 0; t0 < ntasks; t0 += 65535) {
+    const int nt = ntasks - t0 < 65535 ? ntasks - t0 : 65535;
+    hipLaunchKernelGGL(k_sm_flip, dim3(gx, (unsigned)nt), dim3(RF_UTIL_THREADS), 0, st, tasks + t0, meta + t0);
+  }
   return hipGetLastError();
 }

@@ -54,3 +54,80 @@ def test_tree_level_of_headline_sized_leaves():
     mats = [p[2] for p in prods] + [p[3] for p in prods[:2]]
     want = reduce(lambda X, Y: _matmul(X, Y, 2), mats)
     assert rf.tree_product(mats, 2) == want
+
+
+@pytest.mark.parametrize("case", ["low_zero_run", "low_zero_run_later_tile", "power_of_two_negated",
+                                  "small_negative_in_long_buffer", "many_tiles_both_signs"])
+def test_tree_level_conversion_edges(case):
+    # level outputs that stress the in-place conversion to sign and
+    # magnitude: the lowest nonzero limb past the first sweep of 256 limbs
+    # and in a later tile of 2048, a negative magnitude one limb long, a
+    # small negative value under hundreds of all-ones limbs, and entries of
+    # several tiles of either sign
+    rng = random.Random(sum(map(ord, case)))
+    if case == "low_zero_run":
+        d, mats = 1, [[-(1 << (64 * 300))], [1 << (64 * 100)], [3]]
+    elif case == "low_zero_run_later_tile":
+        d, mats = 1, [[-(1 << (64 * 3000 + 17))], [rng.getrandbits(700) | 1], [-(rng.getrandbits(64) | 1)]]
+    elif case == "power_of_two_negated":
+        d, mats = 1, [[-(1 << (64 * 2500))], [1], [1 << 64], [-1]]
+    elif case == "small_negative_in_long_buffer":
+        d = 2
+        x = (1 << 40000) + 7
+        mats = [[1, x, 0, 1], [1, -x - 5, 0, 1], [2, 3, 5, 7], [1, -x, 0, 1], [1, x - 1, 0, 1]]
+    else:
+        d = 2
+        mats = [[rng.getrandbits(rng.randrange(100000, 200000)) * rng.choice([-1, 1]) for _ in range(4)]
+                for _ in range(3)]
+    want = reduce(lambda X, Y: _matmul(X, Y, d), mats)
+    assert rf.tree_product(mats, d) == want
+
+
+def test_tree_product_device_entry_point():
+    # rf_tree_product_device: leaves already in device memory as sign and
+    # magnitude descriptors, root magnitudes written to a device array
+    import ctypes
+    import numpy as np
+    from rforest_amd.device import DeviceBuffer
+
+    class DevEntry(ctypes.Structure):
+        _fields_ = [("limbs", ctypes.c_void_p), ("nlimbs", ctypes.c_uint64), ("sign", ctypes.c_int32),
+                    ("reserved", ctypes.c_int32)]
+
+    rng = random.Random(77)
+    n, d = 6, 2
+    mats = [[rng.getrandbits(rng.randrange(1, 3000)) * rng.choice([-1, 1]) for _ in range(d * d)] for _ in range(n)]
+    mats[2][1] = 0
+    want = reduce(lambda X, Y: _matmul(X, Y, d), mats)
+    flat = [x for M in mats for x in M]
+    nl = [(abs(x).bit_length() + 63) // 64 for x in flat]
+    off = np.cumsum([0] + nl)
+    limbs = np.zeros(max(int(off[-1]), 1), dtype=np.uint64)
+    for i, x in enumerate(flat):
+        if nl[i]:
+            limbs[off[i]:off[i + 1]] = np.frombuffer(abs(x).to_bytes(8 * nl[i], "little"), dtype=np.uint64)
+    src = DeviceBuffer(8 * limbs.size)
+    src.upload(limbs)
+    ents = (DevEntry * len(flat))()
+    for i, x in enumerate(flat):
+        ents[i] = DevEntry(src.ptr + 8 * int(off[i]), nl[i], (x > 0) - (x < 0), 0)
+    cap = max((abs(w).bit_length() + 63) // 64 for w in want) + 1
+    out = DeviceBuffer(8 * cap * d * d)
+    sign = (ctypes.c_int * (d * d))()
+    nlimbs = (ctypes.c_uint64 * (d * d))()
+    L = rf.lib()
+    L.rf_tree_product_device.restype = ctypes.c_int
+    L.rf_tree_product_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                         ctypes.c_void_p, ctypes.c_void_p]
+    rf.check(L.rf_tree_product_device(ents, n, d, ctypes.c_void_p(out.ptr), cap, sign, nlimbs))
+    res = out.download(np.uint64, cap * d * d)
+    got = []
+    for e in range(d * d):
+        mag = int.from_bytes(res[e * cap:e * cap + nlimbs[e]].tobytes(), "little")
+        assert nlimbs[e] == (mag.bit_length() + 63) // 64  # no high zero limbs
+        got.append(sign[e] * mag)
+    assert got == want
+    short = max(nlimbs) - 1
+    assert L.rf_tree_product_device(ents, n, d, ctypes.c_void_p(out.ptr), short, sign, nlimbs) == rf.RF_ERR_CAPACITY
+    src.free()
+    out.free()
