@@ -65,52 +65,51 @@ __global__ __launch_bounds__(RF_UTIL_THREADS) void k_checksum(const RfSpan* __re
   }
 }
 
-// The scan, one workgroup per entry: the sign is the top bit; a negative
-// entry's magnitude ~x + 1 is 0 below its lowest nonzero limb l0, -x[l0] at
-// l0 and ~x above, so its length is found without writing: limb i > l0 of
-// the magnitude is zero exactly when x[i] is all ones.  Both ends are found
-// by early-exit sweeps of RF_UTIL_THREADS limbs (one each, mostly), from the
-// bottom for l0 and from the top for the length.
-__global__ __launch_bounds__(RF_UTIL_THREADS) void k_sm_scan(const RfConvTask* __restrict__ tasks,
+// The scan, one wave per entry (RF_UTIL_THREADS / 64 entries per
+// workgroup): the sign is the top bit; a negative entry's magnitude ~x + 1 is
+// 0 below its lowest nonzero limb l0, -x[l0] at l0 and ~x above, so its
+// length is found without writing: limb i > l0 of the magnitude is zero
+// exactly when x[i] is all ones.  Both ends are found by sweeps of 64 limbs
+// decided by one ballot each, from the bottom for l0 and from the top for the
+// length; the first sweep of each end is loaded together with the sign limb,
+// and a typical entry needs no other.
+__global__ __launch_bounds__(RF_UTIL_THREADS) void k_sm_scan(const RfConvTask* __restrict__ tasks, int ntasks,
                                                             RfEntryMeta* __restrict__ meta) {
-  __shared__ unsigned long long lo_sh, hi_sh;
-  const RfConvTask tk = tasks[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int e = (int)blockIdx.x * (RF_UTIL_THREADS / 64) + (int)(threadIdx.x >> 6);
+  if (e >= ntasks) return;  // the whole wave
+  const RfConvTask tk = tasks[e];
   const uint64_t n = tk.n;
   const uint64_t* __restrict__ x = tk.limbs;
-  const bool neg = n > 0 && (x[n - 1] >> 63) != 0;
-  if (threadIdx.x == 0) {
-    lo_sh = n;
-    hi_sh = 0;
-  }
-  __syncthreads();
+  const uint64_t ul = (uint64_t)lane;
+  const uint64_t top = n ? x[n - 1] : 0;
+  const uint64_t lo = ul < n ? x[ul] : 0;
+  uint64_t v = ul < n ? x[n - 1 - ul] : 0;
+  const bool neg = (top >> 63) != 0;
   uint64_t l0 = 0;
   if (neg) {  // a nonzero limb exists: the top one
-    for (uint64_t b = 0; b < n; b += RF_UTIL_THREADS) {
-      const uint64_t i = b + threadIdx.x;
-      const bool nz = i < n && x[i] != 0;
-      if (__syncthreads_or(nz)) {
-        if (nz) atomicMin(&lo_sh, (unsigned long long)i);
-        __syncthreads();
-        break;
-      }
+    uint64_t b = 0;
+    uint64_t m = __ballot(lo != 0);
+    while (m == 0) {
+      b += 64;
+      m = __ballot(b + ul < n && x[b + ul] != 0);
     }
-    l0 = lo_sh;
+    l0 = b + (uint64_t)(__ffsll((unsigned long long)m) - 1);
   }
   const uint64_t fill = neg ? ~0ull : 0ull;
   uint64_t len = 0;
-  // sweeps of [top - RF_UTIL_THREADS, top) down to l0; a negative entry's
-  // sweep ends at l0 at the latest
-  for (uint64_t top = n; top > l0; top = top - l0 > RF_UTIL_THREADS ? top - RF_UTIL_THREADS : l0) {
-    const uint64_t i = top - 1 - threadIdx.x;
-    const bool sig = threadIdx.x < top - l0 && (x[i] != fill || (neg && i == l0));
-    if (__syncthreads_or(sig)) {
-      if (sig) atomicMax(&hi_sh, (unsigned long long)(i + 1));
-      __syncthreads();
-      len = hi_sh;
+  // sweeps of [t - 64, t) down to l0 (lane i holds limb t - 1 - i); a
+  // negative entry's sweep ends at l0 at the latest
+  for (uint64_t t = n; t > l0;) {
+    const uint64_t m = __ballot(ul < t - l0 && (v != fill || (neg && t - 1 - ul == l0)));
+    if (m) {
+      len = t - (uint64_t)(__ffsll((unsigned long long)m) - 1);
       break;
     }
+    t = t - l0 > 64 ? t - 64 : l0;
+    v = ul < t - l0 ? x[t - 1 - ul] : 0;
   }
-  if (threadIdx.x == 0) meta[blockIdx.x] = RfEntryMeta{len == 0 ? 0 : (neg ? -1 : 1), (uint32_t)len, (uint32_t)l0, 0};
+  if (lane == 0) meta[e] = RfEntryMeta{len == 0 ? 0 : (neg ? -1 : 1), (uint32_t)len, (uint32_t)l0, 0};
 }
 
 // The negation of the negative entries: workgroup (x, y) turns limbs
@@ -450,7 +449,9 @@ hipError_t rf_launch_mod_norm(const RfNormTask* tasks, int ntasks, uint64_t max_
 hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, uint64_t max_n,
                              hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sm_scan, dim3(ntasks), dim3(RF_UTIL_THREADS), 0, st, tasks, meta);
+  constexpr int EPB = RF_UTIL_THREADS / 64;  // entries per workgroup
+  hipLaunchKernelGGL(k_sm_scan, dim3((unsigned)((ntasks + EPB - 1) / EPB)), dim3(RF_UTIL_THREADS), 0, st, tasks,
+                     ntasks, meta);
   if (max_n == 0) return hipGetLastError();
   // every entry's tiles fit the grid's x extent (max_n bounds every n); the
   // y extent is at most 65535
