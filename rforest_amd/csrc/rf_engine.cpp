@@ -61,8 +61,9 @@ struct Ctx {
   char* d_stage = nullptr;  // the staging buffer's device address
   size_t stage_cap = 0;
   // Device blocks of a product tree (its inputs, each level's outputs, its
-  // root), kept from one tree to the next so that a tree of the same shape
-  // allocates nothing: (pointer, bytes), handed out in order from tree_next.
+  // root), kept from one tree to the next (up to 512 MiB in all) so that a
+  // tree of the same shape allocates nothing: (pointer, bytes), handed out in
+  // order from tree_next.
   std::vector<std::pair<uint64_t*, size_t>> tree_blk;
   size_t tree_next = 0;
   // profiling
@@ -1623,6 +1624,21 @@ namespace {
 // A d x d matrix of sign/magnitude entries in device memory.
 using Mat = std::vector<Entry>;
 
+// Blocks kept after a tree, at most this many bytes in all: a larger tree's
+// allocation cost is small beside its transforms, and its blocks go back.
+constexpr size_t kTreeKeepBytes = (size_t)512 << 20;
+
+// After a tree: its blocks stay for the next one when they total at most
+// kTreeKeepBytes, else all are freed.
+void tree_blocks_trim() {
+  size_t total = 0;
+  for (auto& b : g.tree_blk) total += b.second;
+  if (total <= kTreeKeepBytes) return;
+  (void)hipStreamSynchronize(g.stream);
+  for (auto& b : g.tree_blk) (void)hipFree(b.first);
+  g.tree_blk.clear();
+}
+
 // The next device block of the current tree, of at least `bytes`: the block
 // an earlier tree took at the same point when it is large enough, else a new
 // one in its place (nothing of the current tree lives there yet).
@@ -1778,7 +1794,9 @@ int rf_tree_product_device(const rf_dev_entry* mats, int n, int d, uint64_t* out
       E.bits = E.sign ? 64 * x.nlimbs : 0;
     }
   g.tree_next = 0;
-  return tree_product(cur, d, out, out_cap, sign, nlimbs);
+  const int rc = tree_product(cur, d, out, out_cap, sign, nlimbs);
+  tree_blocks_trim();
+  return rc;
 }
 
 int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d) {
@@ -1825,8 +1843,12 @@ int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d) {
   std::vector<int> sg(d * d);
   std::vector<uint64_t> nls(d * d);
   rc = tree_product(cur, d, d_out, bound, sg.data(), nls.data());
+  if (!rc) {
+    const hipError_t x = hipMemcpy(g.h_stage, d_out, (size_t)d * d * bound * 8, hipMemcpyDeviceToHost);
+    if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree download: %s", hipGetErrorString(x));
+  }
+  tree_blocks_trim();
   if (rc) return rc;
-  HIPCHK(hipMemcpy(g.h_stage, d_out, (size_t)d * d * bound * 8, hipMemcpyDeviceToHost));
   segs.clear();
   for (int e = 0; e < d * d; e++) {
     if (!sg[e]) {

@@ -131,3 +131,18 @@ def test_tree_product_device_entry_point():
     assert L.rf_tree_product_device(ents, n, d, ctypes.c_void_p(out.ptr), short, sign, nlimbs) == rf.RF_ERR_CAPACITY
     src.free()
     out.free()
+
+
+def test_trees_above_and_below_the_kept_blocks():
+    # the library keeps a tree's device blocks for the next tree up to 512 MiB
+    # in all; a tree of two 2^30-bit leaves (768 MiB of blocks) frees them at
+    # its end.  Large, small, large again, each exact; the values are of the
+    # form 2^a - c so that the expected products take linear time to form
+    a, b = 1 << 30, (1 << 30) - 12345
+    x, y = (1 << a) - 1, -((1 << b) - 3)
+    big_want = [-((1 << (a + b)) - 3 * (1 << a) - (1 << b) + 3)]  # (2^a - 1)(-(2^b - 3)) expanded
+    small = [[3, -5, 7, 11], [2, 0, -1, 4], [-(1 << 700), 1, 1, 1 << 900]]
+    small_want = reduce(lambda X, Y: _matmul(X, Y, 2), small)
+    assert rf.tree_product([[x], [y]], 1) == big_want
+    assert rf.tree_product(small, 2) == small_want
+    assert rf.tree_product([[x], [y]], 1) == big_want
