@@ -39,7 +39,13 @@ HIPCC_FLAGS = [
 # Per-source additions: the CRT kernel for 22 and 24 primes unrolls its
 # mixed-radix conversion only above LLVM's default pragma-unroll size limit;
 # otherwise its Garner digits are indexed at run time and live in scratch.
-SOURCE_FLAGS = {"rf_crt.hip": ["-mllvm", "-pragma-unroll-threshold=200000"]}
+# The CRT kernels are scheduled for memory clauses (their coefficient loads
+# grouped): the same registers, no spills at any prime count, and k_crt<11>
+# 0.985 ms against 0.999-1.011 per 1024 products (alternating runs on one
+# MI355X).  The transform kernels keep the default schedule: max-ilp spills
+# k_fwd<13, 4> and memory clauses slow it by 2-3 %.
+SOURCE_FLAGS = {"rf_crt.hip": ["-mllvm", "-pragma-unroll-threshold=200000",
+                               "-mllvm", "-amdgpu-sched-strategy=max-memory-clause"]}
 
 
 def build(verbose=False):
