@@ -1746,8 +1746,11 @@ int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next) {
   return RF_OK;
 }
 
-// The whole tree; the root's entries (sign/magnitude, device) are copied to
-// out + e * out_cap, their signs and lengths to sign[e], nlimbs[e].  Every
+// The whole tree; the root's entries (sign/magnitude) are copied to
+// out + e * out_cap in device memory, or with out null to the staging buffer
+// (which then holds at least d * d * out_cap limbs; its address is taken
+// after the levels, which may have grown and moved it), their signs and
+// lengths to sign[e], nlimbs[e].  Every
 // level's block stays the tree's until the end: a carried matrix may point
 // into any earlier level.
 int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, int* sign, uint64_t* nlimbs) {
@@ -1757,6 +1760,8 @@ int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, i
     rc = tree_level(cur, d, next);
     if (!rc) cur.swap(next);
   }
+  const hipMemcpyKind kind = out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (!out) out = (uint64_t*)g.h_stage;
   for (int e = 0; e < d * d && !rc; e++) {
     const Entry& E = cur[0][e];
     if (E.sign && E.nlimbs > out_cap) rc = fail(RF_ERR_CAPACITY, "root entry %d needs %llu limbs", e,
@@ -1764,7 +1769,7 @@ int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, i
     sign[e] = E.sign;
     nlimbs[e] = E.sign ? E.nlimbs : 0;
     if (!rc && E.sign && E.nlimbs) {
-      hipError_t x = hipMemcpyAsync(out + (size_t)e * out_cap, E.dptr, E.nlimbs * 8, hipMemcpyDeviceToDevice, g.stream);
+      hipError_t x = hipMemcpyAsync(out + (size_t)e * out_cap, E.dptr, E.nlimbs * 8, kind, g.stream);
       if (x != hipSuccess) rc = fail(RF_ERR_HIP, "root copy: %s", hipGetErrorString(x));
     }
   }
@@ -1819,11 +1824,11 @@ int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d) {
   while ((1 << cl) < d) cl++;
   for (int m = 0; m < n; m++) bound += mx[m];
   bound += ((uint64_t)(n - 1) * cl + 63) / 64 + 1;
-  // the tree's blocks: its leaves, its root, then a block per level
+  // the tree's blocks: its leaves, then a block per level; the root's
+  // entries come back straight into the staging buffer
   g.tree_next = 0;
-  uint64_t *d_in = nullptr, *d_out = nullptr;
+  uint64_t* d_in = nullptr;
   int rc = tree_block(total * 8, &d_in);
-  if (!rc) rc = tree_block((size_t)d * d * bound * 8, &d_out);
   if (!rc) rc = ensure_stage(std::max<uint64_t>(total * 8, (size_t)d * d * bound * 8));
   if (rc) return rc;
   std::vector<Mat> cur(n, Mat(d * d));
@@ -1842,11 +1847,7 @@ int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d) {
   if (total) HIPCHK(hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream));
   std::vector<int> sg(d * d);
   std::vector<uint64_t> nls(d * d);
-  rc = tree_product(cur, d, d_out, bound, sg.data(), nls.data());
-  if (!rc) {
-    const hipError_t x = hipMemcpy(g.h_stage, d_out, (size_t)d * d * bound * 8, hipMemcpyDeviceToHost);
-    if (x != hipSuccess) rc = fail(RF_ERR_HIP, "tree download: %s", hipGetErrorString(x));
-  }
+  rc = tree_product(cur, d, nullptr, bound, sg.data(), nls.data());
   tree_blocks_trim();
   if (rc) return rc;
   segs.clear();

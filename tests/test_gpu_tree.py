@@ -60,8 +60,8 @@ def test_tree_level_of_headline_sized_leaves():
                                   "small_negative_in_long_buffer", "many_tiles_both_signs"])
 def test_tree_level_conversion_edges(case):
     # level outputs that stress the in-place conversion to sign and
-    # magnitude: the lowest nonzero limb past the first sweep of 256 limbs
-    # and in a later tile of 2048, a negative magnitude one limb long, a
+    # magnitude: the lowest nonzero limb several sweeps of 64 limbs up and in
+    # a later tile of 2048, a negative magnitude one limb long, a
     # small negative value under hundreds of all-ones limbs, and entries of
     # several tiles of either sign
     rng = random.Random(sum(map(ord, case)))
@@ -135,10 +135,11 @@ def test_tree_product_device_entry_point():
 
 def test_trees_above_and_below_the_kept_blocks():
     # the library keeps a tree's device blocks for the next tree up to 512 MiB
-    # in all; a tree of two 2^30-bit leaves (768 MiB of blocks) frees them at
-    # its end.  Large, small, large again, each exact; the values are of the
-    # form 2^a - c so that the expected products take linear time to form
-    a, b = 1 << 30, (1 << 30) - 12345
+    # in all; a tree of two leaves of 1.25 * 2^30 bits (320 MiB of leaves, 320
+    # MiB for the product) frees them at its end.  Large, small, large again,
+    # each exact; the values are of the form 2^a - c so that the expected
+    # products take linear time to form
+    a, b = (1 << 30) + (1 << 28), (1 << 30) + (1 << 28) - 12345
     x, y = (1 << a) - 1, -((1 << b) - 3)
     big_want = [-((1 << (a + b)) - 3 * (1 << a) - (1 << b) + 3)]  # (2^a - 1)(-(2^b - 3)) expanded
     small = [[3, -5, 7, 11], [2, 0, -1, 4], [-(1 << 700), 1, 1, 1 << 900]]
