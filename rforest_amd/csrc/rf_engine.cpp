@@ -11,6 +11,7 @@
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1668,21 +1669,33 @@ int tree_block(size_t bytes, uint64_t** out) {
 // A d x d matrix of sign/magnitude entries in device memory.
 using Mat = std::vector<Entry>;
 
-// One level of a product tree: pairs (M_2i, M_2i+1) multiplied in one
-// pipeline, outputs converted in place to sign and magnitude; an odd last
-// matrix is carried up.  The level's outputs go to the tree's next block.
-// One upload (the task image and the conversion tasks, through the staging
-// buffer) and one wait: the conversion kernels follow the pipeline on the
-// stream, and the signs and lengths come back with them.
-// Device arena: [task image | conversion tasks | metadata | pipeline scratch].
-int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next) {
-  const int np = (int)cur.size() / 2;
-  std::vector<Product> ps(np);
+// One level of a product tree, planned before the level below has run: pairs
+// (M_2i, M_2i+1) multiplied in one pipeline, outputs converted in place to
+// sign and magnitude (the next level's input format); an odd last matrix is
+// carried up.  A level's inputs are planned from their limb bounds (the
+// conversion leaves every output zero-padded to its bound) with a provisional
+// sign; the true signs reach the forward tasks on the device through sign
+// patches from the conversion's metadata, written before the level runs.
+struct TreeLevel {
+  std::vector<Product> ps;
+  Pipeline pp;
+  std::vector<RfConvTask> conv;
+  std::vector<RfSignPatch> patch;
+  uint32_t meta0 = 0;   // the level's first metadata slot (one per output entry)
+  uint64_t max_rn = 0;  // longest output, limbs
+};
+
+// src[m][e]: the metadata slot holding the sign of entry e of matrix m, or -1
+// (the sign is the host's).  Plans and builds the level over `cur` into L and
+// makes `cur`, `src` the level's outputs (plus a carried matrix).
+int tree_plan_level(std::vector<Mat>& cur, std::vector<std::vector<int32_t>>& src, int d, uint32_t& nmeta,
+                    TreeLevel& L) {
+  const int np = (int)cur.size() / 2, dd = d * d;
+  L.ps.resize(np);
   size_t total = 0;
-  uint64_t max_rn = 0;
   std::vector<size_t> base(np);
   for (int i = 0; i < np; i++) {
-    Product& P = ps[i];
+    Product& P = L.ps[i];
     P.r = P.d = d;
     P.A = cur[2 * i];
     P.B = cur[2 * i + 1];
@@ -1690,79 +1703,168 @@ int tree_level(const std::vector<Mat>& cur, int d, std::vector<Mat>& next) {
     if (rc) return rc;
     const uint64_t rn = P.plan.k ? P.plan.rn : 0;
     base[i] = total;
-    total += (size_t)d * d * rn;
-    max_rn = std::max(max_rn, rn);
+    total += (size_t)dd * rn;
+    L.max_rn = std::max(L.max_rn, rn);
   }
   uint64_t* buf = nullptr;
   int rc = tree_block(total * 8, &buf);
   if (rc) return rc;
-  const size_t nconv = (size_t)np * d * d;
-  std::vector<RfConvTask> conv;
-  conv.reserve(nconv);
+  L.meta0 = nmeta;
+  nmeta += (uint32_t)(np * dd);
   for (int i = 0; i < np; i++)
-    for (int e = 0; e < d * d; e++) {
-      const uint64_t rn = ps[i].plan.k ? ps[i].plan.rn : 0;
-      ps[i].out.push_back(rn ? buf + base[i] + (size_t)e * rn : nullptr);
-      conv.push_back(RfConvTask{ps[i].out.back(), rn});
+    for (int e = 0; e < dd; e++) {
+      const uint64_t rn = L.ps[i].plan.k ? L.ps[i].plan.rn : 0;
+      L.ps[i].out.push_back(rn ? buf + base[i] + (size_t)e * rn : nullptr);
+      L.conv.push_back(RfConvTask{L.ps[i].out.back(), rn});
     }
-  Pipeline pp;
-  rc = build_pipeline(ps, pp);
+  rc = build_pipeline(L.ps, L.pp);
   if (rc) return rc;
-  const size_t in_off = align256(pp.L.tasks_b);
-  const size_t conv_b = align256(nconv * sizeof(RfConvTask));
-  const size_t meta_b = align256(nconv * sizeof(RfEntryMeta));
-  rc = ensure_arena(in_off + conv_b + meta_b + pp.L.total());
-  if (rc) return rc;
-  rc = ensure_stage(in_off + conv_b + meta_b);
-  if (rc) return rc;
-  // earlier copies out of the staging buffer (the tree's leaves) must be done
-  HIPCHK(hipStreamSynchronize(g.stream));
-  par_memcpy(g.h_stage, pp.image.data(), pp.image.size());
-  memcpy(g.h_stage + in_off, conv.data(), nconv * sizeof(RfConvTask));
-  HIPCHK(hipMemcpyAsync(g.d_arena, g.h_stage, in_off + nconv * sizeof(RfConvTask), hipMemcpyHostToDevice,
-                        g.stream));
-  rc = launch_pipeline_tasks(pp, g.d_arena + in_off + conv_b + meta_b, g.d_arena, g.stream, g.prof);
-  if (rc) return rc;
-  RfConvTask* d_conv = (RfConvTask*)(g.d_arena + in_off);
-  RfEntryMeta* d_meta = (RfEntryMeta*)(g.d_arena + in_off + conv_b);
-  HIPCHK(rf_launch_signmag(d_conv, (int)nconv, d_meta, max_rn, g.stream));
-  const RfEntryMeta* meta = (const RfEntryMeta*)(g.h_stage + in_off + conv_b);
-  HIPCHK(hipMemcpyAsync((void*)meta, d_meta, nconv * sizeof(RfEntryMeta), hipMemcpyDeviceToHost, g.stream));
-  HIPCHK(hipStreamSynchronize(g.stream));
-  if (g.prof) prof_collect();
-  next.clear();
+  // forward tasks in build_pipeline's order: per group, product by product,
+  // A's entries then B's, prime by prime
+  std::map<int, size_t> goff, gcnt;
+  for (const LaunchRec& l : L.pp.launches) {
+    if (l.kind == K_FWD) goff[l.param] = l.off;
+    if (l.kind == K_FWD_COL) goff[(1 << 12) | l.param] = l.off;
+  }
+  L.patch.reserve((size_t)np * 2 * dd * (L.ps.empty() ? 0 : L.ps[0].plan.k));
   for (int i = 0; i < np; i++) {
-    Mat M(d * d);
-    for (int e = 0; e < d * d; e++) {
-      const RfEntryMeta& m = meta[i * d * d + e];
-      M[e].dptr = ps[i].out[e];
-      M[e].nlimbs = m.nlimbs;
-      M[e].sign = m.sign;
-      M[e].bits = m.sign ? 64 * (uint64_t)m.nlimbs : 0;
+    const RfPlan& pl = L.ps[i].plan;
+    if (!pl.k) continue;
+    const int key = fwd_key(pl);
+    size_t& c = gcnt[key];
+    const size_t g0 = goff[key] + offsetof(RfFwdTask, sign);
+    for (int e = 0; e < 2 * dd; e++) {
+      const int32_t sv = e < dd ? src[2 * i][e] : src[2 * i + 1][e - dd];
+      if (sv < 0) {
+        c += pl.k;
+        continue;
+      }
+      for (int q = 0; q < pl.k; q++, c++) L.patch.push_back(RfSignPatch{g0 + c * sizeof(RfFwdTask), (uint32_t)sv, 0});
+    }
+  }
+  // an output entry, a sum of d products, has at most bits(A) + bits(B) +
+  // ceil(log2 d) bits: the bound the next level plans with (its buffer of rn
+  // limbs holds the magnitude zero-padded)
+  int cl = 0;
+  while ((1 << cl) < d) cl++;
+  std::vector<Mat> next;
+  std::vector<std::vector<int32_t>> nsrc;
+  for (int i = 0; i < np; i++) {
+    Mat M(dd);
+    std::vector<int32_t> sv(dd, -1);
+    const uint64_t rn = L.ps[i].plan.k ? L.ps[i].plan.rn : 0;
+    uint64_t la = 0, lb = 0;
+    for (auto& x : L.ps[i].A) la = std::max(la, x.sign ? x.bits : 0);
+    for (auto& x : L.ps[i].B) lb = std::max(lb, x.sign ? x.bits : 0);
+    const uint64_t bits = rn ? std::min<uint64_t>(la + lb + cl, 64 * rn) : 0;
+    for (int e = 0; e < dd; e++) {
+      M[e].dptr = L.ps[i].out[e];
+      M[e].nlimbs = (bits + 63) / 64;
+      M[e].sign = rn ? 1 : 0;  // provisional: patched on the device
+      M[e].bits = bits;
+      if (rn) sv[e] = (int32_t)(L.meta0 + i * dd + e);
     }
     next.push_back(M);
+    nsrc.push_back(sv);
   }
-  if (cur.size() % 2) next.push_back(cur.back());
+  if (cur.size() % 2) {
+    next.push_back(cur.back());
+    nsrc.push_back(src.back());
+  }
+  cur.swap(next);
+  src.swap(nsrc);
   return RF_OK;
 }
 
-// The whole tree; the root's entries (sign/magnitude) are copied to
+// The whole tree, level after level with no wait between them: the host
+// plans, builds and stages level l + 1 while the GPU runs level l.  Device
+// arena: [pipeline scratch, reused by the levels in turn | per level: task
+// image, conversion tasks, sign patches]; the table slices (and their
+// staging slices) follow one another until a level does not fit, which waits
+// for the levels before it and starts again at the front.  stage_busy: bytes
+// at the front of the staging buffer that an earlier copy may still read
+// (the tree's leaves).  The root's entries (sign/magnitude) are copied to
 // out + e * out_cap in device memory, or with out null to the staging buffer
 // (which then holds at least d * d * out_cap limbs; its address is taken
 // after the levels, which may have grown and moved it), their signs and
-// lengths to sign[e], nlimbs[e].  Every
-// level's block stays the tree's until the end: a carried matrix may point
-// into any earlier level.
-int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, int* sign, uint64_t* nlimbs) {
-  int rc = RF_OK;
-  while (cur.size() > 1 && !rc) {
-    std::vector<Mat> next;
-    rc = tree_level(cur, d, next);
-    if (!rc) cur.swap(next);
+// lengths to sign[e], nlimbs[e].
+// The largest scratch and table totals of the trees so far: a tree makes
+// room for them before its first level, so that one of the same shape runs
+// every level without a wait.
+size_t g_tree_scratch = 0, g_tree_tables = 0;
+
+int tree_product(std::vector<Mat> cur, int d, uint64_t* out, uint64_t out_cap, int* sign, uint64_t* nlimbs,
+                 size_t stage_busy = 0) {
+  const int dd = d * d;
+  std::vector<std::vector<int32_t>> src(cur.size(), std::vector<int32_t>(dd, -1));
+  // metadata slots: one per output entry of every level
+  size_t nslots = 0;
+  for (size_t m = cur.size(); m > 1; m = (m + 1) / 2) nslots += (m / 2) * (size_t)dd;
+  RfEntryMeta* d_meta = nullptr;
+  int rc = tree_block(nslots * sizeof(RfEntryMeta), (uint64_t**)&d_meta);
+  if (rc) return rc;
+  uint32_t nmeta = 0, root_meta = 0;
+  const size_t busy = align256(stage_busy);
+  if (cur.size() > 1) {
+    rc = ensure_arena(g_tree_scratch + g_tree_tables);
+    if (!rc) rc = ensure_stage(busy + g_tree_tables);
+    if (rc) return rc;
+  }
+  size_t S = g_tree_scratch, tab = 0, stg = busy;  // scratch bytes; table and staging cursors
+  size_t tables = 0;                                // this tree's table bytes
+  bool levels = false;
+  while (cur.size() > 1) {
+    TreeLevel L;
+    rc = tree_plan_level(cur, src, d, nmeta, L);
+    if (rc) return rc;
+    const size_t img_b = align256(L.pp.image.size());
+    const size_t conv_b = align256(L.conv.size() * sizeof(RfConvTask));
+    const size_t slice = img_b + conv_b + align256(L.patch.size() * sizeof(RfSignPatch));
+    if (L.pp.L.total() > S || S + tab + slice > g.arena_cap || stg + slice > g.stage_cap) {
+      // the levels before are done: their tables, the scratch and the staging are free
+      HIPCHK(hipStreamSynchronize(g.stream));
+      S = std::max(S, L.pp.L.total());
+      tab = 0;
+      stg = 0;
+      rc = ensure_arena(S + slice);
+      if (!rc) rc = ensure_stage(slice);
+      if (rc) return rc;
+    }
+    char* dt = g.d_arena + S + tab;
+    char* ht = g.h_stage + stg;
+    par_memcpy(ht, L.pp.image.data(), L.pp.image.size());
+    memcpy(ht + img_b, L.conv.data(), L.conv.size() * sizeof(RfConvTask));
+    if (!L.patch.empty()) memcpy(ht + img_b + conv_b, L.patch.data(), L.patch.size() * sizeof(RfSignPatch));
+    HIPCHK(hipMemcpyAsync(dt, ht, slice, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(rf_launch_sign_patch(dt, (const RfSignPatch*)(dt + img_b + conv_b), (int)L.patch.size(), d_meta,
+                                g.stream));
+    rc = launch_pipeline_tasks(L.pp, g.d_arena, dt, g.stream, g.prof);
+    if (rc) return rc;
+    HIPCHK(rf_launch_signmag((const RfConvTask*)(dt + img_b), (int)L.conv.size(), d_meta + L.meta0, L.max_rn,
+                             g.stream));
+    tab += slice;
+    stg += slice;
+    tables += slice;
+    root_meta = L.meta0;
+    levels = true;
+  }
+  g_tree_scratch = std::max(g_tree_scratch, S);
+  g_tree_tables = std::max(g_tree_tables, tables);
+  if (levels) {
+    // the root is the last level's output; the copy follows every upload on the stream
+    HIPCHK(hipMemcpyAsync(g.h_stage, d_meta + root_meta, (size_t)dd * sizeof(RfEntryMeta), hipMemcpyDeviceToHost,
+                          g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (g.prof) prof_collect();
+    const RfEntryMeta* m = (const RfEntryMeta*)g.h_stage;
+    for (int e = 0; e < dd; e++) {
+      cur[0][e].sign = m[e].sign;
+      cur[0][e].nlimbs = m[e].nlimbs;
+    }
   }
   const hipMemcpyKind kind = out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (!out) out = (uint64_t*)g.h_stage;
-  for (int e = 0; e < d * d && !rc; e++) {
+  for (int e = 0; e < dd && !rc; e++) {
     const Entry& E = cur[0][e];
     if (E.sign && E.nlimbs > out_cap) rc = fail(RF_ERR_CAPACITY, "root entry %d needs %llu limbs", e,
                                                 (unsigned long long)E.nlimbs);
@@ -1847,7 +1949,7 @@ int rf_mpz_tree_product(mpz_t* C, mpz_t* mats, int n, int d) {
   if (total) HIPCHK(hipMemcpyAsync(d_in, g.h_stage, total * 8, hipMemcpyHostToDevice, g.stream));
   std::vector<int> sg(d * d);
   std::vector<uint64_t> nls(d * d);
-  rc = tree_product(cur, d, nullptr, bound, sg.data(), nls.data());
+  rc = tree_product(cur, d, nullptr, bound, sg.data(), nls.data(), total * 8);
   tree_blocks_trim();
   if (rc) return rc;
   segs.clear();

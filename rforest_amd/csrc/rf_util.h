@@ -64,6 +64,16 @@ struct RfEntryMeta {
   uint32_t pad;
 };
 #define RF_FLIP_LIMBS 2048
+// A sign written on the device into an uploaded task image: the int32 at
+// byte `off` takes meta[src].sign (a product tree's forward tasks take the
+// signs of the level below, known only once its conversion has run).
+struct RfSignPatch {
+  uint64_t off;
+  uint32_t src;
+  uint32_t pad;
+};
+hipError_t rf_launch_sign_patch(char* image, const RfSignPatch* patches, int n, const RfEntryMeta* meta,
+                                hipStream_t st);
 hipError_t rf_launch_signmag(const RfConvTask* tasks, int ntasks, RfEntryMeta* meta, uint64_t max_n, hipStream_t st);
 
 // Remainder fix-up, one task per remainder X mod D (D of n limbs, top limb

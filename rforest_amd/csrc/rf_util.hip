@@ -384,7 +384,20 @@ __global__ void k_fill(uint64_t* __restrict__ dst, uint64_t n, uint64_t v) {
   if (i < n) dst[i] = v;
 }
 
+__global__ __launch_bounds__(256) void k_sign_patch(char* __restrict__ img, const RfSignPatch* __restrict__ pt,
+                                                      int n, const RfEntryMeta* __restrict__ meta) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) *(int32_t*)(img + pt[i].off) = meta[pt[i].src].sign;
+}
+
 }  // namespace
+
+hipError_t rf_launch_sign_patch(char* image, const RfSignPatch* patches, int n, const RfEntryMeta* meta,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sign_patch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, image, patches, n, meta);
+  return hipGetLastError();
+}
 
 hipError_t rf_launch_gen(const RfSpan* spans, const RfBlock* blocks, int nblocks, hipStream_t st) {
   if (nblocks <= 0) return hipSuccess;
